@@ -1,0 +1,172 @@
+"""Benchmark of the Heat3ds linear-solve path on one MI355X.
+
+A step is one pass of the hot path: calRHS! + PBiCGSTAB! with the 5-sweep
+red-black Gauss-Seidel preconditioner to ε = 1e-4 (208 iterations), FP64,
+on the modelA stack at 240 x 240 x 30 interior cells (the reference's
+README/benchmark case), with all fields resident in HBM.  θ is reset to the
+initial state before every step (untimed).
+
+Printed JSON line (one, on stdout):
+  value      algorithmic HBM GB/s of the whole solve:
+             560 B/cell/iteration (FP64) x cells x iterations / step time
+  roofline   dominant kernel (one red-black GS colour sweep): algorithmic
+             bytes per launch / mean launch time (HIP events, context stream)
+  cpu_baseline  the oracle (OpenMP C restatement of the reference
+             algorithm) on a bounded sample of the same workload
+  fp32_refined  the same case solved to a true relative residual of 1e-8
+             with FP64 and with FP32-inner/FP64-refinement, and the speed-up
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_ITER = {"fp64": 560.0, "fp32": 280.0}
+SWEEP_BYTES_PER_CELL = 32.0  # x, λ, rhs read + x written, FP64, one full red-black sweep
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=240)
+    ap.add_argument("--nz", type=int, default=30)
+    ap.add_argument("--eps", type=float, default=1e-4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-refined", action="store_true", help="skip the FP32-refinement comparison")
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(nx, nz, iters):
+    """Oracle PBiCGSTAB+GS on the same case for a bounded number of
+    iterations, all host cores; returns GB/s in the same algorithmic unit."""
+    from oracle import oracle as O
+    O.build()
+    exe = O.EXE
+    cores = min(os.cpu_count() or 1, 16)
+    cmd = [exe, str(nx), str(nx), str(nz), "pbicgstab", "gs", "1e-30", "1", str(cores)]
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    # cap the iteration count through a tiny solve budget: run with maxItr via env
+    env["ORC_MAXITR"] = str(iters)
+    t0 = time.perf_counter()
+    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env, check=True).stdout
+    wall = time.perf_counter() - t0
+    solve_s = None
+    itr = None
+    for line in out.splitlines():
+        if line.startswith("geometry_s="):
+            solve_s = float(line.split()[1].split("=")[1])
+        if line.startswith("converged="):
+            itr = int(line.split()[1].split("=")[1])
+    cells = nx * nx * nz
+    gbs = BYTES_PER_CELL_ITER["fp64"] * cells * itr / solve_s / 1e9
+    return {"value": round(gbs, 4), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": "oracle (OpenMP C restatement of the reference algorithm) PBiCGSTAB+GS FP64 at %dx%dx%d, "
+                      "first %d iterations, %.1f s solve (%.1f s wall incl. geometry)" % (nx, nx, nz, itr, solve_s, wall),
+            "s_per_iteration": round(solve_s / itr, 4)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
+              "multi-GPU (Y-slab) decomposition is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import h2_amd
+    from h2_amd import _lib as L
+    from h2_amd.heat3ds import WorkBuffers, load_model
+
+    nx, nz = args.nx, args.nz
+    log("building model %dx%dx%d" % (nx, nx, nz))
+    m = h2_amd.build_model(nx, nx, nz)
+    wk = WorkBuffers(*m.shape)
+    load_model(wk, m)
+    th0 = wk.download("theta")
+    dt = 10000.0
+    cells = nx * nx * nz
+
+    def step(precision=L.FP64, tol=args.eps):
+        wk.upload("theta", th0)
+        wk.fill("b", 0.0)
+        t0 = time.perf_counter()
+        wk.calc_rhs(dt, True)
+        conv, itr, res0 = wk.solve(L.SOLVER_PBICGSTAB, dt, tol, 20000, L.SMOOTHER_GS, True, precision)
+        t1 = time.perf_counter()
+        return t1 - t0, wk.last_solve_ms(), conv, itr, res0
+
+    for w in range(args.warmup):
+        r = step()
+        log("warmup %d: %.3f ms, itr %d" % (w, r[0] * 1e3, r[3]))
+    times, dev_ms, itrs = [], [], []
+    for s in range(args.steps):
+        r = step()
+        times.append(r[0])
+        dev_ms.append(r[1])
+        itrs.append(r[3])
+        log("step %d: %.3f ms wall, %.3f ms device, itr %d, converged %s" % (s, r[0] * 1e3, r[1], r[3], r[2]))
+    total = sum(times)
+    ms_per_step = total / args.steps * 1e3
+    itr = itrs[-1]
+    value = BYTES_PER_CELL_ITER["fp64"] * cells * itr / (ms_per_step * 1e-3) / 1e9
+
+    # dominant kernel: one red-black colour sweep (k_rbsor), HIP events on the context stream
+    sweep_ms = wk.time_kernel(0, 200, dt, True)  # both colours
+    half_ms = sweep_ms / 2.0
+    per_launch_bytes = SWEEP_BYTES_PER_CELL / 2.0 * cells
+    achieved = per_launch_bytes / (half_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("k_rbsor_240x240x30")
+    roofline = {"bound": "hbm", "kernel": "k_rbsor<double> (one red-black colour)", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "launch_ms": round(half_ms, 5),
+                "algorithmic_bytes_per_launch": per_launch_bytes}
+
+    refined = None
+    if not args.no_refined:
+        r64 = step(L.FP64, 1e-8)
+        refined = {"tol": 1e-8, "fp64_ms": round(r64[0] * 1e3, 3), "fp64_itr": r64[3]}
+        try:
+            r32 = step(L.FP32_REFINED, 1e-8)
+            refined.update({"fp32_refined_ms": round(r32[0] * 1e3, 3), "fp32_refined_inner_itr": r32[3],
+                            "speedup": round(r64[0] / r32[0], 3), "fp32_refined_converged": r32[2],
+                            "outer_residuals": [float("%.3e" % v) for v in wk.history()]})
+        except L.H3DError as e:
+            refined["fp32_refined_error"] = str(e)
+    wk.close()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(nx, nz, args.cpu_iters)
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"error": str(e)}
+
+    line = {"metric": "solve wall-time & HBM GB/s, 240²×30; FP32+refine vs FP64 at 1e-8 rel resid",
+            "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (modelA geometry built on the host)",
+            "config": {"workload": "modelA %dx%dx%d steady PBiCGSTAB+GS(5) eps=%g, calRHS!+solve"
+                                   % (nx, nx, nz, args.eps), "cells": cells, "iterations": itr,
+                       "device_solve_ms": round(sum(dev_ms) / len(dev_ms), 3)},
+            "roofline": roofline, "cpu_baseline": cpu, "fp32_refined": refined}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

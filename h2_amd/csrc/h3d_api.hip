@@ -1,0 +1,787 @@
+// C ABI of libh3d (declared in include/h3d.h): context, field transfer,
+// boundary-condition set-up, RHS, and the PBiCGSTAB!/CG! drivers.
+//
+// The Krylov scalars (ρ, α, ω, β, residuals, iteration count, status) live
+// in device memory (KState) and are updated by the last workgroup of each
+// reducing kernel, so an iteration is a fixed sequence of kernel launches
+// with no host round trip.  The host enqueues iterations in batches and
+// reads the status word once per batch; kernels enqueued after the solve has
+// stopped see status != running and return at once.
+
+#include "h3d.h"
+#include "h3d_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace h3d;
+
+struct h3d_ctx {
+    int MX = 0, MY = 0, MZ = 0, NX = 0, NY = 0, NZ = 0;
+    size_t H = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *f[H3D_NFIELDS] = {};  // f[H3D_F_MASK] stays null: the mask is implicit
+    double *d_ZC = nullptr, *d_dZ = nullptr, *d_ap0 = nullptr;
+    float *d_ZC32 = nullptr, *d_dZ32 = nullptr, *d_ap032 = nullptr;
+    double dx = 0, dy = 0;
+    bool grid_set = false;
+    int bc_type[6] = {H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX};
+    double bc_T[6] = {}, bc_q[6] = {}, bc_h[6] = {}, bc_ta[6] = {};
+    double HC[6] = {};
+    bool bc_set = false;
+    KState *d_state = nullptr;   // solver state
+    KState *d_unit = nullptr;    // state for single-operator calls (status stays 0)
+    KState *h_state = nullptr;   // pinned mirror
+    double *d_hist = nullptr;
+    int hist_cap = 0;
+    std::vector<double> hist;
+    RedBuf rb{};
+    // FP32 shadows for the refinement inner solve (allocated on first use)
+    float *g32[10] = {};         // lam, rhs, x(=d), p, p_, r, r0, q, s, s_ ... see F32_* below
+    float *g32_t = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int verbose = 0;             // H3D_VERBOSE=1: refinement diagnostics on stderr
+    std::string err;
+};
+
+namespace {
+
+enum { F32_LAM = 0, F32_B, F32_X, F32_P, F32_P_, F32_R, F32_R0, F32_Q, F32_S, F32_S_ };
+
+std::string g_create_err;
+
+int fail(h3d_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, ...)                                                                              \
+    do {                                                                                              \
+        hipError_t e_ = (__VA_ARGS__);                                                                \
+        if (e_ != hipSuccess)                                                                         \
+            return fail((ctx), H3D_ERR_HIP, std::string(#__VA_ARGS__) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+bool field_ok(int f) { return f >= 0 && f < H3D_NFIELDS; }
+
+template <typename T>
+Geo<T> base_geo(const h3d_ctx *c)
+{
+    Geo<T> g{};
+    g.MX = c->MX; g.MY = c->MY; g.MZ = c->MZ;
+    g.NX = c->NX; g.NY = c->NY; g.NZ = c->NZ;
+    g.dx0 = (T)c->dx; g.dy0 = (T)c->dy;
+    // ddx = inv(dx0), ddy = inv(dy0) (NonUniform.jl:472-473)
+    g.ddx = (T)(1.0 / c->dx); g.ddy = (T)(1.0 / c->dy);
+    if (sizeof(T) == 4) { g.ddx = (T)1 / g.dx0; g.ddy = (T)1 / g.dy0; }
+    for (int f = 0; f < 6; ++f) g.HC[f] = (T)c->HC[f];
+    return g;
+}
+
+// Operator constants for FP64; computes ap0 for a transient operator.
+int make_geo64(h3d_ctx *c, double dt, int is_steady, Geo<double> *out)
+{
+    if (!c->grid_set) return fail(c, H3D_ERR_STATE, "h3d_set_grid has not been called");
+    Geo<double> g = base_geo<double>(c);
+    g.ZC = c->d_ZC; g.dZ = c->d_dZ;
+    g.ap0 = nullptr;
+    if (!is_steady) {
+        if (!(dt > 0.0) && !(dt < 0.0)) return fail(c, H3D_ERR_ARG, "transient operator needs dt != 0");
+        if (!c->d_ap0) HIPCHK(c, hipMalloc(&c->d_ap0, c->H * sizeof(double)));
+        HIPCHK(c, hipMemsetAsync(c->d_ap0, 0, c->H * sizeof(double), c->stream));
+        const double ddt = 1.0 / dt;  // ddt = inv(Δt)
+        HIPCHK(c, launch_ap0(c->d_ap0, c->f[H3D_F_RHO], c->f[H3D_F_CP], g, ddt, c->stream));
+        g.ap0 = c->d_ap0;
+    }
+    *out = g;
+    return H3D_OK;
+}
+
+int ensure_hist(h3d_ctx *c, int n)
+{
+    if (n <= c->hist_cap) return H3D_OK;
+    if (c->d_hist) (void)hipFree(c->d_hist);
+    c->d_hist = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_hist, (size_t)n * sizeof(double)));
+    c->hist_cap = n;
+    return H3D_OK;
+}
+
+int reset_state(h3d_ctx *c, double tol, int maxItr)
+{
+    KState s{};
+    s.tol = tol;
+    s.maxItr = maxItr;
+    s.status = ST_RUNNING;
+    s.rho_old = 1.0; s.alpha = 0.0; s.omega = 1.0; s.beta = 0.0;  // NonUniform.jl:276-280
+    // the stream is idle here (every entry point ends synchronised), so the
+    // pinned mirror can stage the initial state
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *c->h_state = s;
+    HIPCHK(c, hipMemcpyAsync(c->d_state, c->h_state, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int read_state(h3d_ctx *c, KState *out)
+{
+    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = *c->h_state;
+    return H3D_OK;
+}
+
+// Preconditioner! (NonUniform.jl:531-559) on the device: :gs = 5 sweeps of
+// colour 0 then colour 1, :none = mycopy!.  xx is expected zeroed (callers do
+// myfill!(xx, 0) first, as the reference does).
+template <typename T>
+hipError_t precond(T *xx, const T *bb, const T *lam, const Geo<T> &g, int smoother, const KState *st, size_t H,
+                   hipStream_t s)
+{
+    if (smoother == H3D_SMOOTHER_NONE) {
+        return hipMemcpyAsync(xx, bb, H * sizeof(T), hipMemcpyDeviceToDevice, s);
+    }
+    hipError_t e = hipSuccess;
+    for (int sweep = 0; sweep < 5 && e == hipSuccess; ++sweep) {
+        e = launch_rbsor<T>(xx, bb, lam, g, 0, st, s);
+        if (e == hipSuccess) e = launch_rbsor<T>(xx, bb, lam, g, 1, st, s);
+    }
+    return e;
+}
+
+// Buffers of one Krylov solve in type T.
+template <typename T>
+struct KVecs {
+    T *th, *b, *lam, *p, *p_, *r, *r0, *q, *s, *s_, *t;
+};
+
+// Enqueue one PBiCGSTAB! iteration (NonUniform.jl:287-347).  ρ of the
+// iteration was formed by the previous iteration's update kernel (or by
+// CalcRK! for iteration 1) and its FIN_BICG_*RHO step.
+template <typename T>
+hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int smoother)
+{
+    hipStream_t s = c->stream;
+    KState *st = c->d_state;
+    hipError_t e;
+    if ((e = launch_bicg_p<T>(v.p, v.r, v.q, g, st, s)) != hipSuccess) return e;
+    if (smoother == H3D_SMOOTHER_GS) {
+        if ((e = launch_fill<T>(v.p_, T(0), c->H, s)) != hipSuccess) return e;
+    }
+    if ((e = precond<T>(v.p_, v.p, v.lam, g, smoother, st, c->H, s)) != hipSuccess) return e;
+    if ((e = launch_calc_ax<T>(v.q, v.p_, v.lam, g, v.r0, nullptr, FIN_BICG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
+        return e;
+    if ((e = launch_bicg_s<T>(v.s, v.q, v.r, g, st, s)) != hipSuccess) return e;
+    if (smoother == H3D_SMOOTHER_GS) {
+        if ((e = launch_fill<T>(v.s_, T(0), c->H, s)) != hipSuccess) return e;
+    }
+    if ((e = precond<T>(v.s_, v.s, v.lam, g, smoother, st, c->H, s)) != hipSuccess) return e;
+    if ((e = launch_calc_ax<T>(v.t, v.s_, v.lam, g, nullptr, v.s, FIN_BICG_OMEGA, st, nullptr, c->rb, s)) != hipSuccess)
+        return e;
+    return launch_bicg_update<T>(v.th, v.r, v.p_, v.s_, v.t, v.s, v.r0, g, st, c->d_hist, c->rb, s);
+}
+
+// Enqueue one CG! iteration (NonUniform.jl:160-223).
+template <typename T>
+hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int smoother)
+{
+    hipStream_t s = c->stream;
+    KState *st = c->d_state;
+    hipError_t e;
+    if ((e = launch_calc_ax<T>(v.q, v.p, v.lam, g, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
+        return e;
+    if ((e = launch_cg_update<T>(v.th, v.r, v.p, v.q, g, st, c->d_hist, c->rb, s)) != hipSuccess) return e;
+    if (smoother == H3D_SMOOTHER_GS) {
+        if ((e = launch_fill<T>(v.s, T(0), c->H, s)) != hipSuccess) return e;
+    }
+    if ((e = precond<T>(v.s, v.r, v.lam, g, smoother, st, c->H, s)) != hipSuccess) return e;
+    if ((e = launch_dot<T>(v.r, v.s, nullptr, g, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
+    return launch_cg_p<T>(v.p, v.s, g, st, s);
+}
+
+// Run a Krylov solve whose vectors are in v, starting from the current θ.
+// Returns status via *fin_state.
+template <typename T>
+int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Geo<T> &g, double tol, int maxItr, int smoother,
+           KState *fin_state)
+{
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = ensure_hist(c, std::max(maxItr, 1))) != H3D_OK) return rc;
+    if ((rc = reset_state(c, tol, maxItr)) != H3D_OK) return rc;
+    // myfill!(q, 0); res0 = CalcRK!(r, θ, b, ...); PBiCGSTAB also copies r0 = r
+    HIPCHK(c, launch_fill<T>(v.q, T(0), c->H, s));
+    const bool bicg = (solver == H3D_SOLVER_PBICGSTAB);
+    HIPCHK(c, launch_calc_rk<T>(v.r, bicg ? v.r0 : nullptr, v.th, v.b, v.lam, g, bicg ? FIN_RES0_BICG : FIN_RES0,
+                                c->d_state, nullptr, c->rb, s));
+    if (solver == H3D_SOLVER_CG) {
+        // z = M⁻¹ r; p = z; rho_old = (r, z)   (NonUniform.jl:147-155)
+        if (smoother == H3D_SMOOTHER_GS) HIPCHK(c, launch_fill<T>(v.s, T(0), c->H, s));
+        HIPCHK(c, precond<T>(v.s, v.r, v.lam, g, smoother, c->d_state, c->H, s));
+        HIPCHK(c, launch_copy_interior<T>(v.p, v.s, g, c->d_state, s));
+        HIPCHK(c, launch_dot<T>(v.r, v.s, nullptr, g, FIN_CG_RHO0, c->d_state, nullptr, c->rb, s));
+    }
+    // batches of iterations; the status word is read once per batch
+    int launched = 0, batch = 1;
+    KState st{};
+    while (true) {
+        const int nb = std::min(batch, maxItr - launched);
+        for (int it = 0; it < nb; ++it) {
+            hipError_t e = (solver == H3D_SOLVER_CG) ? enqueue_cg_iter<T>(c, v, g, smoother)
+                                                     : enqueue_bicg_iter<T>(c, v, g, smoother);
+            if (e != hipSuccess) return fail(c, H3D_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        }
+        launched += nb;
+        if ((rc = read_state(c, &st)) != H3D_OK) return rc;
+        if (st.status != ST_RUNNING || launched >= maxItr) break;
+        batch = std::min(batch * 2, 16);
+    }
+    *fin_state = st;
+    return H3D_OK;
+}
+
+template <typename T>
+KVecs<T> fp64_vecs(h3d_ctx *c)
+{
+    KVecs<T> v;
+    v.th = c->f[H3D_F_THETA]; v.b = c->f[H3D_F_B]; v.lam = c->f[H3D_F_LAMBDA];
+    v.p = c->f[H3D_F_PCG_P]; v.p_ = c->f[H3D_F_PCG_P_]; v.r = c->f[H3D_F_PCG_R];
+    v.r0 = c->f[H3D_F_PCG_R0]; v.q = c->f[H3D_F_PCG_Q]; v.s = c->f[H3D_F_PCG_S];
+    v.s_ = c->f[H3D_F_PCG_S_]; v.t = c->f[H3D_F_PCG_T_];
+    return v;
+}
+
+int finish_solve(h3d_ctx *c, const KState &st, int *isconverged, int *itr, double *res0)
+{
+    c->hist.assign((size_t)std::max(st.itr, 0), 0.0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st.itr > 0)
+        HIPCHK(c, hipMemcpy(c->hist.data(), c->d_hist, (size_t)st.itr * sizeof(double), hipMemcpyDeviceToHost));
+    if (isconverged) *isconverged = (st.status == ST_CONVERGED) ? 1 : 0;
+    if (itr) *itr = st.itr;
+    if (res0) *res0 = st.res0;
+    if (st.status == ST_NAN) return fail(c, H3D_ERR_NAN, "NaN or Inf residual during the solve");
+    return H3D_OK;
+}
+
+int solve_fp64(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
+               int *isconverged, int *itr, double *res0)
+{
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    KState st{};
+    rc = krylov<double>(c, solver, fp64_vecs<double>(c), g, tol, maxItr, smoother, &st);
+    if (rc != H3D_OK) return rc;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return finish_solve(c, st, isconverged, itr, res0);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 inner solves in FP64 iterative refinement
+//   r = b - Aθ (FP64 CalcRK!), stop when ||r||/res0 < tol;
+//   solve A d = r in FP32 with the same Krylov method to inner_tol;
+//   θ += d (FP64).
+// ---------------------------------------------------------------------------
+int ensure_fp32(h3d_ctx *c)
+{
+    for (int a = 0; a < 10; ++a)
+        if (!c->g32[a]) HIPCHK(c, hipMalloc(&c->g32[a], c->H * sizeof(float)));
+    if (!c->g32_t) HIPCHK(c, hipMalloc(&c->g32_t, c->H * sizeof(float)));
+    if (!c->d_ZC32) HIPCHK(c, hipMalloc(&c->d_ZC32, c->MZ * sizeof(float)));
+    if (!c->d_dZ32) HIPCHK(c, hipMalloc(&c->d_dZ32, c->MZ * sizeof(float)));
+    return H3D_OK;
+}
+
+int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
+                  double inner_tol, int *isconverged, int *itr, double *res0_out)
+{
+    Geo<double> g64;
+    int rc = make_geo64(c, dt, is_steady, &g64);
+    if (rc != H3D_OK) return rc;
+    if ((rc = ensure_fp32(c)) != H3D_OK) return rc;
+    hipStream_t s = c->stream;
+    if (!(inner_tol > 0.0)) inner_tol = 1e-4;
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    // FP32 operator: λ, ZC, ΔZ, ap0 rounded to float
+    HIPCHK(c, launch_convert<float, double>(c->g32[F32_LAM], c->f[H3D_F_LAMBDA], c->H, s));
+    HIPCHK(c, launch_convert<float, double>(c->d_ZC32, c->d_ZC, (size_t)c->MZ, s));
+    HIPCHK(c, launch_convert<float, double>(c->d_dZ32, c->d_dZ, (size_t)c->MZ, s));
+    Geo<float> g32 = base_geo<float>(c);
+    g32.ZC = c->d_ZC32; g32.dZ = c->d_dZ32; g32.ap0 = nullptr;
+    if (!is_steady) {
+        if (!c->d_ap032) HIPCHK(c, hipMalloc(&c->d_ap032, c->H * sizeof(float)));
+        HIPCHK(c, launch_convert<float, double>(c->d_ap032, c->d_ap0, c->H, s));
+        g32.ap0 = c->d_ap032;
+    }
+    KVecs<float> v;
+    v.th = c->g32[F32_X]; v.b = c->g32[F32_B]; v.lam = c->g32[F32_LAM];
+    v.p = c->g32[F32_P]; v.p_ = c->g32[F32_P_]; v.r = c->g32[F32_R]; v.r0 = c->g32[F32_R0];
+    v.q = c->g32[F32_Q]; v.s = c->g32[F32_S]; v.s_ = c->g32[F32_S_]; v.t = c->g32_t;
+    // Krylov halos must be zero
+    for (int a = F32_X; a <= F32_S_; ++a) HIPCHK(c, launch_fill<float>(c->g32[a], 0.f, c->H, s));
+    HIPCHK(c, launch_fill<float>(c->g32_t, 0.f, c->H, s));
+    HIPCHK(c, launch_fill<float>(c->g32[F32_B], 0.f, c->H, s));
+
+    double *r64 = c->f[H3D_F_PCG_R];
+    std::vector<double> outer;
+    int total_inner = 0;
+    double res0 = 0.0;
+    bool converged = false, nan = false;
+    const int max_outer = 100;
+    for (int outer_it = 0; outer_it <= max_outer; ++outer_it) {
+        // true FP64 residual r = b - Aθ
+        HIPCHK(c, launch_calc_rk<double>(r64, nullptr, c->f[H3D_F_THETA], c->f[H3D_F_B], c->f[H3D_F_LAMBDA], g64,
+                                         FIN_STORE, c->d_unit, nullptr, c->rb, s));
+        KState u{};
+        HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        u = *c->h_state;
+        const double rn = std::sqrt(u.out[0]);
+        if (outer_it == 0) {
+            res0 = rn;
+            if (res0 == 0.0) { converged = true; break; }
+        } else {
+            outer.push_back(rn / res0);
+        }
+        if (!std::isfinite(rn)) { nan = true; break; }
+        if (outer_it > 0 && rn / res0 < tol) { converged = true; break; }
+        if (outer_it == max_outer || total_inner >= maxItr) break;
+        // inner FP32 solve of A d = r from d = 0
+        HIPCHK(c, launch_convert<float, double>(v.b, r64, c->H, s));
+        HIPCHK(c, launch_fill<float>(v.th, 0.f, c->H, s));
+        KState st{};
+        rc = krylov<float>(c, solver, v, g32, inner_tol, std::max(1, maxItr - total_inner), smoother, &st);
+        if (rc != H3D_OK) return rc;
+        total_inner += st.itr;
+        if (c->verbose) {
+            std::fprintf(stderr, "h3d refine: outer %d true res %.6e, inner itr %d status %d inner res %.3e\n",
+                         outer_it, rn / res0, st.itr, st.status, st.res);
+            if (c->verbose > 1 && st.itr > 0) {
+                std::vector<double> h((size_t)st.itr);
+                HIPCHK(c, hipMemcpy(h.data(), c->d_hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+                for (int k = 0; k < st.itr; k += std::max(1, st.itr / 40))
+                    std::fprintf(stderr, "   inner %5d %.6e\n", k + 1, h[(size_t)k]);
+            }
+        }
+        if (st.status == ST_NAN) { nan = true; break; }
+        HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], v.th, g64, s));
+        if (st.status == ST_BREAKDOWN && st.itr <= 1) break;  // no progress possible
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, s));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->hist = outer;
+    if (isconverged) *isconverged = converged ? 1 : 0;
+    if (itr) *itr = total_inner;
+    if (res0_out) *res0_out = res0;
+    if (nan) return fail(c, H3D_ERR_NAN, "NaN or Inf residual during the refined solve");
+    return H3D_OK;
+}
+
+int solve(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady, int precision,
+          double inner_tol, int *isconverged, int *itr, double *res0)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (smoother != H3D_SMOOTHER_NONE && smoother != H3D_SMOOTHER_GS)
+        return fail(c, H3D_ERR_ARG, "Unsupported smoother. Use none (0) or gs (1)");
+    if (maxItr < 1) return fail(c, H3D_ERR_ARG, "maxItr must be >= 1");
+    if (precision == H3D_FP64)
+        return solve_fp64(c, solver, dt, tol, maxItr, smoother, is_steady, isconverged, itr, res0);
+    if (precision == H3D_FP32_REFINED)
+        return solve_refined(c, solver, dt, tol, maxItr, smoother, is_steady, inner_tol, isconverged, itr, res0);
+    return fail(c, H3D_ERR_ARG, "unknown precision");
+}
+
+int unit_reduce_result(h3d_ctx *c, double *out, int n)
+{
+    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < n; ++a) out[a] = c->h_state->out[a];
+    return H3D_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *h3d_create_error(void) { return g_create_err.c_str(); }
+
+int h3d_create(h3d_ctx **out, int MX, int MY, int MZ, int device)
+{
+    if (!out) return H3D_ERR_ARG;
+    *out = nullptr;
+    if (MX < 3 || MY < 3 || MZ < 3) {
+        g_create_err = "MX, MY, MZ must be >= 3 (one interior cell plus the halo)";
+        return H3D_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        g_create_err = "no such HIP device";
+        return H3D_ERR_HIP;
+    }
+    h3d_ctx *c = new h3d_ctx();
+    c->MX = MX; c->MY = MY; c->MZ = MZ;
+    c->NX = MX - 2; c->NY = MY - 2; c->NZ = MZ - 2;
+    c->H = (size_t)MX * MY * MZ;
+    c->device = device;
+    if (const char *v = std::getenv("H3D_VERBOSE")) c->verbose = std::atoi(v);
+    auto bail = [&](const char *what, hipError_t e) {
+        g_create_err = std::string(what) + ": " + hipGetErrorString(e);
+        h3d_destroy(c);
+        return H3D_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail("hipSetDevice", e);
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
+    for (int f = 0; f < H3D_NFIELDS; ++f) {
+        if (f == H3D_F_MASK) continue;
+        if ((e = hipMalloc(&c->f[f], c->H * sizeof(double))) != hipSuccess) return bail("hipMalloc field", e);
+        if ((e = launch_fill<double>(c->f[f], 0.0, c->H, c->stream)) != hipSuccess) return bail("fill", e);
+    }
+    // λ = cp = 1 initially (common.jl:77-78)
+    if ((e = launch_fill<double>(c->f[H3D_F_LAMBDA], 1.0, c->H, c->stream)) != hipSuccess) return bail("fill", e);
+    if ((e = launch_fill<double>(c->f[H3D_F_CP], 1.0, c->H, c->stream)) != hipSuccess) return bail("fill", e);
+    if ((e = hipMalloc(&c->d_ZC, MZ * sizeof(double))) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = hipMalloc(&c->d_dZ, MZ * sizeof(double))) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = hipMalloc(&c->d_state, sizeof(KState))) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = hipMalloc(&c->d_unit, sizeof(KState))) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = hipMemsetAsync(c->d_state, 0, sizeof(KState), c->stream)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipMemsetAsync(c->d_unit, 0, sizeof(KState), c->stream)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipHostMalloc(&c->h_state, sizeof(KState), hipHostMallocDefault)) != hipSuccess) return bail("hipHostMalloc", e);
+    // reduction scratch: enough partials for any interior launch shape
+    const long maxb = ((long)(c->NX + 63) / 64) * ((c->NY + 3) / 4) * (long)c->NZ + 64;
+    c->rb.nblocks_max = (int)std::min<long>(maxb, 1L << 30);
+    if ((e = hipMalloc(&c->rb.partials, (size_t)c->rb.nblocks_max * 6 * sizeof(double))) != hipSuccess)
+        return bail("hipMalloc partials", e);
+    if ((e = hipMalloc(&c->rb.counter, sizeof(unsigned int))) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = hipMemsetAsync(c->rb.counter, 0, sizeof(unsigned int), c->stream)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail("event", e);
+    if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("sync", e);
+    *out = c;
+    return H3D_OK;
+}
+
+int h3d_destroy(h3d_ctx *c)
+{
+    if (!c) return H3D_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int f = 0; f < H3D_NFIELDS; ++f)
+        if (c->f[f]) (void)hipFree(c->f[f]);
+    for (int a = 0; a < 10; ++a)
+        if (c->g32[a]) (void)hipFree(c->g32[a]);
+    void *ps[] = {c->g32_t, c->d_ZC, c->d_dZ, c->d_ap0, c->d_ZC32, c->d_dZ32, c->d_ap032,
+                  c->d_state, c->d_unit, c->d_hist, c->rb.partials, c->rb.counter};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (c->h_state) (void)hipHostFree(c->h_state);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return H3D_OK;
+}
+
+const char *h3d_last_error(const h3d_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int h3d_set_grid(h3d_ctx *c, double dx, double dy, const double *ZC, const double *dZ, int mz)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!ZC || !dZ || mz != c->MZ) return fail(c, H3D_ERR_ARG, "ZC and dZ must have MZ entries");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fail(c, H3D_ERR_ARG, "dx, dy must be > 0");
+    c->dx = dx; c->dy = dy;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_ZC, ZC, mz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_dZ, dZ, mz * sizeof(double), hipMemcpyHostToDevice));
+    c->grid_set = true;
+    return H3D_OK;
+}
+
+int h3d_set_bc(h3d_ctx *c, const int type[6], const double temperature[6], const double heat_flux[6],
+               const double htc[6], const double ambient[6])
+{
+    if (!c || !type || !temperature || !heat_flux || !htc || !ambient) return c ? fail(c, H3D_ERR_ARG, "null bc array") : H3D_ERR_ARG;
+    for (int f = 0; f < 6; ++f)
+        if (type[f] < H3D_ISOTHERMAL || type[f] > H3D_CONVECTION) return fail(c, H3D_ERR_ARG, "bad boundary type");
+    for (int f = 0; f < 6; ++f) {
+        c->bc_type[f] = type[f];
+        c->bc_T[f] = temperature[f];
+        c->bc_q[f] = heat_flux[f];
+        c->bc_h[f] = htc[f];
+        c->bc_ta[f] = ambient[f];
+        // set_BC_coef (boundary_conditions.jl:360-394)
+        c->HC[f] = (type[f] == H3D_CONVECTION) ? htc[f] : 0.0;
+    }
+    c->bc_set = true;
+    return H3D_OK;
+}
+
+int h3d_get_hc(const h3d_ctx *c, double HC[6])
+{
+    if (!c || !HC) return H3D_ERR_ARG;
+    for (int f = 0; f < 6; ++f) HC[f] = c->HC[f];
+    return H3D_OK;
+}
+
+int h3d_upload(h3d_ctx *c, int field, const double *host, size_t n)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(field) || !host || n != c->H) return fail(c, H3D_ERR_ARG, "bad field id, pointer or size");
+    if (field == H3D_F_MASK) {
+        for (int k = 0; k < c->MZ; ++k)
+            for (int j = 0; j < c->MY; ++j)
+                for (int i = 0; i < c->MX; ++i) {
+                    const bool inner = i > 0 && i < c->MX - 1 && j > 0 && j < c->MY - 1 && k > 0 && k < c->MZ - 1;
+                    const double m = host[(size_t)i + (size_t)c->MX * ((size_t)j + (size_t)c->MY * k)];
+                    if (m != (inner ? 1.0 : 0.0))
+                        return fail(c, H3D_ERR_UNSUPPORTED,
+                                    "mask must be 0 on the halo and 1 inside (the post-boundary-condition mask)");
+                }
+        return H3D_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_download(h3d_ctx *c, int field, double *host, size_t n)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(field) || !host || n != c->H) return fail(c, H3D_ERR_ARG, "bad field id, pointer or size");
+    if (field == H3D_F_MASK) {
+        for (int k = 0; k < c->MZ; ++k)
+            for (int j = 0; j < c->MY; ++j)
+                for (int i = 0; i < c->MX; ++i) {
+                    const bool inner = i > 0 && i < c->MX - 1 && j > 0 && j < c->MY - 1 && k > 0 && k < c->MZ - 1;
+                    host[(size_t)i + (size_t)c->MX * ((size_t)j + (size_t)c->MY * k)] = inner ? 1.0 : 0.0;
+                }
+        return H3D_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(host, c->f[field], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_fill(h3d_ctx *c, int field, double value)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(field)) return fail(c, H3D_ERR_ARG, "bad field id");
+    if (field == H3D_F_MASK) return fail(c, H3D_ERR_UNSUPPORTED, "the mask is implicit");
+    HIPCHK(c, launch_fill<double>(c->f[field], value, c->H, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_copy(h3d_ctx *c, int dst, int src)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(dst) || !field_ok(src) || dst == H3D_F_MASK || src == H3D_F_MASK)
+        return fail(c, H3D_ERR_ARG, "bad field id");
+    HIPCHK(c, hipMemcpyAsync(c->f[dst], c->f[src], c->H * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_apply_bc(h3d_ctx *c)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!c->bc_set) return fail(c, H3D_ERR_STATE, "h3d_set_bc has not been called");
+    for (int f = 0; f < 6; ++f)
+        HIPCHK(c, launch_apply_bc_face(f, c->bc_type[f], c->bc_T[f], c->bc_ta[f], c->f[H3D_F_THETA],
+                                       c->f[H3D_F_LAMBDA], c->f[H3D_F_RHO], c->f[H3D_F_CP], c->MX, c->MY, c->MZ,
+                                       c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_calc_rhs(h3d_ctx *c, double dt, int is_steady)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!c->bc_set) return fail(c, H3D_ERR_STATE, "h3d_set_bc has not been called");
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    // apply_bc2RHS! face order x-,x+,y-,y+,z-,z+ (RHS.jl:30-48)
+    for (int f = 0; f < 6; ++f) {
+        const int t = c->bc_type[f];
+        if (t != H3D_HEAT_FLUX && t != H3D_CONVECTION) continue;
+        HIPCHK(c, launch_bc2rhs_face(f, t, c->bc_q[f], c->bc_h[f], c->bc_ta[f], c->f[H3D_F_B], c->f[H3D_F_THETA], g,
+                                     c->stream));
+    }
+    HIPCHK(c, launch_rhs_final(c->f[H3D_F_B], c->f[H3D_F_THETA], c->f[H3D_F_HSRC], g, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_pbicgstab(h3d_ctx *c, double dt, double tol, int maxItr, int smoother, int is_steady, int precision,
+                  double inner_tol, int *isconverged, int *itr, double *res0)
+{
+    return solve(c, H3D_SOLVER_PBICGSTAB, dt, tol, maxItr, smoother, is_steady, precision, inner_tol, isconverged,
+                 itr, res0);
+}
+
+int h3d_cg(h3d_ctx *c, double dt, double tol, int maxItr, int smoother, int is_steady, int precision,
+           double inner_tol, int *isconverged, int *itr, double *res0)
+{
+    return solve(c, H3D_SOLVER_CG, dt, tol, maxItr, smoother, is_steady, precision, inner_tol, isconverged, itr,
+                 res0);
+}
+
+int h3d_get_history(const h3d_ctx *c, double *res, int cap, int *n)
+{
+    if (!c || !n) return H3D_ERR_ARG;
+    *n = (int)c->hist.size();
+    if (res)
+        for (int a = 0; a < std::min(cap, *n); ++a) res[a] = c->hist[a];
+    return H3D_OK;
+}
+
+int h3d_last_solve_ms(const h3d_ctx *c, double *ms)
+{
+    if (!c || !ms) return H3D_ERR_ARG;
+    *ms = c->last_ms;
+    return H3D_OK;
+}
+
+int h3d_calc_ax(h3d_ctx *c, int ax, int x, double dt, int is_steady)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(ax) || !field_ok(x) || ax == H3D_F_MASK || x == H3D_F_MASK) return fail(c, H3D_ERR_ARG, "bad field");
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    HIPCHK(c, launch_calc_ax<double>(c->f[ax], c->f[x], c->f[H3D_F_LAMBDA], g, nullptr, nullptr, -1, c->d_unit, nullptr,
+                                     c->rb, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_calc_rk(h3d_ctx *c, int r, int theta, int b, double dt, int is_steady, double *res)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(r) || !field_ok(theta) || !field_ok(b) || r == H3D_F_MASK || theta == H3D_F_MASK || b == H3D_F_MASK)
+        return fail(c, H3D_ERR_ARG, "bad field");
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    HIPCHK(c, launch_calc_rk<double>(c->f[r], nullptr, c->f[theta], c->f[b], c->f[H3D_F_LAMBDA], g, FIN_STORE, c->d_unit,
+                                     nullptr, c->rb, c->stream));
+    double s = 0;
+    if ((rc = unit_reduce_result(c, &s, 1)) != H3D_OK) return rc;
+    if (res) *res = std::sqrt(s);
+    return H3D_OK;
+}
+
+int h3d_rbsor_color(h3d_ctx *c, int x, int b, int color, double dt, int is_steady)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(x) || !field_ok(b) || x == H3D_F_MASK || b == H3D_F_MASK || (color != 0 && color != 1))
+        return fail(c, H3D_ERR_ARG, "bad field or colour");
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    HIPCHK(c, launch_rbsor<double>(c->f[x], c->f[b], c->f[H3D_F_LAMBDA], g, color, c->d_unit, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_precond(h3d_ctx *c, int xx, int bb, int smoother, double dt, int is_steady)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(xx) || !field_ok(bb) || xx == H3D_F_MASK || bb == H3D_F_MASK) return fail(c, H3D_ERR_ARG, "bad field");
+    if (smoother != H3D_SMOOTHER_NONE && smoother != H3D_SMOOTHER_GS)
+        return fail(c, H3D_ERR_ARG, "Unsupported smoother. Use none (0) or gs (1)");
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    HIPCHK(c, precond<double>(c->f[xx], c->f[bb], c->f[H3D_F_LAMBDA], g, smoother, c->d_unit, c->H, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_fdot1(h3d_ctx *c, int x, double *out)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(x) || x == H3D_F_MASK || !out) return fail(c, H3D_ERR_ARG, "bad field");
+    Geo<double> g = base_geo<double>(c);
+    HIPCHK(c, launch_dot<double>(c->f[x], nullptr, nullptr, g, FIN_STORE, c->d_unit, nullptr, c->rb, c->stream));
+    return unit_reduce_result(c, out, 1);
+}
+
+int h3d_fdot2(h3d_ctx *c, int x, int y, double *out)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(x) || !field_ok(y) || x == H3D_F_MASK || y == H3D_F_MASK || !out)
+        return fail(c, H3D_ERR_ARG, "bad field");
+    Geo<double> g = base_geo<double>(c);
+    HIPCHK(c, launch_dot<double>(c->f[x], c->f[y], nullptr, g, FIN_STORE, c->d_unit, nullptr, c->rb, c->stream));
+    return unit_reduce_result(c, out, 1);
+}
+
+static int axpby_call(h3d_ctx *c, int op, int z, int x, int y, double a, double b)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(z) || !field_ok(x) || !field_ok(y) || z == H3D_F_MASK || x == H3D_F_MASK || y == H3D_F_MASK)
+        return fail(c, H3D_ERR_ARG, "bad field");
+    Geo<double> g = base_geo<double>(c);
+    HIPCHK(c, launch_axpby<double>(op, c->f[z], c->f[x], c->f[y], nullptr, a, b, g, c->d_unit, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return H3D_OK;
+}
+
+int h3d_bicg1(h3d_ctx *c, int p, int r, int q, double beta, double omg)
+{
+    return axpby_call(c, AX_BICG1, p, r, q, beta, omg);
+}
+int h3d_triad(h3d_ctx *c, int z, int x, int y, double a) { return axpby_call(c, AX_TRIAD, z, x, y, a, 0.0); }
+int h3d_bicg2(h3d_ctx *c, int z, int x, int y, double a, double b) { return axpby_call(c, AX_BICG2, z, x, y, a, b); }
+
+int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, double *ms)
+{
+    if (!c || !ms || reps < 1) return c ? fail(c, H3D_ERR_ARG, "bad arguments") : H3D_ERR_ARG;
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    double *x = c->f[H3D_F_PCG_P_], *b = c->f[H3D_F_PCG_P], *q = c->f[H3D_F_PCG_Q], *lam = c->f[H3D_F_LAMBDA];
+    auto once = [&]() -> hipError_t {
+        if (which == 0) {
+            hipError_t e = launch_rbsor<double>(x, b, lam, g, 0, c->d_unit, c->stream);
+            return e != hipSuccess ? e : launch_rbsor<double>(x, b, lam, g, 1, c->d_unit, c->stream);
+        }
+        if (which == 1)
+            return launch_calc_ax<double>(q, x, lam, g, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
+        return launch_calc_ax<double>(q, x, lam, g, c->f[H3D_F_PCG_R0], nullptr, FIN_STORE, c->d_unit, nullptr, c->rb,
+                                      c->stream);
+    };
+    if (which < 0 || which > 2) return fail(c, H3D_ERR_ARG, "unknown kernel id");
+    HIPCHK(c, once());
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(c, once());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = (double)t / reps;
+    return H3D_OK;
+}
+
+}  // extern "C"

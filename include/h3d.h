@@ -1,0 +1,161 @@
+/*
+ * libh3d — MI355X (gfx950) native linear-solve path of Heat3ds.
+ *
+ * C ABI: plain pointers and sizes, every function returns an int status
+ * (H3D_OK == 0, negative on error; h3d_last_error() gives the message).
+ * Nothing here ever exits or aborts.  A context owns device-resident copies
+ * of the reference's WorkBuffers arrays and runs all work on its own HIP
+ * stream.  One context per host thread; a context is not re-entrant.
+ *
+ * Array layout everywhere is the reference's Julia Array{Float64,3} of size
+ * (MX,MY,MZ) = (NX+2,NY+2,NZ+2), column-major, i fastest, with a one-cell
+ * halo: 0-based offset = i + MX*(j + MY*k).  A Julia caller passes
+ * pointer(A) directly.
+ *
+ * Reference interfaces replaced (paths are into the kenoogl/H2 tree):
+ *   WorkBuffers            src/common.jl:49-89        -> h3d_create/upload/download
+ *   myfill!/mycopy!        src/common.jl:95-113       -> h3d_fill/h3d_copy
+ *   apply_boundary_conditions!  src/boundary_conditions.jl:94-119 -> h3d_apply_bc
+ *   set_BC_coef            src/boundary_conditions.jl:360-394 -> h3d_set_bc
+ *   calRHS!/apply_bc2RHS!  src/NonUniform.jl:57-91, src/RHS.jl:23-192 -> h3d_calc_rhs
+ *   PBiCGSTAB!             src/NonUniform.jl:248-350  -> h3d_pbicgstab
+ *   CG!                    src/NonUniform.jl:115-226  -> h3d_cg
+ *   CalcAX!                src/NonUniform.jl:454-511  -> h3d_calc_ax
+ *   CalcRK!                src/NonUniform.jl:372-434  -> h3d_calc_rk
+ *   Preconditioner!        src/NonUniform.jl:531-559  -> h3d_precond
+ *   rbsor_core!            src/NonUniform.jl:848-915  -> h3d_rbsor_color
+ *   Fdot1/Fdot2            src/NonUniform.jl:567-592  -> h3d_fdot1/h3d_fdot2
+ *   BiCG1!/Triad!/BICG2!   src/NonUniform.jl:603-659  -> h3d_bicg1/h3d_triad/h3d_bicg2
+ */
+#ifndef H3D_H
+#define H3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define H3D_OK 0
+#define H3D_ERR_ARG (-1)         /* bad argument (size, field id, enum) */
+#define H3D_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define H3D_ERR_NAN (-3)         /* NaN/Inf residual during a solve */
+#define H3D_ERR_UNSUPPORTED (-4) /* input the device path does not support */
+#define H3D_ERR_STATE (-5)       /* call order (e.g. solve before set_grid) */
+
+/* boundary types: src/common.jl:20-24 */
+#define H3D_ISOTHERMAL 0
+#define H3D_HEAT_FLUX 1
+#define H3D_CONVECTION 2
+
+#define H3D_SOLVER_PBICGSTAB 0
+#define H3D_SOLVER_CG 1
+#define H3D_SMOOTHER_NONE 0 /* Preconditioner! :none (copy) */
+#define H3D_SMOOTHER_GS 1   /* Preconditioner! :gs (5 red-black sweeps) */
+
+/* arithmetic of the Krylov iteration */
+#define H3D_FP64 0          /* FP64 throughout (the reference) */
+#define H3D_FP32_REFINED 1  /* FP32 inner PBiCGSTAB/CG inside FP64 refinement */
+
+/* fields: the WorkBuffers members, src/common.jl:49-65, in that order */
+#define H3D_F_THETA 0
+#define H3D_F_B 1
+#define H3D_F_MASK 2
+#define H3D_F_RHO 3
+#define H3D_F_LAMBDA 4
+#define H3D_F_CP 5
+#define H3D_F_PCG_P 6
+#define H3D_F_PCG_P_ 7
+#define H3D_F_PCG_R 8
+#define H3D_F_PCG_R0 9
+#define H3D_F_PCG_Q 10
+#define H3D_F_PCG_S 11
+#define H3D_F_PCG_S_ 12
+#define H3D_F_PCG_T_ 13
+#define H3D_F_HSRC 14
+#define H3D_NFIELDS 15
+
+typedef struct h3d_ctx h3d_ctx;
+
+/* WorkBuffers(mx,my,mz) (src/common.jl:71-89): all fields zero except
+ * mask = lambda = cp = 1.  device = HIP device ordinal. */
+int h3d_create(h3d_ctx **ctx, int MX, int MY, int MZ, int device);
+int h3d_destroy(h3d_ctx *ctx);
+const char *h3d_last_error(const h3d_ctx *ctx);
+/* message of the last h3d_create failure (no context exists then) */
+const char *h3d_create_error(void);
+
+/* Δh[1], Δh[2] and the Z metrics ZC, ΔZ (both MZ entries, genZ! layout,
+ * src/Zcoord.jl:133-166). */
+int h3d_set_grid(h3d_ctx *ctx, double dx, double dy, const double *ZC, const double *dZ, int mz);
+
+/* The six faces x-,x+,y-,y+,z-,z+ (BoundaryConditionSet,
+ * src/boundary_conditions.jl:24-31).  Also forms HC[6] as set_BC_coef does. */
+int h3d_set_bc(h3d_ctx *ctx, const int type[6], const double temperature[6],
+               const double heat_flux[6], const double htc[6], const double ambient[6]);
+int h3d_get_hc(const h3d_ctx *ctx, double HC[6]);
+
+/* Host <-> device copies of a whole field (n must equal MX*MY*MZ).
+ * The mask is not stored: the device path requires the mask the
+ * reference's apply_boundary_conditions! leaves (0 on every halo cell, 1
+ * inside); uploading any other mask returns H3D_ERR_UNSUPPORTED.  Before
+ * h3d_apply_bc the reference mask is 1 everywhere; the operator kernels
+ * always use the post-boundary-condition mask. */
+int h3d_upload(h3d_ctx *ctx, int field, const double *host, size_t n);
+int h3d_download(h3d_ctx *ctx, int field, double *host, size_t n);
+int h3d_fill(h3d_ctx *ctx, int field, double value);           /* myfill! */
+int h3d_copy(h3d_ctx *ctx, int dst, int src);                   /* mycopy! */
+
+/* apply_boundary_conditions! on the device copies of θ, λ, ρ, cp. */
+int h3d_apply_bc(h3d_ctx *ctx);
+
+/* calRHS!(wk, Δh, Δt, ΔZ, bc_set, qsrf, par; is_steady): adds the heat-flux
+ * and convection face terms into b, then b = -(ap0*θ + hsrc*dx*dy*dz + b).
+ * Like the reference it does not clear b first (h3d_fill(ctx,H3D_F_B,0)). */
+int h3d_calc_rhs(h3d_ctx *ctx, double dt, int is_steady);
+
+/* PBiCGSTAB!/CG!(wk, Δh, Δt, ZC, ΔZ, HC; tol, maxItr, smoother, is_steady).
+ * Outputs mirror the reference's (isconverged, itr, res0) tuple.  Non-
+ * convergence and Krylov breakdown are not errors (isconverged = 0);
+ * a NaN/Inf residual returns H3D_ERR_NAN.  precision = H3D_FP64 runs the
+ * reference iteration; H3D_FP32_REFINED runs FP32 inner solves (relative
+ * tolerance inner_tol, 1e-4 if <= 0) inside an FP64 refinement loop that
+ * stops when the true FP64 ||b - A θ|| / res0 < tol; then *itr counts
+ * inner iterations and the history holds the outer true residuals. */
+int h3d_pbicgstab(h3d_ctx *ctx, double dt, double tol, int maxItr, int smoother,
+                  int is_steady, int precision, double inner_tol,
+                  int *isconverged, int *itr, double *res0);
+int h3d_cg(h3d_ctx *ctx, double dt, double tol, int maxItr, int smoother,
+           int is_steady, int precision, double inner_tol,
+           int *isconverged, int *itr, double *res0);
+
+/* Relative residuals of the last solve (what the reference prints as
+ * "%10d %24.14E" per iteration).  *n = number available. */
+int h3d_get_history(const h3d_ctx *ctx, double *res, int cap, int *n);
+/* Device time of the last h3d_pbicgstab/h3d_cg call (HIP events on the
+ * context stream, initial residual to final residual). */
+int h3d_last_solve_ms(const h3d_ctx *ctx, double *ms);
+
+/* Single operators on fields, for unit tests and custom drivers. */
+int h3d_calc_ax(h3d_ctx *ctx, int ax, int x, double dt, int is_steady);
+int h3d_calc_rk(h3d_ctx *ctx, int r, int theta, int b, double dt, int is_steady, double *res);
+int h3d_rbsor_color(h3d_ctx *ctx, int x, int b, int color, double dt, int is_steady);
+int h3d_precond(h3d_ctx *ctx, int xx, int bb, int smoother, double dt, int is_steady);
+int h3d_fdot1(h3d_ctx *ctx, int x, double *out);
+int h3d_fdot2(h3d_ctx *ctx, int x, int y, double *out);
+int h3d_bicg1(h3d_ctx *ctx, int p, int r, int q, double beta, double omg);
+int h3d_triad(h3d_ctx *ctx, int z, int x, int y, double a);
+int h3d_bicg2(h3d_ctx *ctx, int z, int x, int y, double a, double b);
+
+/* Timing of one kernel family on the context stream (HIP events around
+ * reps back-to-back launches after one warm-up launch; *ms = mean per
+ * launch).  which: 0 = one full red-black GS sweep (both colours),
+ * 1 = CalcAX!, 2 = CalcAX! with its fused dot. */
+int h3d_time_kernel(h3d_ctx *ctx, int which, int reps, double dt, int is_steady, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -33,6 +33,107 @@ int orc_get_threads(void)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Grid sums.  Mode 0 (default) is the reference's: a running FP64 sum in    */
+/* sequential (k,j,i) order.  Mode 1 returns the exactly rounded sum of the  */
+/* same rounded terms (round to nearest even), computed with a fixed-point   */
+/* accumulator wide enough for every finite double: each term is           */
+/* mant * 2^e with e >= -1074; bit position e + 1088 is split over int64     */
+/* limbs that each carry 32 bits, so up to 2^31 terms add without overflow. */
+/* The result does not depend on the order of the terms; it is what the     */
+/* device library's order-independent sums round to.                        */
+/* ------------------------------------------------------------------------ */
+static int g_sum_mode = 0;
+void orc_set_sum_mode(int mode) { g_sum_mode = mode; }
+int orc_get_sum_mode(void) { return g_sum_mode; }
+
+#define XS_LIMBS 72
+#define XS_BIAS 1088
+typedef struct {
+    int64_t limb[XS_LIMBS];
+    double nonfinite;   /* sum of non-finite terms (NaN/Inf propagate) */
+    int has_nonfinite;
+} xsum_t;
+
+static void xsum_init(xsum_t *a) { memset(a, 0, sizeof(*a)); }
+
+static void xsum_add(xsum_t *a, double t)
+{
+    if (!isfinite(t)) {
+        a->nonfinite += t;
+        a->has_nonfinite = 1;
+        return;
+    }
+    if (t == 0.0) return;
+    uint64_t bits;
+    memcpy(&bits, &t, sizeof bits);
+    const int neg = (int)(bits >> 63);
+    const int E = (int)((bits >> 52) & 0x7FF);
+    uint64_t mant = bits & ((1ull << 52) - 1);
+    int e2;
+    if (E == 0) e2 = -1074;
+    else { mant |= 1ull << 52; e2 = E - 1075; }
+    const int p = e2 + XS_BIAS;
+    const int L = p >> 5, sh = p & 31;
+    const unsigned __int128 v = (unsigned __int128)mant << sh;
+    const int64_t c0 = (int64_t)(uint64_t)(v & 0xFFFFFFFFu);
+    const int64_t c1 = (int64_t)(uint64_t)((v >> 32) & 0xFFFFFFFFu);
+    const int64_t c2 = (int64_t)(uint64_t)(v >> 64);
+    if (neg) { a->limb[L] -= c0; a->limb[L + 1] -= c1; a->limb[L + 2] -= c2; }
+    else { a->limb[L] += c0; a->limb[L + 1] += c1; a->limb[L + 2] += c2; }
+}
+
+static void xsum_carry(int64_t *l)
+{
+    for (int i = 0; i < XS_LIMBS - 1; ++i) {
+        const int64_t c = l[i] >> 32;   /* floor division by 2^32 */
+        l[i] -= c * ((int64_t)1 << 32);
+        l[i + 1] += c;
+    }
+}
+
+static double xsum_round(const xsum_t *a)
+{
+    if (a->has_nonfinite) return a->nonfinite;
+    int64_t l[XS_LIMBS];
+    memcpy(l, a->limb, sizeof l);
+    xsum_carry(l);
+    int neg = 0;
+    if (l[XS_LIMBS - 1] < 0) {
+        neg = 1;
+        for (int i = 0; i < XS_LIMBS; ++i) l[i] = -l[i];
+        xsum_carry(l);
+    }
+    int H = XS_LIMBS - 1;
+    while (H >= 0 && l[H] == 0) --H;
+    if (H < 0) return 0.0;
+    /* 96-bit window from limbs H, H-1, H-2; sticky from the rest */
+    unsigned __int128 top = (unsigned __int128)(uint64_t)l[H] << 64;
+    if (H >= 1) top |= (unsigned __int128)(uint64_t)l[H - 1] << 32;
+    if (H >= 2) top |= (unsigned __int128)(uint64_t)l[H - 2];
+    int sticky = 0;
+    for (int i = 0; i < H - 2; ++i)
+        if (l[i]) sticky = 1;
+    int nb = 0;
+    for (unsigned __int128 t = top; t; t >>= 1) ++nb;
+    const int base = 32 * (H - 2) - XS_BIAS;   /* weight of bit 0 of top */
+    uint64_t m;
+    int ex;
+    if (nb <= 53) {
+        m = (uint64_t)top;
+        ex = base;
+    } else {
+        const int shift = nb - 53;
+        m = (uint64_t)(top >> shift);
+        const int rbit = (int)((top >> (shift - 1)) & 1);
+        if (shift > 1 && (top & ((((unsigned __int128)1) << (shift - 1)) - 1))) sticky = 1;
+        if (rbit && (sticky || (m & 1))) m += 1;
+        ex = base + shift;
+    }
+    const double r = ldexp((double)m, ex);
+    return neg ? -r : r;
+}
+
+/* ------------------------------------------------------------------------ */
 /* constants: src/common.jl:10-16, src/modelA.jl:10-131                      */
 /* ------------------------------------------------------------------------ */
 static const double FloatMin = 1.0e-37;     /* common.jl:12 */
@@ -627,17 +728,39 @@ void orc_calc_ax(double *ax, const double *x, const orc_work_t *wk, const orc_gr
             }
 }
 
-/* sum of squares of the interior in sequential (k,j,i) order */
-static double seq_sum_sq(const double *r, int MX, int MY, int MZ)
+/* Σ x·y (y == NULL: Σ x²) over the interior: sequential (k,j,i) order in
+ * mode 0 (FLoops SequentialEx @reduce), exactly rounded in mode 1. */
+static double grid_sum(const double *x, const double *y, int MX, int MY, int MZ)
 {
+    if (g_sum_mode == 1) {
+        xsum_t acc;
+        xsum_init(&acc);
+        for (int k = 2; k <= MZ - 1; ++k)
+            for (int j = 2; j <= MY - 1; ++j)
+                for (int i = 2; i <= MX - 1; ++i) {
+                    const size_t c = I3(i, j, k);
+                    xsum_add(&acc, x[c] * (y ? y[c] : x[c]));
+                }
+        return xsum_round(&acc);
+    }
     double res = 0.0;
     for (int k = 2; k <= MZ - 1; ++k)
         for (int j = 2; j <= MY - 1; ++j)
             for (int i = 2; i <= MX - 1; ++i) {
-                double v = r[I3(i, j, k)];
-                res = res + v * v;
+                const size_t c = I3(i, j, k);
+                res = res + x[c] * (y ? y[c] : x[c]);
             }
     return res;
+}
+static double seq_sum_sq(const double *r, int MX, int MY, int MZ) { return grid_sum(r, NULL, MX, MY, MZ); }
+
+/* exact-mode sum of an arbitrary term list, exported for tests */
+double orc_exact_sum(const double *t, size_t n)
+{
+    xsum_t acc;
+    xsum_init(&acc);
+    for (size_t i = 0; i < n; ++i) xsum_add(&acc, t[i]);
+    return xsum_round(&acc);
 }
 
 /* CalcRK! (NonUniform.jl:372-434): r = (b - Aθ)*m, returns sqrt(Σ r²) */
@@ -712,14 +835,7 @@ double orc_fdot1(const double *x, int MX, int MY, int MZ)
 }
 double orc_fdot2(const double *x, const double *y, int MX, int MY, int MZ)
 {
-    double sum = 0.0;
-    for (int k = 2; k <= MZ - 1; ++k)
-        for (int j = 2; j <= MY - 1; ++j)
-            for (int i = 2; i <= MX - 1; ++i) {
-                size_t c = I3(i, j, k);
-                sum = sum + x[c] * y[c];
-            }
-    return sum;
+    return grid_sum(x, y, MX, MY, MZ);
 }
 
 #define INTERIOR_LOOP(body)                                              \

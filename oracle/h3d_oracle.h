@@ -104,6 +104,12 @@ int orc_cg(orc_work_t *wk, const orc_grid_t *g, double tol, int maxItr,
            int smoother, int verbose, int *itr, double *res0,
            double *hist, int cap);                         /* NonUniform.jl:115-226 */
 
+/* Grid-sum mode: 0 = the reference's sequential (k,j,i) running sum
+ * (default), 1 = exactly rounded sum of the same terms (order-independent). */
+void orc_set_sum_mode(int mode);
+int orc_get_sum_mode(void);
+double orc_exact_sum(const double *t, size_t n);
+
 /* Number of OpenMP threads the element-wise loops use (0 = runtime default). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -4,6 +4,7 @@
  * TEST INFRASTRUCTURE ONLY (CPU baseline and fixture generation).
  *
  *   orc_q3d NX NY NZ solver smoother eps [steady=1] [threads=0] [out.bin]
+ *   (environment ORC_MAXITR caps the iteration count, default 20000)
  *
  * Prints the reference's log lines ("Initial residual", "%10d %24.14E",
  * "Converged at"), theta min/max over the interior and the solve time of
@@ -65,15 +66,18 @@ int main(int argc, char **argv)
     orc_set_bc_coef(bc, g.HC);
 
     int sm = strcmp(smoother, "gs") == 0 ? 1 : 0;
-    int cap = 20000, itr = 0;
+    const char *mi = getenv("ORC_MAXITR");   /* bounded runs for the CPU baseline */
+    int maxItr = mi ? atoi(mi) : 20000;
+    if (maxItr < 1) maxItr = 1;
+    int cap = maxItr, itr = 0;
     double res0 = 0.0, *hist = calloc(cap, sizeof(double));
     double t1 = now();
     orc_cal_rhs(&wk, &g, bc);
     int conv;
     if (strcmp(solver, "cg") == 0)
-        conv = orc_cg(&wk, &g, eps, 20000, sm, 1, &itr, &res0, hist, cap);
+        conv = orc_cg(&wk, &g, eps, maxItr, sm, 1, &itr, &res0, hist, cap);
     else
-        conv = orc_pbicgstab(&wk, &g, eps, 20000, sm, 1, &itr, &res0, hist, cap);
+        conv = orc_pbicgstab(&wk, &g, eps, maxItr, sm, 1, &itr, &res0, hist, cap);
     double t_solve = now() - t1;
 
     double mn = 1e300, mx = -1e300, l2 = 0.0;
