@@ -2,11 +2,12 @@
 // boundary-condition set-up, RHS, and the PBiCGSTAB!/CG! drivers.
 //
 // The Krylov scalars (ρ, α, ω, β, residuals, iteration count, status) live
-// in device memory (KState) and are updated by the last workgroup of each
-// reducing kernel, so an iteration is a fixed sequence of kernel launches
-// with no host round trip.  The host enqueues iterations in batches and
-// reads the status word once per batch; kernels enqueued after the solve has
-// stopped see status != running and return at once.
+// in device memory (KState) and are updated by single-workgroup finalize
+// kernels, so an iteration is a fixed sequence of kernel launches with no
+// host round trip.  One iteration is captured once per solve as a HIP graph
+// and replayed; the host reads the status word once per batch of replays,
+// keeping one batch in flight.  Kernels that run after the solve has stopped
+// see status != running and return at once.
 
 #include "h3d.h"
 #include "h3d_internal.hpp"
@@ -20,6 +21,10 @@
 #include <vector>
 
 using namespace h3d;
+
+namespace {
+enum { F32_LAM = 0, F32_B, F32_X, F32_P, F32_P_, F32_R, F32_R0, F32_Q, F32_S, F32_S_, F32_T, F32_N };
+}
 
 struct h3d_ctx {
     int MX = 0, MY = 0, MZ = 0, NX = 0, NY = 0, NZ = 0;
@@ -37,23 +42,29 @@ struct h3d_ctx {
     bool bc_set = false;
     KState *d_state = nullptr;   // solver state
     KState *d_unit = nullptr;    // state for single-operator calls (status stays 0)
-    KState *h_state = nullptr;   // pinned mirror
+    KState *h_state = nullptr;   // pinned mirrors: [0] staging, [1..2] batch status ring
     double *d_hist = nullptr;
     int hist_cap = 0;
     std::vector<double> hist;
     RedBuf rb{};
+    // conductivity classes (table form of the operator)
+    int coef_form = H3D_COEF_AUTO;
+    unsigned long lam_version = 1, cls_version = 0;
+    int K = 0;                   // 0: table form unavailable for the current λ
+    uint8_t *d_cls = nullptr;
+    uint64_t *d_vals = nullptr;
+    double *d_tab64 = nullptr;
+    float *d_tab32 = nullptr;
     // FP32 shadows for the refinement inner solve (allocated on first use)
-    float *g32[10] = {};         // lam, rhs, x(=d), p, p_, r, r0, q, s, s_ ... see F32_* below
-    float *g32_t = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float *g32[F32_N] = {};
+    double *d_corr = nullptr;    // FP64 correction accumulated by the FP32 inner solve
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evb[2] = {nullptr, nullptr};
     double last_ms = 0.0;
     int verbose = 0;             // H3D_VERBOSE=1: refinement diagnostics on stderr
     std::string err;
 };
 
 namespace {
-
-enum { F32_LAM = 0, F32_B, F32_X, F32_P, F32_P_, F32_R, F32_R0, F32_Q, F32_S, F32_S_ };
 
 std::string g_create_err;
 
@@ -79,24 +90,93 @@ Geo<T> base_geo(const h3d_ctx *c)
     g.MX = c->MX; g.MY = c->MY; g.MZ = c->MZ;
     g.NX = c->NX; g.NY = c->NY; g.NZ = c->NZ;
     g.dx0 = (T)c->dx; g.dy0 = (T)c->dy;
-    // ddx = inv(dx0), ddy = inv(dy0) (NonUniform.jl:472-473)
-    g.ddx = (T)(1.0 / c->dx); g.ddy = (T)(1.0 / c->dy);
-    if (sizeof(T) == 4) { g.ddx = (T)1 / g.dx0; g.ddy = (T)1 / g.dy0; }
+    // ddx = inv(dx0), ddy = inv(dy0) (NonUniform.jl:472-473), in the stencil's type
+    g.ddx = (T)1 / g.dx0; g.ddy = (T)1 / g.dy0;
     for (int f = 0; f < 6; ++f) g.HC[f] = (T)c->HC[f];
     return g;
+}
+
+// λf(a, b, ma, 1) exactly as the device forms it (src/common.jl:35)
+template <typename T>
+T lamf_host(T a, T b, int m)
+{
+    const T f = m ? T(1) : T(2);
+    return T(2) * a * b / (a + b) * f;
+}
+
+// Classes of the conductivity field for the table form: the distinct λ bit
+// patterns (at most kMaxClasses), every cell's index among them, and the
+// tables tab[m][a][b] = λf(v_a, v_b, m, 1) in FP64 and FP32.  Rebuilt when
+// λ changed since the last build; K = 0 when there are too many values.
+int ensure_classes(h3d_ctx *c)
+{
+    if (c->cls_version == c->lam_version) return H3D_OK;
+    c->K = 0;
+    c->cls_version = c->lam_version;
+    if (c->coef_form == H3D_COEF_DIRECT) return H3D_OK;
+    std::vector<double> lam(c->H);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(lam.data(), c->f[H3D_F_LAMBDA], c->H * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> vals;
+    uint64_t last = 0;
+    bool have_last = false;
+    for (size_t a = 0; a < c->H; ++a) {
+        uint64_t v;
+        std::memcpy(&v, &lam[a], sizeof v);
+        if (have_last && v == last) continue;
+        last = v;
+        have_last = true;
+        if (std::find(vals.begin(), vals.end(), v) == vals.end()) {
+            if ((int)vals.size() == kMaxClasses) return H3D_OK;  // direct form
+            vals.push_back(v);
+        }
+    }
+    std::sort(vals.begin(), vals.end());
+    const int K = (int)vals.size();
+    std::vector<double> t64((size_t)2 * K * K);
+    std::vector<float> t32((size_t)2 * K * K);
+    for (int m = 0; m < 2; ++m)
+        for (int a = 0; a < K; ++a)
+            for (int b = 0; b < K; ++b) {
+                double va, vb;
+                std::memcpy(&va, &vals[a], 8);
+                std::memcpy(&vb, &vals[b], 8);
+                t64[((size_t)m * K + a) * K + b] = lamf_host<double>(va, vb, m);
+                t32[((size_t)m * K + a) * K + b] = lamf_host<float>((float)va, (float)vb, m);
+            }
+    if (!c->d_cls) HIPCHK(c, hipMalloc(&c->d_cls, c->H));
+    if (!c->d_vals) HIPCHK(c, hipMalloc(&c->d_vals, kMaxClasses * sizeof(uint64_t)));
+    if (!c->d_tab64) HIPCHK(c, hipMalloc(&c->d_tab64, 2 * kMaxClasses * kMaxClasses * sizeof(double)));
+    if (!c->d_tab32) HIPCHK(c, hipMalloc(&c->d_tab32, 2 * kMaxClasses * kMaxClasses * sizeof(float)));
+    HIPCHK(c, hipMemcpy(c->d_vals, vals.data(), K * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, launch_classify(c->d_cls, c->f[H3D_F_LAMBDA], c->d_vals, K, c->H, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->K = K;
+    return H3D_OK;
 }
 
 // Operator constants for FP64; computes ap0 for a transient operator.
 int make_geo64(h3d_ctx *c, double dt, int is_steady, Geo<double> *out)
 {
     if (!c->grid_set) return fail(c, H3D_ERR_STATE, "h3d_set_grid has not been called");
+    int rc = ensure_classes(c);
+    if (rc != H3D_OK) return rc;
     Geo<double> g = base_geo<double>(c);
     g.ZC = c->d_ZC; g.dZ = c->d_dZ;
+    g.lam = c->f[H3D_F_LAMBDA];
+    if (c->K > 0) {
+        g.cls = c->d_cls;
+        g.tab = c->d_tab64;
+        g.K = c->K;
+    }
+    g.flux = 0;
     g.ap0 = nullptr;
     if (!is_steady) {
         if (!(dt > 0.0) && !(dt < 0.0)) return fail(c, H3D_ERR_ARG, "transient operator needs dt != 0");
         if (!c->d_ap0) HIPCHK(c, hipMalloc(&c->d_ap0, c->H * sizeof(double)));
-        HIPCHK(c, hipMemsetAsync(c->d_ap0, 0, c->H * sizeof(double), c->stream));
+        HIPCHK(c, launch_fill<double>(c->d_ap0, 0.0, c->H, c->stream));
         const double ddt = 1.0 / dt;  // ddt = inv(Δt)
         HIPCHK(c, launch_ap0(c->d_ap0, c->f[H3D_F_RHO], c->f[H3D_F_CP], g, ddt, c->stream));
         g.ap0 = c->d_ap0;
@@ -108,6 +188,7 @@ int make_geo64(h3d_ctx *c, double dt, int is_steady, Geo<double> *out)
 int ensure_hist(h3d_ctx *c, int n)
 {
     if (n <= c->hist_cap) return H3D_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->d_hist) (void)hipFree(c->d_hist);
     c->d_hist = nullptr;
     HIPCHK(c, hipMalloc(&c->d_hist, (size_t)n * sizeof(double)));
@@ -125,68 +206,79 @@ int reset_state(h3d_ctx *c, double tol, int maxItr)
     // the stream is idle here (every entry point ends synchronised), so the
     // pinned mirror can stage the initial state
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *c->h_state = s;
-    HIPCHK(c, hipMemcpyAsync(c->d_state, c->h_state, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+    c->h_state[0] = s;
+    HIPCHK(c, hipMemcpyAsync(c->d_state, &c->h_state[0], sizeof(KState), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return H3D_OK;
 }
 
-int read_state(h3d_ctx *c, KState *out)
-{
-    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    *out = *c->h_state;
-    return H3D_OK;
-}
-
-// Preconditioner! (NonUniform.jl:531-559) on the device: :gs = 5 sweeps of
-// colour 0 then colour 1, :none = mycopy!.  xx is expected zeroed (callers do
-// myfill!(xx, 0) first, as the reference does).
+// Preconditioner! (NonUniform.jl:531-559): :gs = 5 sweeps of colour 0 then
+// colour 1 on xx in place, :none = mycopy!(xx, bb).
 template <typename T>
-hipError_t precond(T *xx, const T *bb, const T *lam, const Geo<T> &g, int smoother, const KState *st, size_t H,
-                   hipStream_t s)
+hipError_t precond(T *xx, const T *bb, const Geo<T> &g, int smoother, const KState *st, size_t H, hipStream_t s)
 {
-    if (smoother == H3D_SMOOTHER_NONE) {
-        return hipMemcpyAsync(xx, bb, H * sizeof(T), hipMemcpyDeviceToDevice, s);
-    }
+    if (smoother == H3D_SMOOTHER_NONE) return hipMemcpyAsync(xx, bb, H * sizeof(T), hipMemcpyDeviceToDevice, s);
     hipError_t e = hipSuccess;
     for (int sweep = 0; sweep < 5 && e == hipSuccess; ++sweep) {
-        e = launch_rbsor<T>(xx, bb, lam, g, 0, st, s);
-        if (e == hipSuccess) e = launch_rbsor<T>(xx, bb, lam, g, 1, st, s);
+        e = launch_rbsor<T>(xx, bb, g, 0, st, s);
+        if (e == hipSuccess) e = launch_rbsor<T>(xx, bb, g, 1, st, s);
     }
     return e;
 }
 
-// Buffers of one Krylov solve in type T.
+// myfill!(xx, 0) followed by Preconditioner!: the fill and colour 0 of the
+// first sweep run in one kernel, which also forms bb first when mode asks
+// (BiCG1!/Triad!, see launch_rbsor_first).
+template <typename T>
+hipError_t precond_from_zero(int mode, T *xx, T *bb, const T *r, const T *q, const Geo<T> &g, int smoother,
+                             const KState *st, size_t H, hipStream_t s)
+{
+    hipError_t e;
+    if (smoother == H3D_SMOOTHER_NONE) {
+        if (mode != FIRST_COPY && (e = launch_form_rhs<T>(mode, bb, r, q, g, st, s)) != hipSuccess) return e;
+        return hipMemcpyAsync(xx, bb, H * sizeof(T), hipMemcpyDeviceToDevice, s);
+    }
+    if ((e = launch_rbsor_first<T>(mode, xx, bb, r, q, g, st, s)) != hipSuccess) return e;
+    if ((e = launch_rbsor<T>(xx, bb, g, 1, st, s)) != hipSuccess) return e;
+    for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
+        e = launch_rbsor<T>(xx, bb, g, 0, st, s);
+        if (e == hipSuccess) e = launch_rbsor<T>(xx, bb, g, 1, st, s);
+    }
+    return e;
+}
+
+// Buffers of one Krylov solve in type T.  x is the FP64 iterate the
+// updates accumulate into; th is the starting iterate CalcRK! reads (x
+// itself in FP64, a zero field in the FP32 inner solve, which starts at 0).
 template <typename T>
 struct KVecs {
-    T *th, *b, *lam, *p, *p_, *r, *r0, *q, *s, *s_, *t;
+    double *x;
+    T *th, *b, *p, *p_, *r, *r0, *q, *s, *s_, *t;
 };
 
 // Enqueue one PBiCGSTAB! iteration (NonUniform.jl:287-347).  ρ of the
-// iteration was formed by the previous iteration's update kernel (or by
-// CalcRK! for iteration 1) and its FIN_BICG_*RHO step.
+// iteration was formed by the previous iteration's update (or by CalcRK!
+// for iteration 1) and its FIN_BICG_*RHO step.
 template <typename T>
 hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int smoother)
 {
     hipStream_t s = c->stream;
     KState *st = c->d_state;
     hipError_t e;
-    if ((e = launch_bicg_p<T>(v.p, v.r, v.q, g, st, s)) != hipSuccess) return e;
-    if (smoother == H3D_SMOOTHER_GS) {
-        if ((e = launch_fill<T>(v.p_, T(0), c->H, s)) != hipSuccess) return e;
-    }
-    if ((e = precond<T>(v.p_, v.p, v.lam, g, smoother, st, c->H, s)) != hipSuccess) return e;
-    if ((e = launch_calc_ax<T>(v.q, v.p_, v.lam, g, v.r0, nullptr, FIN_BICG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
+    // p = r or BiCG1!; myfill!(p_, 0); Preconditioner!(p_, p)
+    if ((e = precond_from_zero<T>(FIRST_BICG_P, v.p_, v.p, v.r, v.q, g, smoother, st, c->H, s)) != hipSuccess)
         return e;
-    if ((e = launch_bicg_s<T>(v.s, v.q, v.r, g, st, s)) != hipSuccess) return e;
-    if (smoother == H3D_SMOOTHER_GS) {
-        if ((e = launch_fill<T>(v.s_, T(0), c->H, s)) != hipSuccess) return e;
-    }
-    if ((e = precond<T>(v.s_, v.s, v.lam, g, smoother, st, c->H, s)) != hipSuccess) return e;
-    if ((e = launch_calc_ax<T>(v.t, v.s_, v.lam, g, nullptr, v.s, FIN_BICG_OMEGA, st, nullptr, c->rb, s)) != hipSuccess)
+    // q = A p_; alpha = rho / (q, r0)
+    if ((e = launch_calc_ax<T>(v.q, v.p_, g, v.r0, nullptr, FIN_BICG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
         return e;
-    return launch_bicg_update<T>(v.th, v.r, v.p_, v.s_, v.t, v.s, v.r0, g, st, c->d_hist, c->rb, s);
+    // s = -alpha q + r; myfill!(s_, 0); Preconditioner!(s_, s)
+    if ((e = precond_from_zero<T>(FIRST_BICG_S, v.s_, v.s, v.r, v.q, g, smoother, st, c->H, s)) != hipSuccess)
+        return e;
+    // t = A s_; omega = (t, s) / (t, t)
+    if ((e = launch_calc_ax<T>(v.t, v.s_, g, nullptr, v.s, FIN_BICG_OMEGA, st, nullptr, c->rb, s)) != hipSuccess)
+        return e;
+    // θ += alpha p_ + omega s_; r = -omega t + s; res; next ρ = (r, r0)
+    return launch_bicg_update<T>(v.x, v.r, v.p_, v.s_, v.t, v.s, v.r0, g, st, c->d_hist, c->rb, s);
 }
 
 // Enqueue one CG! iteration (NonUniform.jl:160-223).
@@ -196,19 +288,16 @@ hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int s
     hipStream_t s = c->stream;
     KState *st = c->d_state;
     hipError_t e;
-    if ((e = launch_calc_ax<T>(v.q, v.p, v.lam, g, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
+    if ((e = launch_calc_ax<T>(v.q, v.p, g, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
         return e;
-    if ((e = launch_cg_update<T>(v.th, v.r, v.p, v.q, g, st, c->d_hist, c->rb, s)) != hipSuccess) return e;
-    if (smoother == H3D_SMOOTHER_GS) {
-        if ((e = launch_fill<T>(v.s, T(0), c->H, s)) != hipSuccess) return e;
-    }
-    if ((e = precond<T>(v.s, v.r, v.lam, g, smoother, st, c->H, s)) != hipSuccess) return e;
+    if ((e = launch_cg_update<T>(v.x, v.r, v.p, v.q, g, st, c->d_hist, c->rb, s)) != hipSuccess) return e;
+    if ((e = precond_from_zero<T>(FIRST_COPY, v.s, v.r, nullptr, nullptr, g, smoother, st, c->H, s)) != hipSuccess)
+        return e;
     if ((e = launch_dot<T>(v.r, v.s, nullptr, g, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
     return launch_cg_p<T>(v.p, v.s, g, st, s);
 }
 
 // Run a Krylov solve whose vectors are in v, starting from the current θ.
-// Returns status via *fin_state.
 template <typename T>
 int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Geo<T> &g, double tol, int maxItr, int smoother,
            KState *fin_state)
@@ -217,42 +306,73 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Geo<T> &g, double to
     int rc;
     if ((rc = ensure_hist(c, std::max(maxItr, 1))) != H3D_OK) return rc;
     if ((rc = reset_state(c, tol, maxItr)) != H3D_OK) return rc;
-    // myfill!(q, 0); res0 = CalcRK!(r, θ, b, ...); PBiCGSTAB also copies r0 = r
-    HIPCHK(c, launch_fill<T>(v.q, T(0), c->H, s));
+    // The Krylov vectors keep a zero halo, as in the reference where only
+    // their interiors are ever written (WorkBuffers starts them at zero).
+    T *vecs[] = {v.p, v.p_, v.r, v.r0, v.q, v.s, v.s_, v.t};
+    for (T *a : vecs) HIPCHK(c, launch_fill<T>(a, T(0), c->H, s));
+    // res0 = CalcRK!(r, θ, b, ...); PBiCGSTAB also copies r0 = r
     const bool bicg = (solver == H3D_SOLVER_PBICGSTAB);
-    HIPCHK(c, launch_calc_rk<T>(v.r, bicg ? v.r0 : nullptr, v.th, v.b, v.lam, g, bicg ? FIN_RES0_BICG : FIN_RES0,
+    HIPCHK(c, launch_calc_rk<T>(v.r, bicg ? v.r0 : nullptr, v.th, v.b, g, bicg ? FIN_RES0_BICG : FIN_RES0,
                                 c->d_state, nullptr, c->rb, s));
-    if (solver == H3D_SOLVER_CG) {
+    if (!bicg) {
         // z = M⁻¹ r; p = z; rho_old = (r, z)   (NonUniform.jl:147-155)
-        if (smoother == H3D_SMOOTHER_GS) HIPCHK(c, launch_fill<T>(v.s, T(0), c->H, s));
-        HIPCHK(c, precond<T>(v.s, v.r, v.lam, g, smoother, c->d_state, c->H, s));
+        HIPCHK(c, precond_from_zero<T>(FIRST_COPY, v.s, v.r, nullptr, nullptr, g, smoother, c->d_state, c->H, s));
         HIPCHK(c, launch_copy_interior<T>(v.p, v.s, g, c->d_state, s));
         HIPCHK(c, launch_dot<T>(v.r, v.s, nullptr, g, FIN_CG_RHO0, c->d_state, nullptr, c->rb, s));
     }
-    // batches of iterations; the status word is read once per batch
-    int launched = 0, batch = 1;
+    // capture one iteration as a graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = bicg ? enqueue_bicg_iter<T>(c, v, g, smoother) : enqueue_cg_iter<T>(c, v, g, smoother);
+    hipError_t ee = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(c, H3D_ERR_HIP, std::string("capturing the iteration: ") +
+                                        hipGetErrorString(ce != hipSuccess ? ce : ee));
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(c, H3D_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    // Replay in batches.  After each batch the state is copied into a pinned
+    // slot and an event recorded; the host checks the previous batch's slot
+    // while the next batch runs.
+    int launched = 0, nbatch = 0;
+    const int batch = 4;
     KState st{};
-    while (true) {
+    bool done = false;
+    while (!done) {
         const int nb = std::min(batch, maxItr - launched);
         for (int it = 0; it < nb; ++it) {
-            hipError_t e = (solver == H3D_SOLVER_CG) ? enqueue_cg_iter<T>(c, v, g, smoother)
-                                                     : enqueue_bicg_iter<T>(c, v, g, smoother);
-            if (e != hipSuccess) return fail(c, H3D_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+            hipError_t e = hipGraphLaunch(exec, s);
+            if (e != hipSuccess) {
+                (void)hipGraphExecDestroy(exec);
+                return fail(c, H3D_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+            }
         }
         launched += nb;
-        if ((rc = read_state(c, &st)) != H3D_OK) return rc;
-        if (st.status != ST_RUNNING || launched >= maxItr) break;
-        batch = std::min(batch * 2, 16);
+        const int slot = 1 + (nbatch & 1);
+        HIPCHK(c, hipMemcpyAsync(&c->h_state[slot], c->d_state, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipEventRecord(c->evb[nbatch & 1], s));
+        if (nbatch > 0) {
+            const int prev = 1 + ((nbatch - 1) & 1);
+            HIPCHK(c, hipEventSynchronize(c->evb[(nbatch - 1) & 1]));
+            if (c->h_state[prev].status != ST_RUNNING) done = true;
+        }
+        if (launched >= maxItr) done = true;
+        ++nbatch;
     }
+    (void)hipGraphExecDestroy(exec);
+    HIPCHK(c, hipStreamSynchronize(s));
+    st = c->h_state[1 + ((nbatch - 1) & 1)];
     *fin_state = st;
     return H3D_OK;
 }
 
-template <typename T>
-KVecs<T> fp64_vecs(h3d_ctx *c)
+KVecs<double> fp64_vecs(h3d_ctx *c)
 {
-    KVecs<T> v;
-    v.th = c->f[H3D_F_THETA]; v.b = c->f[H3D_F_B]; v.lam = c->f[H3D_F_LAMBDA];
+    KVecs<double> v;
+    v.x = v.th = c->f[H3D_F_THETA]; v.b = c->f[H3D_F_B];
     v.p = c->f[H3D_F_PCG_P]; v.p_ = c->f[H3D_F_PCG_P_]; v.r = c->f[H3D_F_PCG_R];
     v.r0 = c->f[H3D_F_PCG_R0]; v.q = c->f[H3D_F_PCG_Q]; v.s = c->f[H3D_F_PCG_S];
     v.s_ = c->f[H3D_F_PCG_S_]; v.t = c->f[H3D_F_PCG_T_];
@@ -280,7 +400,7 @@ int solve_fp64(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int sm
     if (rc != H3D_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     KState st{};
-    rc = krylov<double>(c, solver, fp64_vecs<double>(c), g, tol, maxItr, smoother, &st);
+    rc = krylov<double>(c, solver, fp64_vecs(c), g, tol, maxItr, smoother, &st);
     if (rc != H3D_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -293,14 +413,14 @@ int solve_fp64(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int sm
 // ---------------------------------------------------------------------------
 // FP32 inner solves in FP64 iterative refinement
 //   r = b - Aθ (FP64 CalcRK!), stop when ||r||/res0 < tol;
-//   solve A d = r in FP32 with the same Krylov method to inner_tol;
+//   solve A d = r in FP32 (flux form of A, FP64 dot products) to inner_tol;
 //   θ += d (FP64).
 // ---------------------------------------------------------------------------
 int ensure_fp32(h3d_ctx *c)
 {
-    for (int a = 0; a < 10; ++a)
+    for (int a = 0; a < F32_N; ++a)
         if (!c->g32[a]) HIPCHK(c, hipMalloc(&c->g32[a], c->H * sizeof(float)));
-    if (!c->g32_t) HIPCHK(c, hipMalloc(&c->g32_t, c->H * sizeof(float)));
+    if (!c->d_corr) HIPCHK(c, hipMalloc(&c->d_corr, c->H * sizeof(double)));
     if (!c->d_ZC32) HIPCHK(c, hipMalloc(&c->d_ZC32, c->MZ * sizeof(float)));
     if (!c->d_dZ32) HIPCHK(c, hipMalloc(&c->d_dZ32, c->MZ * sizeof(float)));
     return H3D_OK;
@@ -316,25 +436,30 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
     hipStream_t s = c->stream;
     if (!(inner_tol > 0.0)) inner_tol = 1e-4;
     HIPCHK(c, hipEventRecord(c->ev0, s));
-    // FP32 operator: λ, ZC, ΔZ, ap0 rounded to float
+    // FP32 operator: λ, ZC, ΔZ, ap0 rounded to float; the same classes
     HIPCHK(c, launch_convert<float, double>(c->g32[F32_LAM], c->f[H3D_F_LAMBDA], c->H, s));
     HIPCHK(c, launch_convert<float, double>(c->d_ZC32, c->d_ZC, (size_t)c->MZ, s));
     HIPCHK(c, launch_convert<float, double>(c->d_dZ32, c->d_dZ, (size_t)c->MZ, s));
     Geo<float> g32 = base_geo<float>(c);
     g32.ZC = c->d_ZC32; g32.dZ = c->d_dZ32; g32.ap0 = nullptr;
+    g32.lam = c->g32[F32_LAM];
+    if (c->K > 0) {
+        g32.cls = c->d_cls;
+        g32.tab = c->d_tab32;
+        g32.K = c->K;
+    }
+    g32.flux = 1;
     if (!is_steady) {
         if (!c->d_ap032) HIPCHK(c, hipMalloc(&c->d_ap032, c->H * sizeof(float)));
         HIPCHK(c, launch_convert<float, double>(c->d_ap032, c->d_ap0, c->H, s));
         g32.ap0 = c->d_ap032;
     }
     KVecs<float> v;
-    v.th = c->g32[F32_X]; v.b = c->g32[F32_B]; v.lam = c->g32[F32_LAM];
+    v.x = c->d_corr;
+    v.th = c->g32[F32_X]; v.b = c->g32[F32_B];
     v.p = c->g32[F32_P]; v.p_ = c->g32[F32_P_]; v.r = c->g32[F32_R]; v.r0 = c->g32[F32_R0];
-    v.q = c->g32[F32_Q]; v.s = c->g32[F32_S]; v.s_ = c->g32[F32_S_]; v.t = c->g32_t;
-    // Krylov halos must be zero
-    for (int a = F32_X; a <= F32_S_; ++a) HIPCHK(c, launch_fill<float>(c->g32[a], 0.f, c->H, s));
-    HIPCHK(c, launch_fill<float>(c->g32_t, 0.f, c->H, s));
-    HIPCHK(c, launch_fill<float>(c->g32[F32_B], 0.f, c->H, s));
+    v.q = c->g32[F32_Q]; v.s = c->g32[F32_S]; v.s_ = c->g32[F32_S_]; v.t = c->g32[F32_T];
+    HIPCHK(c, launch_fill<float>(v.b, 0.f, c->H, s));
 
     double *r64 = c->f[H3D_F_PCG_R];
     std::vector<double> outer;
@@ -344,13 +469,11 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
     const int max_outer = 100;
     for (int outer_it = 0; outer_it <= max_outer; ++outer_it) {
         // true FP64 residual r = b - Aθ
-        HIPCHK(c, launch_calc_rk<double>(r64, nullptr, c->f[H3D_F_THETA], c->f[H3D_F_B], c->f[H3D_F_LAMBDA], g64,
-                                         FIN_STORE, c->d_unit, nullptr, c->rb, s));
-        KState u{};
-        HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, launch_calc_rk<double>(r64, nullptr, c->f[H3D_F_THETA], c->f[H3D_F_B], g64, FIN_STORE, c->d_unit,
+                                         nullptr, c->rb, s));
+        HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
-        u = *c->h_state;
-        const double rn = std::sqrt(u.out[0]);
+        const double rn = std::sqrt(c->h_state[0].out[0]);
         if (outer_it == 0) {
             res0 = rn;
             if (res0 == 0.0) { converged = true; break; }
@@ -363,6 +486,7 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         // inner FP32 solve of A d = r from d = 0
         HIPCHK(c, launch_convert<float, double>(v.b, r64, c->H, s));
         HIPCHK(c, launch_fill<float>(v.th, 0.f, c->H, s));
+        HIPCHK(c, launch_fill<double>(v.x, 0.0, c->H, s));
         KState st{};
         rc = krylov<float>(c, solver, v, g32, inner_tol, std::max(1, maxItr - total_inner), smoother, &st);
         if (rc != H3D_OK) return rc;
@@ -377,8 +501,23 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
                     std::fprintf(stderr, "   inner %5d %.6e\n", k + 1, h[(size_t)k]);
             }
         }
+        if (c->verbose > 1) {
+            // FP32-operator residual of the accumulated correction, against the
+            // recurrence residual the inner solve stopped on
+            HIPCHK(c, launch_convert<float, double>(c->g32[F32_P], v.x, c->H, s));
+            HIPCHK(c, launch_calc_rk<float>(c->g32[F32_Q], nullptr, c->g32[F32_P], v.b, g32, FIN_STORE, c->d_unit,
+                                            nullptr, c->rb, s));
+            HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            const double r32 = std::sqrt(c->h_state[0].out[0]);
+            HIPCHK(c, launch_dot<float>(v.b, nullptr, nullptr, g32, FIN_STORE, c->d_unit, nullptr, c->rb, s));
+            HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            std::fprintf(stderr, "   FP32-operator true residual of the correction: %.3e (relative to |b32|)\n",
+                         r32 / std::sqrt(c->h_state[0].out[0]));
+        }
         if (st.status == ST_NAN) { nan = true; break; }
-        HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], v.th, g64, s));
+        HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], v.x, g64, s));
         if (st.status == ST_BREAKDOWN && st.itr <= 1) break;  // no progress possible
     }
     HIPCHK(c, hipEventRecord(c->ev1, s));
@@ -410,9 +549,9 @@ int solve(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoothe
 
 int unit_reduce_result(h3d_ctx *c, double *out, int n)
 {
-    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int a = 0; a < n; ++a) out[a] = c->h_state->out[a];
+    for (int a = 0; a < n; ++a) out[a] = c->h_state[0].out[a];
     return H3D_OK;
 }
 
@@ -466,16 +605,17 @@ int h3d_create(h3d_ctx **out, int MX, int MY, int MZ, int device)
     if ((e = hipMalloc(&c->d_unit, sizeof(KState))) != hipSuccess) return bail("hipMalloc", e);
     if ((e = hipMemsetAsync(c->d_state, 0, sizeof(KState), c->stream)) != hipSuccess) return bail("hipMemset", e);
     if ((e = hipMemsetAsync(c->d_unit, 0, sizeof(KState), c->stream)) != hipSuccess) return bail("hipMemset", e);
-    if ((e = hipHostMalloc(&c->h_state, sizeof(KState), hipHostMallocDefault)) != hipSuccess) return bail("hipHostMalloc", e);
+    if ((e = hipHostMalloc(&c->h_state, 3 * sizeof(KState), hipHostMallocDefault)) != hipSuccess)
+        return bail("hipHostMalloc", e);
     // reduction scratch: enough partials for any interior launch shape
     const long maxb = ((long)(c->NX + 63) / 64) * ((c->NY + 3) / 4) * (long)c->NZ + 64;
     c->rb.nblocks_max = (int)std::min<long>(maxb, 1L << 30);
     if ((e = hipMalloc(&c->rb.partials, (size_t)c->rb.nblocks_max * 6 * sizeof(double))) != hipSuccess)
         return bail("hipMalloc partials", e);
-    if ((e = hipMalloc(&c->rb.counter, sizeof(unsigned int))) != hipSuccess) return bail("hipMalloc", e);
-    if ((e = hipMemsetAsync(c->rb.counter, 0, sizeof(unsigned int), c->stream)) != hipSuccess) return bail("hipMemset", e);
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail("event", e);
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
+    for (int a = 0; a < 2; ++a)
+        if ((e = hipEventCreateWithFlags(&c->evb[a], hipEventDisableTiming)) != hipSuccess) return bail("event", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("sync", e);
     *out = c;
     return H3D_OK;
@@ -487,21 +627,50 @@ int h3d_destroy(h3d_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int f = 0; f < H3D_NFIELDS; ++f)
         if (c->f[f]) (void)hipFree(c->f[f]);
-    for (int a = 0; a < 10; ++a)
+    for (int a = 0; a < F32_N; ++a)
         if (c->g32[a]) (void)hipFree(c->g32[a]);
-    void *ps[] = {c->g32_t, c->d_ZC, c->d_dZ, c->d_ap0, c->d_ZC32, c->d_dZ32, c->d_ap032,
-                  c->d_state, c->d_unit, c->d_hist, c->rb.partials, c->rb.counter};
+    void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_state,
+                  c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_tab64, c->d_tab32, c->rb.partials,
+                  c->d_corr};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (int a = 0; a < 2; ++a)
+        if (c->evb[a]) (void)hipEventDestroy(c->evb[a]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return H3D_OK;
 }
 
 const char *h3d_last_error(const h3d_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int h3d_set_option(h3d_ctx *c, int option, int value)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (option == H3D_OPT_COEF_FORM) {
+        if (value != H3D_COEF_AUTO && value != H3D_COEF_DIRECT) return fail(c, H3D_ERR_ARG, "bad coefficient form");
+        c->coef_form = value;
+        c->lam_version++;  // rebuild (or drop) the class tables at the next use
+        return H3D_OK;
+    }
+    return fail(c, H3D_ERR_ARG, "unknown option");
+}
+
+int h3d_get_option(const h3d_ctx *c, int option, int *value)
+{
+    if (!c || !value) return H3D_ERR_ARG;
+    if (option == H3D_OPT_COEF_FORM) {
+        *value = c->coef_form;
+        return H3D_OK;
+    }
+    if (option == H3D_OPT_NUM_CLASSES) {
+        *value = c->K;
+        return H3D_OK;
+    }
+    return H3D_ERR_ARG;
+}
 
 int h3d_set_grid(h3d_ctx *c, double dx, double dy, const double *ZC, const double *dZ, int mz)
 {
@@ -519,7 +688,8 @@ int h3d_set_grid(h3d_ctx *c, double dx, double dy, const double *ZC, const doubl
 int h3d_set_bc(h3d_ctx *c, const int type[6], const double temperature[6], const double heat_flux[6],
                const double htc[6], const double ambient[6])
 {
-    if (!c || !type || !temperature || !heat_flux || !htc || !ambient) return c ? fail(c, H3D_ERR_ARG, "null bc array") : H3D_ERR_ARG;
+    if (!c) return H3D_ERR_ARG;
+    if (!type || !temperature || !heat_flux || !htc || !ambient) return fail(c, H3D_ERR_ARG, "null bc array");
     for (int f = 0; f < 6; ++f)
         if (type[f] < H3D_ISOTHERMAL || type[f] > H3D_CONVECTION) return fail(c, H3D_ERR_ARG, "bad boundary type");
     for (int f = 0; f < 6; ++f) {
@@ -542,6 +712,11 @@ int h3d_get_hc(const h3d_ctx *c, double HC[6])
     return H3D_OK;
 }
 
+static bool inner_cell(const h3d_ctx *c, int i, int j, int k)
+{
+    return i > 0 && i < c->MX - 1 && j > 0 && j < c->MY - 1 && k > 0 && k < c->MZ - 1;
+}
+
 int h3d_upload(h3d_ctx *c, int field, const double *host, size_t n)
 {
     if (!c) return H3D_ERR_ARG;
@@ -550,9 +725,8 @@ int h3d_upload(h3d_ctx *c, int field, const double *host, size_t n)
         for (int k = 0; k < c->MZ; ++k)
             for (int j = 0; j < c->MY; ++j)
                 for (int i = 0; i < c->MX; ++i) {
-                    const bool inner = i > 0 && i < c->MX - 1 && j > 0 && j < c->MY - 1 && k > 0 && k < c->MZ - 1;
                     const double m = host[(size_t)i + (size_t)c->MX * ((size_t)j + (size_t)c->MY * k)];
-                    if (m != (inner ? 1.0 : 0.0))
+                    if (m != (inner_cell(c, i, j, k) ? 1.0 : 0.0))
                         return fail(c, H3D_ERR_UNSUPPORTED,
                                     "mask must be 0 on the halo and 1 inside (the post-boundary-condition mask)");
                 }
@@ -560,6 +734,7 @@ int h3d_upload(h3d_ctx *c, int field, const double *host, size_t n)
     }
     HIPCHK(c, hipMemcpyAsync(c->f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (field == H3D_F_LAMBDA) c->lam_version++;
     return H3D_OK;
 }
 
@@ -570,10 +745,8 @@ int h3d_download(h3d_ctx *c, int field, double *host, size_t n)
     if (field == H3D_F_MASK) {
         for (int k = 0; k < c->MZ; ++k)
             for (int j = 0; j < c->MY; ++j)
-                for (int i = 0; i < c->MX; ++i) {
-                    const bool inner = i > 0 && i < c->MX - 1 && j > 0 && j < c->MY - 1 && k > 0 && k < c->MZ - 1;
-                    host[(size_t)i + (size_t)c->MX * ((size_t)j + (size_t)c->MY * k)] = inner ? 1.0 : 0.0;
-                }
+                for (int i = 0; i < c->MX; ++i)
+                    host[(size_t)i + (size_t)c->MX * ((size_t)j + (size_t)c->MY * k)] = inner_cell(c, i, j, k) ? 1.0 : 0.0;
         return H3D_OK;
     }
     HIPCHK(c, hipMemcpyAsync(host, c->f[field], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -588,6 +761,7 @@ int h3d_fill(h3d_ctx *c, int field, double value)
     if (field == H3D_F_MASK) return fail(c, H3D_ERR_UNSUPPORTED, "the mask is implicit");
     HIPCHK(c, launch_fill<double>(c->f[field], value, c->H, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (field == H3D_F_LAMBDA) c->lam_version++;
     return H3D_OK;
 }
 
@@ -598,6 +772,7 @@ int h3d_copy(h3d_ctx *c, int dst, int src)
         return fail(c, H3D_ERR_ARG, "bad field id");
     HIPCHK(c, hipMemcpyAsync(c->f[dst], c->f[src], c->H * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (dst == H3D_F_LAMBDA) c->lam_version++;
     return H3D_OK;
 }
 
@@ -610,6 +785,7 @@ int h3d_apply_bc(h3d_ctx *c)
                                        c->f[H3D_F_LAMBDA], c->f[H3D_F_RHO], c->f[H3D_F_CP], c->MX, c->MY, c->MZ,
                                        c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lam_version++;
     return H3D_OK;
 }
 
@@ -669,8 +845,8 @@ int h3d_calc_ax(h3d_ctx *c, int ax, int x, double dt, int is_steady)
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
-    HIPCHK(c, launch_calc_ax<double>(c->f[ax], c->f[x], c->f[H3D_F_LAMBDA], g, nullptr, nullptr, -1, c->d_unit, nullptr,
-                                     c->rb, c->stream));
+    HIPCHK(c, launch_calc_ax<double>(c->f[ax], c->f[x], g, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb,
+                                     c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return H3D_OK;
 }
@@ -683,8 +859,8 @@ int h3d_calc_rk(h3d_ctx *c, int r, int theta, int b, double dt, int is_steady, d
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
-    HIPCHK(c, launch_calc_rk<double>(c->f[r], nullptr, c->f[theta], c->f[b], c->f[H3D_F_LAMBDA], g, FIN_STORE, c->d_unit,
-                                     nullptr, c->rb, c->stream));
+    HIPCHK(c, launch_calc_rk<double>(c->f[r], nullptr, c->f[theta], c->f[b], g, FIN_STORE, c->d_unit, nullptr, c->rb,
+                                     c->stream));
     double s = 0;
     if ((rc = unit_reduce_result(c, &s, 1)) != H3D_OK) return rc;
     if (res) *res = std::sqrt(s);
@@ -699,7 +875,7 @@ int h3d_rbsor_color(h3d_ctx *c, int x, int b, int color, double dt, int is_stead
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
-    HIPCHK(c, launch_rbsor<double>(c->f[x], c->f[b], c->f[H3D_F_LAMBDA], g, color, c->d_unit, c->stream));
+    HIPCHK(c, launch_rbsor<double>(c->f[x], c->f[b], g, color, c->d_unit, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return H3D_OK;
 }
@@ -713,7 +889,7 @@ int h3d_precond(h3d_ctx *c, int xx, int bb, int smoother, double dt, int is_stea
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
-    HIPCHK(c, precond<double>(c->f[xx], c->f[bb], c->f[H3D_F_LAMBDA], g, smoother, c->d_unit, c->H, c->stream));
+    HIPCHK(c, precond<double>(c->f[xx], c->f[bb], g, smoother, c->d_unit, c->H, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return H3D_OK;
 }
@@ -743,7 +919,7 @@ static int axpby_call(h3d_ctx *c, int op, int z, int x, int y, double a, double 
     if (!field_ok(z) || !field_ok(x) || !field_ok(y) || z == H3D_F_MASK || x == H3D_F_MASK || y == H3D_F_MASK)
         return fail(c, H3D_ERR_ARG, "bad field");
     Geo<double> g = base_geo<double>(c);
-    HIPCHK(c, launch_axpby<double>(op, c->f[z], c->f[x], c->f[y], nullptr, a, b, g, c->d_unit, c->stream));
+    HIPCHK(c, launch_axpby<double>(op, c->f[z], c->f[x], c->f[y], a, b, g, c->d_unit, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return H3D_OK;
 }
@@ -757,22 +933,23 @@ int h3d_bicg2(h3d_ctx *c, int z, int x, int y, double a, double b) { return axpb
 
 int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, double *ms)
 {
-    if (!c || !ms || reps < 1) return c ? fail(c, H3D_ERR_ARG, "bad arguments") : H3D_ERR_ARG;
+    if (!c) return H3D_ERR_ARG;
+    if (!ms || reps < 1) return fail(c, H3D_ERR_ARG, "bad arguments");
+    if (which < 0 || which > 2) return fail(c, H3D_ERR_ARG, "unknown kernel id");
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
-    double *x = c->f[H3D_F_PCG_P_], *b = c->f[H3D_F_PCG_P], *q = c->f[H3D_F_PCG_Q], *lam = c->f[H3D_F_LAMBDA];
+    double *x = c->f[H3D_F_PCG_P_], *b = c->f[H3D_F_PCG_P], *q = c->f[H3D_F_PCG_Q];
     auto once = [&]() -> hipError_t {
         if (which == 0) {
-            hipError_t e = launch_rbsor<double>(x, b, lam, g, 0, c->d_unit, c->stream);
-            return e != hipSuccess ? e : launch_rbsor<double>(x, b, lam, g, 1, c->d_unit, c->stream);
+            hipError_t e = launch_rbsor<double>(x, b, g, 0, c->d_unit, c->stream);
+            return e != hipSuccess ? e : launch_rbsor<double>(x, b, g, 1, c->d_unit, c->stream);
         }
         if (which == 1)
-            return launch_calc_ax<double>(q, x, lam, g, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
-        return launch_calc_ax<double>(q, x, lam, g, c->f[H3D_F_PCG_R0], nullptr, FIN_STORE, c->d_unit, nullptr, c->rb,
+            return launch_calc_ax<double>(q, x, g, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
+        return launch_calc_ax<double>(q, x, g, c->f[H3D_F_PCG_R0], nullptr, FIN_STORE, c->d_unit, nullptr, c->rb,
                                       c->stream);
     };
-    if (which < 0 || which > 2) return fail(c, H3D_ERR_ARG, "unknown kernel id");
     HIPCHK(c, once());
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < reps; ++r) HIPCHK(c, once());

@@ -18,9 +18,9 @@ enum : int {
     ST_NAN = 4,         // NaN/Inf residual (guard the reference lacks)
 };
 
-// Finalisers run by the last workgroup of a reducing kernel on the summed
-// partials s[0..2].  Each one restates one scalar step of PBiCGSTAB!
-// (src/NonUniform.jl:287-347) or CG! (src/NonUniform.jl:139-224).
+// Scalar steps run by the finalize kernel once the grid sums s[0..2] of the
+// preceding reducing kernel are known.  Each one restates one scalar step of
+// PBiCGSTAB! (src/NonUniform.jl:287-347) or CG! (src/NonUniform.jl:139-224).
 enum : int {
     FIN_STORE = 0,       // out[0..ns-1] = s
     FIN_RES0,            // res0 = sqrt(s0); converged at 0 if res0 == 0
@@ -42,18 +42,28 @@ struct KState {
     int itr, status, maxItr, pad;
 };
 
-// Scratch for cross-workgroup reductions: per-block partial sums (up to 3
-// per block, each a double-double hi/lo pair) and the arrival counter of the
-// last-block pattern.
+// Per-workgroup partial sums of a reducing kernel (up to three sums per
+// workgroup, each a double-double hi/lo pair), read by the finalize kernel.
 struct RedBuf {
     double *partials;       // [nblocks_max][3][2]
-    unsigned int *counter;  // reset to 0 by the last block
     int nblocks_max;
 };
 
 // Per-solve operator constants.  T is the storage/arithmetic type of the
 // stencil (double for the reference iteration, float for the FP32 inner
 // solve of the refinement loop).
+//
+// Conductivities reach the kernels in one of two equivalent forms:
+//   cls == nullptr : the λ field, harmonic means formed per face
+//                    (the reference's on-the-fly form);
+//   cls != nullptr : a one-byte class per cell (its index among the K
+//                    distinct λ values) and tab[m][a][b] = λf(v_a, v_b, m, 1)
+//                    for neighbour class a, centre class b, neighbour mask m.
+//                    Bitwise the same values, without the per-face division.
+// flux selects how A·x is formed: 0 = ss - dd·x0 (the reference's
+// expression, FP64); 1 = Σ c_n (x_n - x0) - (Σ conv + ap0)·x0, the same
+// operator without the cancellation that makes the reference form useless
+// in FP32 for smooth x (used by the FP32 inner solve only).
 template <typename T>
 struct Geo {
     int MX, MY, MZ;         // stored extents (with halo)
@@ -63,62 +73,85 @@ struct Geo {
     const T *ZC;            // device, MZ entries
     const T *dZ;            // device, MZ entries
     const T *ap0;           // device, per-cell ρ·cp·dx·dy·dz/Δt (transient); nullptr when steady
+    const T *lam;           // device λ field (direct form)
+    const uint8_t *cls;     // device class field (table form) or nullptr
+    const T *tab;           // device [2][K][K] λf table (table form)
+    int K;                  // number of classes (table form)
+    int flux;               // 1: flux form of A·x (FP32 inner operator)
 };
 
-// Launch helpers implemented in h3d_kernels.hip.  All take the stream and
-// return hipError_t of the launch.
-template <typename T> hipError_t launch_fill(T *a, T v, size_t n, hipStream_t s);
-template <typename T> hipError_t launch_copy_interior(T *dst, const T *src, const Geo<T> &g, const KState *st, hipStream_t s);
-template <typename T>
-hipError_t launch_calc_ax(T *ax, const T *x, const T *lam, const Geo<T> &g,
-                          const T *dot_y, const T *dot_z, int fin, KState *st, double *hist,
-                          const RedBuf &rb, hipStream_t s);
-template <typename T>
-hipError_t launch_calc_rk(T *r, T *r0copy, const T *th, const T *b, const T *lam, const Geo<T> &g,
-                          int fin, KState *st, double *hist, const RedBuf &rb, hipStream_t s);
-template <typename T>
-hipError_t launch_rbsor(T *x, const T *b, const T *lam, const Geo<T> &g, int color,
-                        const KState *st, hipStream_t s);
-template <typename T>
-hipError_t launch_dot(const T *x, const T *y, const T *z, const Geo<T> &g, int fin, KState *st,
-                      double *hist, const RedBuf &rb, hipStream_t s);
-template <typename T>
-hipError_t launch_axpby(int op, T *z, const T *x, const T *y, const T *w, double a, double b,
-                        const Geo<T> &g, const KState *st, hipStream_t s);
-template <typename T>
-hipError_t launch_bicg_p(T *p, const T *r, const T *q, const Geo<T> &g, const KState *st, hipStream_t s);
-template <typename T>
-hipError_t launch_bicg_s(T *s_, const T *q, const T *r, const Geo<T> &g, const KState *st, hipStream_t s);
-template <typename T>
-hipError_t launch_bicg_update(T *th, T *r, const T *p_, const T *s_, const T *t, const T *sv, const T *r0,
-                              const Geo<T> &g, KState *st, double *hist, const RedBuf &rb, hipStream_t s);
-template <typename T>
-hipError_t launch_cg_update(T *th, T *r, const T *p, const T *q, const Geo<T> &g, KState *st,
-                            double *hist, const RedBuf &rb, hipStream_t s);
-template <typename T>
-hipError_t launch_cg_p(T *p, const T *z, const Geo<T> &g, const KState *st, hipStream_t s);
+constexpr int kMaxClasses = 32;
 
-// axpby ops (interior only), src/NonUniform.jl:603-659
+// ---- launchers (h3d_kernels.hip); all return hipError_t of the launch -----
+template <typename T> hipError_t launch_fill(T *a, T v, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_copy_interior(T *dst, const T *src, const Geo<T> &g, const KState *st, hipStream_t s);
+
+// Reducing kernels write per-workgroup partials into rb; a second,
+// single-workgroup kernel sums them in a fixed order and runs finalizer fin.
+// CalcAX! (src/NonUniform.jl:454-511): ax = A x; with fin >= 0 also
+// s0 = Σ ax·(dot_y ? dot_y : ax), s1 = Σ ax·dot_z.
+template <typename T>
+hipError_t launch_calc_ax(T *ax, const T *x, const Geo<T> &g, const T *dot_y, const T *dot_z, int fin, KState *st,
+                          double *hist, const RedBuf &rb, hipStream_t s);
+// CalcRK! (src/NonUniform.jl:372-434): r = (b - Aθ)·m, s0 = Σ r², optional r0 copy
+template <typename T>
+hipError_t launch_calc_rk(T *r, T *r0copy, const T *th, const T *b, const Geo<T> &g, int fin, KState *st,
+                          double *hist, const RedBuf &rb, hipStream_t s);
+// rbsor_core! (src/NonUniform.jl:848-915), one colour, ω = 1
+template <typename T>
+hipError_t launch_rbsor(T *x, const T *b, const Geo<T> &g, int color, const KState *st, hipStream_t s);
+// Colour 0 of the first sweep of Preconditioner! from x = 0, fused with the
+// myfill!(x, 0) before it and with forming its right-hand side bb:
+//   FIRST_COPY   bb given (not written)
+//   FIRST_BICG_P bb = p: iteration 1 p = r, later p = r + β(p - ω q)   (BiCG1!)
+//   FIRST_BICG_S bb = s = (-α) q + r                                   (Triad!)
+enum : int { FIRST_COPY = 0, FIRST_BICG_P = 1, FIRST_BICG_S = 2 };
+template <typename T>
+hipError_t launch_rbsor_first(int mode, T *x, T *bb, const T *r, const T *q, const Geo<T> &g, const KState *st,
+                              hipStream_t s);
+// The right-hand side alone (mode FIRST_BICG_P or FIRST_BICG_S), for the
+// solve without a smoother
+template <typename T>
+hipError_t launch_form_rhs(int mode, T *bb, const T *r, const T *q, const Geo<T> &g, const KState *st,
+                           hipStream_t s);
+// Fdot1/Fdot2 (src/NonUniform.jl:567-592): s0 = Σ x·(y ? y : x), s1 = Σ x·z
+template <typename T>
+hipError_t launch_dot(const T *x, const T *y, const T *z, const Geo<T> &g, int fin, KState *st, double *hist,
+                      const RedBuf &rb, hipStream_t s);
+// BiCG1!/Triad!/BICG2! with host scalars (src/NonUniform.jl:603-659)
 enum : int {
     AX_BICG1 = 0,   // z = x + a*(z - b*y)          BiCG1!(p=z, r=x, q=y, beta=a, omg=b)
     AX_TRIAD = 1,   // z = a*x + y                  Triad!
     AX_BICG2 = 2,   // z = z + (a*x + b*y)          BICG2!
 };
+template <typename T>
+hipError_t launch_axpby(int op, T *z, const T *x, const T *y, double a, double b, const Geo<T> &g,
+                        const KState *st, hipStream_t s);
+// BICG2!(θ,p_,s_,α,ω); Triad!(r,t,s,-ω); Σ r²; with r0 also Σ r·r0
+template <typename T>
+hipError_t launch_bicg_update(double *th, T *r, const T *p_, const T *s_, const T *t, const T *sv, const T *r0,
+                              const Geo<T> &g, KState *st, double *hist, const RedBuf &rb, hipStream_t s);
+// CG!: θ += α p, r -= α q, Σ r²
+template <typename T>
+hipError_t launch_cg_update(double *th, T *r, const T *p, const T *q, const Geo<T> &g, KState *st, double *hist,
+                            const RedBuf &rb, hipStream_t s);
+// CG!: p = z + β p
+template <typename T>
+hipError_t launch_cg_p(T *p, const T *z, const Geo<T> &g, const KState *st, hipStream_t s);
 
-// boundary-condition set-up and RHS (h3d_kernels.hip)
-hipError_t launch_apply_bc_face(int face, int type, double temperature, double ambient,
-                                double *th, double *lam, double *rho, double *cp,
-                                int MX, int MY, int MZ, hipStream_t s);
-hipError_t launch_bc2rhs_face(int face, int type, double q, double h, double ta, double *b,
-                              const double *th, const Geo<double> &g, hipStream_t s);
-hipError_t launch_rhs_final(double *b, const double *th, const double *hsrc, const Geo<double> &g,
-                            hipStream_t s);
-hipError_t launch_ap0(double *ap0, const double *rho, const double *cp, const Geo<double> &g,
-                      double ddt, hipStream_t s);
-hipError_t launch_check_mask(const double *mask, int MX, int MY, int MZ, unsigned int *bad, hipStream_t s);
-template <typename D, typename S>
-hipError_t launch_convert(D *dst, const S *src, size_t n, hipStream_t s);
-// θ += d (FP64 += FP32) on the interior, refinement update
-hipError_t launch_add_correction(double *th, const float *d, const Geo<double> &g, hipStream_t s);
-// b32 = (float) r64 on the whole array
+// boundary conditions, RHS and set-up (FP64)
+hipError_t launch_apply_bc_face(int face, int type, double temperature, double ambient, double *th, double *lam,
+                                double *rho, double *cp, int MX, int MY, int MZ, hipStream_t s);
+hipError_t launch_bc2rhs_face(int face, int type, double q, double h, double ta, double *b, const double *th,
+                              const Geo<double> &g, hipStream_t s);
+hipError_t launch_rhs_final(double *b, const double *th, const double *hsrc, const Geo<double> &g, hipStream_t s);
+hipError_t launch_ap0(double *ap0, const double *rho, const double *cp, const Geo<double> &g, double ddt,
+                      hipStream_t s);
+// class of every cell: index of λ's bit pattern in the sorted list vals[0..K-1]
+hipError_t launch_classify(uint8_t *cls, const double *lam, const uint64_t *vals, int K, size_t n, hipStream_t s);
+template <typename D, typename S> hipError_t launch_convert(D *dst, const S *src, size_t n, hipStream_t s);
+// θ += d on the interior: the refinement update
+hipError_t launch_add_correction(double *th, const double *d, const Geo<double> &g, hipStream_t s);
+
 }  // namespace h3d

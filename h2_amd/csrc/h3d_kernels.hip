@@ -1,13 +1,13 @@
 // HIP kernels for the Heat3ds linear-solve path on gfx950 (MI355X).
 //
-// Every floating-point expression here performs the same IEEE operations,
-// in the same order, as the reference Julia code it cites (the library is
-// built with -ffp-contract=off, and FP64 division on gfx950 is correctly
-// rounded), so element-wise results are bitwise equal to the reference's.
-// Only sums over the grid differ: they run as fixed-order block trees (wave
-// butterfly, then LDS, then a last-workgroup pass over per-block partials),
-// which are deterministic run to run but not the reference's sequential
-// order.
+// Every floating-point expression of the FP64 path performs the same IEEE
+// operations, in the same order, as the reference Julia code it cites (the
+// library is built with -ffp-contract=off, and FP64 division and square
+// root on gfx950 are correctly rounded), so element-wise results are
+// bitwise the reference's.  Grid sums differ from the reference's
+// sequential running sums: they are accumulated in double-double through a
+// fixed reduction tree and rounded once, which yields the exactly rounded
+// sum independent of the launch decomposition.
 //
 // The mask array of the reference is implicit: after
 // apply_boundary_conditions! it is 0 on every halo cell and 1 inside, so for
@@ -15,8 +15,8 @@
 //
 // Work decomposition: a workgroup is 64 (x) by 4 (y) threads; each thread
 // owns one cell column (i,j) of the interior and marches over KC planes in
-// z.  Red-black sweeps give each thread the cells of one colour in a row
-// pair i = i0 + 2t.
+// z.  Red-black sweeps give each thread the cells of one colour in a row,
+// i = i0 + 2t.
 
 #include "h3d_internal.hpp"
 
@@ -31,71 +31,133 @@ constexpr int NWAVE = NT / 64;
 constexpr double kFloatMin = 1.0e-37;  // src/common.jl:12
 
 // ---------------------------------------------------------------------------
-// stencil (src/NonUniform.jl:477-507 and the identical blocks in CalcRK! and
-// rbsor_core!).  (i,j,k) are 0-based interior indices, 1..N.
+// conductances (src/NonUniform.jl:477-503 and the identical blocks of
+// CalcRK! and rbsor_core!).  (i,j,k) are 0-based interior indices, 1..N.
 // ---------------------------------------------------------------------------
 
 // λf(a, b, ma, mb) (src/common.jl:35) with mb = m0 = 1: the factor
 // 2 - div(ma+1, 2) is 1 for an interior neighbour and 2 for a halo one.
 template <typename T>
-__device__ __forceinline__ T lamf(T a, T b, T ma)
+__host__ __device__ __forceinline__ T lamf(T a, T b, T ma)
 {
     const T f = (ma == T(1)) ? T(1) : T(2);
     return T(2) * a * b / (a + b) * f;
 }
 
 template <typename T>
-struct Sten {
-    T dd, ss;
+struct Cond {
+    T xm, xp, ym, yp, zm, zp;            // cond_* (conduction, area/distance included)
+    T cxm, cxp, cym, cyp, czm, czp;      // conv_* (convective boundary terms)
+    T ap0;                               // a_p_0 (transient)
 };
 
-template <typename T>
-__device__ __forceinline__ Sten<T> stencil(const T *__restrict__ x, const T *__restrict__ lam,
-                                           const Geo<T> &g, int i, int j, int k, size_t c)
+// stab: the λf table staged in LDS (table form), unused in the direct form
+template <typename T, bool CLS>
+__device__ __forceinline__ Cond<T> conductances(const Geo<T> &g, const T *stab, int i, int j, int k, size_t c)
 {
     const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
     const T dz_k = g.dZ[k];
-    const T l0 = lam[c];
-    const T mxm = (i > 1) ? T(1) : T(0), mxp = (i < g.NX) ? T(1) : T(0);
-    const T mym = (j > 1) ? T(1) : T(0), myp = (j < g.NY) ? T(1) : T(0);
-    const T mzm = (k > 1) ? T(1) : T(0), mzp = (k < g.NZ) ? T(1) : T(0);
+    const bool ixm = i > 1, ixp = i < g.NX, iym = j > 1, iyp = j < g.NY, izm = k > 1, izp = k < g.NZ;
+    T fxm, fxp, fym, fyp, fzm, fzp;
+    if (CLS) {
+        const int K = g.K;
+        const int b = g.cls[c];
+        fxm = stab[((ixm ? K : 0) + g.cls[c - 1]) * K + b];
+        fxp = stab[((ixp ? K : 0) + g.cls[c + 1]) * K + b];
+        fym = stab[((iym ? K : 0) + g.cls[c - sy]) * K + b];
+        fyp = stab[((iyp ? K : 0) + g.cls[c + sy]) * K + b];
+        fzm = stab[((izm ? K : 0) + g.cls[c - sz]) * K + b];
+        fzp = stab[((izp ? K : 0) + g.cls[c + sz]) * K + b];
+    } else {
+        const T *lam = g.lam;
+        const T l0 = lam[c];
+        fxm = lamf(lam[c - 1], l0, ixm ? T(1) : T(0));
+        fxp = lamf(lam[c + 1], l0, ixp ? T(1) : T(0));
+        fym = lamf(lam[c - sy], l0, iym ? T(1) : T(0));
+        fyp = lamf(lam[c + sy], l0, iyp ? T(1) : T(0));
+        fzm = lamf(lam[c - sz], l0, izm ? T(1) : T(0));
+        fzp = lamf(lam[c + sz], l0, izp ? T(1) : T(0));
+    }
+    Cond<T> o;
+    o.xm = fxm * g.dy0 * dz_k * g.ddx;
+    o.xp = fxp * g.dy0 * dz_k * g.ddx;
+    o.ym = fym * g.dx0 * dz_k * g.ddy;
+    o.yp = fyp * g.dx0 * dz_k * g.ddy;
+    o.zm = fzm * g.dx0 * g.dy0 / (g.ZC[k] - g.ZC[k - 1]);
+    o.zp = fzp * g.dx0 * g.dy0 / (g.ZC[k + 1] - g.ZC[k]);
+    o.cxm = g.HC[0] * g.dy0 * dz_k * (ixm ? T(0) : T(1));
+    o.cxp = g.HC[1] * g.dy0 * dz_k * (ixp ? T(0) : T(1));
+    o.cym = g.HC[2] * g.dx0 * dz_k * (iym ? T(0) : T(1));
+    o.cyp = g.HC[3] * g.dx0 * dz_k * (iyp ? T(0) : T(1));
+    o.czm = g.HC[4] * g.dx0 * g.dy0 * (izm ? T(0) : T(1));
+    o.czp = g.HC[5] * g.dx0 * g.dy0 * (izp ? T(0) : T(1));
+    o.ap0 = g.ap0 ? g.ap0[c] : T(0);
+    return o;
+}
 
-    const T cond_xm = lamf(lam[c - 1], l0, mxm) * g.dy0 * dz_k * g.ddx;
-    const T cond_xp = lamf(lam[c + 1], l0, mxp) * g.dy0 * dz_k * g.ddx;
-    const T cond_ym = lamf(lam[c - sy], l0, mym) * g.dx0 * dz_k * g.ddy;
-    const T cond_yp = lamf(lam[c + sy], l0, myp) * g.dx0 * dz_k * g.ddy;
-    const T cond_zm = lamf(lam[c - sz], l0, mzm) * g.dx0 * g.dy0 / (g.ZC[k] - g.ZC[k - 1]);
-    const T cond_zp = lamf(lam[c + sz], l0, mzp) * g.dx0 * g.dy0 / (g.ZC[k + 1] - g.ZC[k]);
-
-    const T conv_xm = g.HC[0] * g.dy0 * dz_k * (T(1) - mxm);
-    const T conv_xp = g.HC[1] * g.dy0 * dz_k * (T(1) - mxp);
-    const T conv_ym = g.HC[2] * g.dx0 * dz_k * (T(1) - mym);
-    const T conv_yp = g.HC[3] * g.dx0 * dz_k * (T(1) - myp);
-    const T conv_zm = g.HC[4] * g.dx0 * g.dy0 * (T(1) - mzm);
-    const T conv_zp = g.HC[5] * g.dx0 * g.dy0 * (T(1) - mzp);
-
-    const T a_p_0 = g.ap0 ? g.ap0[c] : T(0);
-
-    Sten<T> s;
-    s.dd = (T(1) - T(1)) + (cond_xp + cond_xm + cond_yp + cond_ym + cond_zp + cond_zm + conv_xp +
-                            conv_xm + conv_yp + conv_ym + conv_zp + conv_zm + a_p_0) *
+// dd = (1-m0) + (Σcond + Σconv + a_p_0)·m0 in the reference's order
+template <typename T>
+__device__ __forceinline__ T diag(const Cond<T> &o)
+{
+    return (T(1) - T(1)) + (o.xp + o.xm + o.yp + o.ym + o.zp + o.zm + o.cxp + o.cxm + o.cyp + o.cym + o.czp +
+                            o.czm + o.ap0) *
                                T(1);
-    s.ss = (cond_xp * x[c + 1] + cond_xm * x[c - 1] + cond_yp * x[c + sy] + cond_ym * x[c - sy] +
-            cond_zp * x[c + sz] + cond_zm * x[c - sz]);
-    return s;
+}
+
+// ss = Σ cond_n x_n in the reference's order
+template <typename T>
+__device__ __forceinline__ T offdiag(const Cond<T> &o, const T *x, size_t c, size_t sy, size_t sz)
+{
+    return (o.xp * x[c + 1] + o.xm * x[c - 1] + o.yp * x[c + sy] + o.ym * x[c - sy] + o.zp * x[c + sz] +
+            o.zm * x[c - sz]);
+}
+
+// A·x at c in the flux form: Σ cond_n (x_n - x0) - (Σconv + a_p_0)·x0
+template <typename T>
+__device__ __forceinline__ T flux_ax(const Cond<T> &o, const T *x, size_t c, size_t sy, size_t sz)
+{
+    const T x0 = x[c];
+    const T f = o.xp * (x[c + 1] - x0) + o.xm * (x[c - 1] - x0) + o.yp * (x[c + sy] - x0) +
+                o.ym * (x[c - sy] - x0) + o.zp * (x[c + sz] - x0) + o.zm * (x[c - sz] - x0);
+    const T sink = o.cxp + o.cxm + o.cyp + o.cym + o.czp + o.czm + o.ap0;
+    return f - sink * x0;
+}
+
+// A·x at c (CalcAX!, NonUniform.jl:509), in the form g.flux selects
+template <typename T>
+__device__ __forceinline__ T apply_a(const Geo<T> &g, const Cond<T> &o, const T *x, size_t c, size_t sy, size_t sz)
+{
+    if (g.flux) return flux_ax(o, x, c, sy, sz);
+    return (offdiag(o, x, c, sy, sz) - diag(o) * x[c]) * T(1);
+}
+
+// Stage the λf table of the table form in LDS (all threads must call).
+template <typename T, bool CLS>
+__device__ __forceinline__ const T *stage_table(const Geo<T> &g)
+{
+    extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
+    if (!CLS) return nullptr;
+    T *t = reinterpret_cast<T *>(h3d_dyn_smem);
+    const int n = 2 * g.K * g.K;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    for (int a = tid; a < n; a += blockDim.x * blockDim.y) t[a] = g.tab[a];
+    __syncthreads();
+    return t;
+}
+
+template <typename T>
+static size_t table_bytes(const Geo<T> &g)
+{
+    return g.cls ? (size_t)2 * g.K * g.K * sizeof(T) : 0;
 }
 
 // ---------------------------------------------------------------------------
-// reductions: fixed-order block tree + last-workgroup finaliser
+// grid sums: double-double through a fixed reduction tree
 // ---------------------------------------------------------------------------
-
-// Grid sums are accumulated in double-double (an unevaluated hi + lo pair,
-// error-free TwoSum additions) through every level of the reduction tree,
-// then rounded once.  With at most a few dozen terms per thread and a tree
-// above that, the pair carries the sum to far below one ulp of the result,
-// so the rounded value is the exactly rounded sum of the terms independent
-// of how the grid is split into threads and workgroups; it is not the
-// reference's sequential running sum, whose rounding depends on the order.
+// An unevaluated hi + lo pair with error-free TwoSum additions.  Each thread
+// adds at most a few dozen terms, the tree above adds pairs; the pair keeps
+// the sum to far below one ulp of the result, so the final rounding gives
+// the exactly rounded sum of the terms whatever the decomposition.
 struct dd {
     double hi, lo;
 };
@@ -124,7 +186,7 @@ __device__ __forceinline__ dd dd_add(dd x, dd y)
 {
     const dd s = two_sum(x.hi, y.hi);
     const dd t = two_sum(x.lo, y.lo);
-    dd u = quick_two_sum(s.hi, s.lo + t.hi);
+    const dd u = quick_two_sum(s.hi, s.lo + t.hi);
     return quick_two_sum(u.hi, u.lo + t.lo);
 }
 
@@ -140,13 +202,36 @@ __device__ __forceinline__ dd wave_sum(dd v)
     return v;
 }
 
-__device__ __forceinline__ double ld_coherent(const double *p)
+// Reduce up to three per-thread sums over the workgroup and store the
+// workgroup's partials.  Every thread must call it.
+template <int NS>
+__device__ __forceinline__ void block_partials(const dd (&v)[3], const RedBuf &rb)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ dd sh[3][NWAVE];
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const dd w = wave_sum(v[s]);
+        if (lane == 0) sh[s][wave] = w;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            dd b = sh[s][0];
+#pragma unroll
+            for (int w = 1; w < NWAVE; ++w) b = dd_add(b, sh[s][w]);
+            rb.partials[6 * (size_t)bid + 2 * s] = b.hi;
+            rb.partials[6 * (size_t)bid + 2 * s + 1] = b.lo;
+        }
+    }
 }
 
-// Scalar steps of the Krylov recurrences, run by a single thread once the
-// sums s[] over the whole interior are known.
+// ---------------------------------------------------------------------------
+// scalar steps (one thread, after the sums are known)
+// ---------------------------------------------------------------------------
 __device__ void fin_res0(double s0, KState *st)  // CalcRK! result; NonUniform.jl:263-272
 {
     const double r0 = sqrt(s0);
@@ -252,50 +337,24 @@ __device__ void finalize(int fin, const double *s, KState *st, double *hist)
     }
 }
 
-// Block-reduce up to three per-thread sums, publish the block's partials and
-// let the last block to arrive reduce all partials and run the finaliser.
-// Every thread of the block must call this.
+__device__ __forceinline__ bool halted(const KState *st) { return st != nullptr && st->status != ST_RUNNING; }
+
+// Sum nb workgroup partials in a fixed order and run the finaliser.  One
+// workgroup of NT threads.
 template <int NS>
-__device__ void block_finish(const dd (&v)[3], int fin, KState *st, double *hist, const RedBuf &rb)
+__global__ void __launch_bounds__(256) k_finalize(const double *partials, unsigned nb, int fin, KState *st,
+                                                  double *hist)
 {
+    if (halted(st)) return;
     __shared__ dd sh[3][NWAVE];
-    __shared__ int am_last;
-    const int tid = threadIdx.x + BX * threadIdx.y;
+    const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const dd w = wave_sum(v[s]);
-        if (lane == 0) sh[s][wave] = w;
-    }
-    __syncthreads();
-    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if (tid == 0) {
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            dd b = sh[s][0];
-            for (int w = 1; w < NWAVE; ++w) b = dd_add(b, sh[s][w]);
-            rb.partials[6 * (size_t)bid + 2 * s] = b.hi;
-            rb.partials[6 * (size_t)bid + 2 * s + 1] = b.lo;
-        }
-        __threadfence();
-        const unsigned prev = atomicAdd(rb.counter, 1u);
-        am_last = (prev == nb - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
     dd acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     for (unsigned b = tid; b < nb; b += NT) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            dd p;
-            p.hi = ld_coherent(&rb.partials[6 * (size_t)b + 2 * s]);
-            p.lo = ld_coherent(&rb.partials[6 * (size_t)b + 2 * s + 1]);
-            acc[s] = dd_add(acc[s], p);
-        }
+        for (int s = 0; s < NS; ++s)
+            acc[s] = dd_add(acc[s], dd{partials[6 * (size_t)b + 2 * s], partials[6 * (size_t)b + 2 * s + 1]});
     }
-    __syncthreads();
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const dd w = wave_sum(acc[s]);
@@ -307,15 +366,13 @@ __device__ void block_finish(const dd (&v)[3], int fin, KState *st, double *hist
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             dd t = sh[s][0];
+#pragma unroll
             for (int w = 1; w < NWAVE; ++w) t = dd_add(t, sh[s][w]);
             tot[s] = t.hi + t.lo;
         }
         finalize(fin, tot, st, hist);
-        __hip_atomic_store(rb.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
-
-__device__ __forceinline__ bool halted(const KState *st) { return st != nullptr && st->status != ST_RUNNING; }
 
 // ---------------------------------------------------------------------------
 // launch geometry
@@ -323,6 +380,7 @@ __device__ __forceinline__ bool halted(const KState *st) { return st != nullptr 
 struct Shape {
     dim3 grid, block;
     int KC;
+    unsigned nb() const { return grid.x * grid.y * grid.z; }
 };
 
 static Shape interior_shape(int nx_threads, int NY, int NZ)
@@ -340,6 +398,25 @@ static Shape interior_shape(int nx_threads, int NY, int NZ)
     sh.KC = kc;
     return sh;
 }
+
+template <int NS>
+static hipError_t launch_finalize(const Shape &sh, int fin, KState *st, double *hist, const RedBuf &rb,
+                                  hipStream_t s)
+{
+    if (sh.nb() > (unsigned)rb.nblocks_max) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_finalize<NS>, dim3(1), dim3(NT), 0, s, rb.partials, sh.nb(), fin, st, hist);
+    return hipGetLastError();
+}
+
+#define INTERIOR_COLUMN(g, KC)                                              \
+    const int i = blockIdx.x * BX + threadIdx.x + 1;                        \
+    const int j = blockIdx.y * BY + threadIdx.y + 1;                        \
+    const bool active = (i <= (g).NX) && (j <= (g).NY);                     \
+    const int kb = 1 + (int)blockIdx.z * (KC);                              \
+    const int ke = min(kb + (KC), (g).NZ + 1);                              \
+    const size_t sy_ = (size_t)(g).MX;                                      \
+    const size_t sz_ = (size_t)(g).MX * (size_t)(g).MY;                     \
+    const size_t c0_ = (size_t)i + sy_ * (size_t)j;
 
 // ---------------------------------------------------------------------------
 // element-wise kernels
@@ -359,15 +436,6 @@ hipError_t launch_fill(T *a, T v, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-#define INTERIOR_COLUMN(g, KC)                                              \
-    const int i = blockIdx.x * BX + threadIdx.x + 1;                        \
-    const int j = blockIdx.y * BY + threadIdx.y + 1;                        \
-    const bool active = (i <= (g).NX) && (j <= (g).NY);                     \
-    const int kb = 1 + (int)blockIdx.z * (KC);                              \
-    const int ke = min(kb + (KC), (g).NZ + 1);                              \
-    const size_t sz_ = (size_t)(g).MX * (size_t)(g).MY;                     \
-    const size_t c0_ = (size_t)i + (size_t)(g).MX * (size_t)j;
-
 template <typename T>
 __global__ void __launch_bounds__(256) k_copy_interior(T *dst, const T *src, Geo<T> g, const KState *st, int KC)
 {
@@ -385,112 +453,260 @@ hipError_t launch_copy_interior(T *dst, const T *src, const Geo<T> &g, const KSt
     return hipGetLastError();
 }
 
-// CalcAX! (src/NonUniform.jl:454-511), optionally fused with one or two dot
-// products of the result: s0 = Σ ax·(y ? y : ax), s1 = Σ ax·z.
-template <typename T, int NS>
-__global__ void __launch_bounds__(256) k_calc_ax(T *ax, const T *x, const T *lam, Geo<T> g, const T *dy,
-                                                 const T *dz, int fin, KState *st, double *hist, RedBuf rb, int KC)
+// CalcAX! (src/NonUniform.jl:454-511): ax = (ss - dd·x0)·m0, optionally
+// with one or two dot products of the result.
+template <typename T, int NS, bool CLS>
+__global__ void __launch_bounds__(256) k_calc_ax(T *ax, const T *x, Geo<T> g, const T *dy, const T *dz, KState *st,
+                                                 RedBuf rb, int KC)
 {
     if (halted(st)) return;
+    const T *stab = stage_table<T, CLS>(g);
     INTERIOR_COLUMN(g, KC)
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     if (active) {
         for (int k = kb; k < ke; ++k) {
             const size_t c = c0_ + sz_ * k;
-            const Sten<T> s = stencil(x, lam, g, i, j, k, c);
-            const T a = (s.ss - s.dd * x[c]) * T(1);
+            const Cond<T> o = conductances<T, CLS>(g, stab, i, j, k, c);
+            const T a = apply_a(g, o, x, c, sy_, sz_);
             ax[c] = a;
             if (NS >= 1) v[0] = dd_add(v[0], (double)a * (double)(dy ? dy[c] : a));
             if (NS >= 2) v[1] = dd_add(v[1], (double)a * (double)dz[c]);
         }
     }
-    if (NS > 0) block_finish<NS>(v, fin, st, hist, rb);
+    if (NS > 0) block_partials<NS>(v, rb);
 }
 
 template <typename T>
-hipError_t launch_calc_ax(T *ax, const T *x, const T *lam, const Geo<T> &g, const T *dot_y, const T *dot_z,
-                          int fin, KState *st, double *hist, const RedBuf &rb, hipStream_t s)
+hipError_t launch_calc_ax(T *ax, const T *x, const Geo<T> &g, const T *dot_y, const T *dot_z, int fin, KState *st,
+                          double *hist, const RedBuf &rb, hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    if (fin < 0)
-        hipLaunchKernelGGL((k_calc_ax<T, 0>), sh.grid, sh.block, 0, s, ax, x, lam, g, dot_y, dot_z, fin, st, hist, rb, sh.KC);
-    else if (dot_z == nullptr)
-        hipLaunchKernelGGL((k_calc_ax<T, 1>), sh.grid, sh.block, 0, s, ax, x, lam, g, dot_y, dot_z, fin, st, hist, rb, sh.KC);
-    else
-        hipLaunchKernelGGL((k_calc_ax<T, 2>), sh.grid, sh.block, 0, s, ax, x, lam, g, dot_y, dot_z, fin, st, hist, rb, sh.KC);
-    return hipGetLastError();
+    const bool cls = g.cls != nullptr;
+    const size_t lds = table_bytes(g);
+    const int ns = fin < 0 ? 0 : (dot_z ? 2 : 1);
+#define AX_LAUNCH(NS, CLS)                                                                                      \
+    hipLaunchKernelGGL((k_calc_ax<T, NS, CLS>), sh.grid, sh.block, lds, s, ax, x, g, dot_y, dot_z, st, rb, sh.KC)
+    if (ns == 0) { if (cls) AX_LAUNCH(0, true); else AX_LAUNCH(0, false); }
+    else if (ns == 1) { if (cls) AX_LAUNCH(1, true); else AX_LAUNCH(1, false); }
+    else { if (cls) AX_LAUNCH(2, true); else AX_LAUNCH(2, false); }
+#undef AX_LAUNCH
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || ns == 0) return e;
+    return ns == 1 ? launch_finalize<1>(sh, fin, st, hist, rb, s) : launch_finalize<2>(sh, fin, st, hist, rb, s);
 }
 
-// CalcRK! (src/NonUniform.jl:372-434): r = (b - Aθ)·m and Σ r².  With
+// CalcRK! (src/NonUniform.jl:372-434): r = (b - A θ)·m and Σ r².  With
 // r0copy it also performs PBiCGSTAB!'s mycopy!(r0, r) (NonUniform.jl:274).
-template <typename T>
-__global__ void __launch_bounds__(256) k_calc_rk(T *r, T *r0copy, const T *th, const T *b, const T *lam, Geo<T> g,
-                                                 int fin, KState *st, double *hist, RedBuf rb, int KC)
+template <typename T, bool CLS>
+__global__ void __launch_bounds__(256) k_calc_rk(T *r, T *r0copy, const T *th, const T *b, Geo<T> g, KState *st,
+                                                 RedBuf rb, int KC)
 {
     if (halted(st)) return;
+    const T *stab = stage_table<T, CLS>(g);
     INTERIOR_COLUMN(g, KC)
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     if (active) {
         for (int k = kb; k < ke; ++k) {
             const size_t c = c0_ + sz_ * k;
-            const Sten<T> s = stencil(th, lam, g, i, j, k, c);
-            const T rs = (b[c] - (s.ss - s.dd * th[c])) * T(1);
+            const Cond<T> o = conductances<T, CLS>(g, stab, i, j, k, c);
+            T rs;
+            if (g.flux) rs = b[c] - flux_ax(o, th, c, sy_, sz_);
+            else rs = (b[c] - (offdiag(o, th, c, sy_, sz_) - diag(o) * th[c])) * T(1);
             r[c] = rs;
             if (r0copy) r0copy[c] = rs;
             v[0] = dd_add(v[0], (double)rs * (double)rs);
         }
     }
-    block_finish<1>(v, fin, st, hist, rb);
+    block_partials<1>(v, rb);
 }
 
 template <typename T>
-hipError_t launch_calc_rk(T *r, T *r0copy, const T *th, const T *b, const T *lam, const Geo<T> &g, int fin,
-                          KState *st, double *hist, const RedBuf &rb, hipStream_t s)
+hipError_t launch_calc_rk(T *r, T *r0copy, const T *th, const T *b, const Geo<T> &g, int fin, KState *st,
+                          double *hist, const RedBuf &rb, hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    hipLaunchKernelGGL(k_calc_rk<T>, sh.grid, sh.block, 0, s, r, r0copy, th, b, lam, g, fin, st, hist, rb, sh.KC);
-    return hipGetLastError();
+    if (g.cls)
+        hipLaunchKernelGGL((k_calc_rk<T, true>), sh.grid, sh.block, table_bytes(g), s, r, r0copy, th, b, g, st, rb,
+                           sh.KC);
+    else
+        hipLaunchKernelGGL((k_calc_rk<T, false>), sh.grid, sh.block, 0, s, r, r0copy, th, b, g, st, rb, sh.KC);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
+}
+
+// Gauss-Seidel update of one cell, ω = 1 (rbsor_core!, NonUniform.jl:907-908):
+// dp = ((ss - b)/dd - pp)·m0; x = pp + ω·dp.  The flux form computes the
+// same correction as dp = ((A x)_c - b)/dd.
+template <typename T>
+__device__ __forceinline__ T gs_update(const Geo<T> &g, const Cond<T> &o, const T *x, T bc, size_t c, size_t sy,
+                                       size_t sz)
+{
+    const T pp = x[c];
+    const T dd = diag(o);
+    if (g.flux) return pp + (flux_ax(o, x, c, sy, sz) - bc) / dd;
+    const T dp = (((offdiag(o, x, c, sy, sz) - bc) / dd - pp)) * T(1);
+    return pp + T(1) * dp;
 }
 
 // rbsor_core! (src/NonUniform.jl:848-915) with ω = 1: one colour of
 // Gauss-Seidel in place.  Colour c updates 0-based (i+j+k+c) odd, i.e. the
 // reference's i in 2+mod(k+j+c,2):2:MX-1.  The residual sum the reference
 // returns is discarded by Preconditioner! and is not formed.
-template <typename T>
-__global__ void __launch_bounds__(256) k_rbsor(T *x, const T *b, const T *lam, Geo<T> g, int color,
-                                               const KState *st, int KC)
+template <typename T, bool CLS>
+__global__ void __launch_bounds__(256) k_rbsor(T *x, const T *b, Geo<T> g, int color, const KState *st, int KC)
 {
     if (halted(st)) return;
+    const T *stab = stage_table<T, CLS>(g);
     const int t = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
     if (j > g.NY) return;
     const int kb = 1 + (int)blockIdx.z * KC;
     const int ke = min(kb + KC, g.NZ + 1);
-    const size_t sz = (size_t)g.MX * (size_t)g.MY;
+    const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
     for (int k = kb; k < ke; ++k) {
         const int i = (((j + k + color) & 1) ? 2 : 1) + 2 * t;
         if (i > g.NX) continue;
-        const size_t c = (size_t)i + (size_t)g.MX * (size_t)j + sz * (size_t)k;
-        const T pp = x[c];
-        const Sten<T> s = stencil(x, lam, g, i, j, k, c);
-        const T dp = (((s.ss - b[c]) / s.dd - pp)) * T(1);
-        x[c] = pp + T(1) * dp;
+        const size_t c = (size_t)i + sy * (size_t)j + sz * (size_t)k;
+        const Cond<T> o = conductances<T, CLS>(g, stab, i, j, k, c);
+        x[c] = gs_update(g, o, x, b[c], c, sy, sz);
     }
 }
 
 template <typename T>
-hipError_t launch_rbsor(T *x, const T *b, const T *lam, const Geo<T> &g, int color, const KState *st, hipStream_t s)
+hipError_t launch_rbsor(T *x, const T *b, const Geo<T> &g, int color, const KState *st, hipStream_t s)
 {
     Shape sh = interior_shape((g.NX + 1) / 2, g.NY, g.NZ);
-    hipLaunchKernelGGL(k_rbsor<T>, sh.grid, sh.block, 0, s, x, b, lam, g, color, st, sh.KC);
+    if (g.cls)
+        hipLaunchKernelGGL((k_rbsor<T, true>), sh.grid, sh.block, table_bytes(g), s, x, b, g, color, st, sh.KC);
+    else
+        hipLaunchKernelGGL((k_rbsor<T, false>), sh.grid, sh.block, 0, s, x, b, g, color, st, sh.KC);
     return hipGetLastError();
 }
 
-// Fdot1/Fdot2 (src/NonUniform.jl:567-592), up to three dots in one pass:
-// s0 = Σ x·y (y == nullptr: Σ x²), s1 = Σ x·z.
+// Colour 0 of the first sweep of Preconditioner! (NonUniform.jl:553-558)
+// starting from the zero vector the callers' myfill!(x, 0) leaves
+// (NonUniform.jl:303, 313; :147, 201).  A thread owns the cell pair
+// (2t+1, 2t+2) of a row: it forms the right-hand side of both cells (BiCG1!
+// or Triad! when mode asks), writes 0 to the colour-1 cell and the Gauss-
+// Seidel value to the colour-0 cell, computed from neighbours that are all
+// zero (ss = Σ cond·0, pp = 0), exactly as the reference evaluates it.
+template <typename T, bool CLS>
+__global__ void __launch_bounds__(256) k_rbsor_first(int mode, T *x, T *bb, const T *r, const T *q, Geo<T> g,
+                                                     const KState *st, int KC)
+{
+    if (halted(st)) return;
+    const T *stab = stage_table<T, CLS>(g);
+    const int t = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int i1 = 2 * t + 1;
+    if (j > g.NY || i1 > g.NX) return;
+    const bool two = i1 + 1 <= g.NX;
+    const int kb = 1 + (int)blockIdx.z * KC;
+    const int ke = min(kb + KC, g.NZ + 1);
+    const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
+    T beta = T(0), omg = T(0), r_alpha = T(0);
+    bool first_itr = false;
+    if (mode == FIRST_BICG_P) {
+        first_itr = (st->itr == 1);
+        beta = (T)st->beta;
+        omg = (T)st->omega;
+    } else if (mode == FIRST_BICG_S) {
+        r_alpha = (T)(-st->alpha);
+    }
+    for (int k = kb; k < ke; ++k) {
+        const size_t ca = (size_t)i1 + sy * (size_t)j + sz * (size_t)k;
+        T ba, bbv = T(0);
+        if (mode == FIRST_BICG_P) {
+            ba = first_itr ? r[ca] : r[ca] + beta * (bb[ca] - omg * q[ca]);
+            bb[ca] = ba;
+            if (two) {
+                bbv = first_itr ? r[ca + 1] : r[ca + 1] + beta * (bb[ca + 1] - omg * q[ca + 1]);
+                bb[ca + 1] = bbv;
+            }
+        } else if (mode == FIRST_BICG_S) {
+            ba = r_alpha * q[ca] + r[ca];
+            bb[ca] = ba;
+            if (two) {
+                bbv = r_alpha * q[ca + 1] + r[ca + 1];
+                bb[ca + 1] = bbv;
+            }
+        } else {
+            ba = bb[ca];
+            if (two) bbv = bb[ca + 1];
+        }
+        // colour 0 updates (i+j+k) odd: i1 (odd) when j+k is even
+        const bool first_is_red = ((j + k) & 1) == 0;
+        const int ir = first_is_red ? i1 : i1 + 1;
+        if (ir <= g.NX) {
+            const size_t cr = first_is_red ? ca : ca + 1;
+            const T br = first_is_red ? ba : bbv;
+            const Cond<T> o = conductances<T, CLS>(g, stab, ir, j, k, cr);
+            const T pp = T(0);
+            const T dd = diag(o);
+            T xn;
+            if (g.flux) {
+                xn = pp + (T(0) - br) / dd;
+            } else {
+                const T ss = (o.xp * T(0) + o.xm * T(0) + o.yp * T(0) + o.ym * T(0) + o.zp * T(0) + o.zm * T(0));
+                const T dp = (((ss - br) / dd - pp)) * T(1);
+                xn = pp + T(1) * dp;
+            }
+            x[cr] = xn;
+        }
+        // the colour-1 cell of the pair keeps the fill value
+        if (first_is_red) {
+            if (two) x[ca + 1] = T(0);
+        } else {
+            x[ca] = T(0);
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_rbsor_first(int mode, T *x, T *bb, const T *r, const T *q, const Geo<T> &g, const KState *st,
+                              hipStream_t s)
+{
+    Shape sh = interior_shape((g.NX + 1) / 2, g.NY, g.NZ);
+    if (g.cls)
+        hipLaunchKernelGGL((k_rbsor_first<T, true>), sh.grid, sh.block, table_bytes(g), s, mode, x, bb, r, q, g, st,
+                           sh.KC);
+    else
+        hipLaunchKernelGGL((k_rbsor_first<T, false>), sh.grid, sh.block, 0, s, mode, x, bb, r, q, g, st, sh.KC);
+    return hipGetLastError();
+}
+
+// The right-hand side of the preconditioner alone (no smoothing): BiCG1!
+// p = r + β(p - ω q) (iteration 1: p = r) or Triad! s = (-α) q + r.
+template <typename T>
+__global__ void __launch_bounds__(256) k_form_rhs(int mode, T *bb, const T *r, const T *q, Geo<T> g,
+                                                  const KState *st, int KC)
+{
+    if (halted(st)) return;
+    INTERIOR_COLUMN(g, KC)
+    if (!active) return;
+    const bool first_itr = (st->itr == 1);
+    const T beta = (T)st->beta, omg = (T)st->omega, r_alpha = (T)(-st->alpha);
+    for (int k = kb; k < ke; ++k) {
+        const size_t c = c0_ + sz_ * k;
+        if (mode == FIRST_BICG_P) bb[c] = first_itr ? r[c] : r[c] + beta * (bb[c] - omg * q[c]);
+        else bb[c] = r_alpha * q[c] + r[c];
+    }
+}
+
+template <typename T>
+hipError_t launch_form_rhs(int mode, T *bb, const T *r, const T *q, const Geo<T> &g, const KState *st,
+                           hipStream_t s)
+{
+    Shape sh = interior_shape(g.NX, g.NY, g.NZ);
+    hipLaunchKernelGGL(k_form_rhs<T>, sh.grid, sh.block, 0, s, mode, bb, r, q, g, st, sh.KC);
+    return hipGetLastError();
+}
+
+// Fdot1/Fdot2 (src/NonUniform.jl:567-592): s0 = Σ x·y (y == nullptr: Σ x²),
+// s1 = Σ x·z.
 template <typename T, int NS>
-__global__ void __launch_bounds__(256) k_dot(const T *x, const T *y, const T *z, Geo<T> g, int fin, KState *st,
-                                             double *hist, RedBuf rb, int KC)
+__global__ void __launch_bounds__(256) k_dot(const T *x, const T *y, const T *z, Geo<T> g, KState *st, RedBuf rb,
+                                             int KC)
 {
     if (halted(st)) return;
     INTERIOR_COLUMN(g, KC)
@@ -503,7 +719,7 @@ __global__ void __launch_bounds__(256) k_dot(const T *x, const T *y, const T *z,
             if (NS >= 2) v[1] = dd_add(v[1], xv * (double)z[c]);
         }
     }
-    block_finish<NS>(v, fin, st, hist, rb);
+    block_partials<NS>(v, rb);
 }
 
 template <typename T>
@@ -511,11 +727,15 @@ hipError_t launch_dot(const T *x, const T *y, const T *z, const Geo<T> &g, int f
                       const RedBuf &rb, hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    if (z == nullptr)
-        hipLaunchKernelGGL((k_dot<T, 1>), sh.grid, sh.block, 0, s, x, y, z, g, fin, st, hist, rb, sh.KC);
-    else
-        hipLaunchKernelGGL((k_dot<T, 2>), sh.grid, sh.block, 0, s, x, y, z, g, fin, st, hist, rb, sh.KC);
-    return hipGetLastError();
+    hipError_t e;
+    if (z == nullptr) {
+        hipLaunchKernelGGL((k_dot<T, 1>), sh.grid, sh.block, 0, s, x, y, z, g, st, rb, sh.KC);
+        e = hipGetLastError();
+        return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
+    }
+    hipLaunchKernelGGL((k_dot<T, 2>), sh.grid, sh.block, 0, s, x, y, z, g, st, rb, sh.KC);
+    e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize<2>(sh, fin, st, hist, rb, s);
 }
 
 // BiCG1! / Triad! / BICG2! (src/NonUniform.jl:603-659) with host scalars.
@@ -535,127 +755,88 @@ __global__ void __launch_bounds__(256) k_axpby(int op, T *z, const T *x, const T
 }
 
 template <typename T>
-hipError_t launch_axpby(int op, T *z, const T *x, const T *y, const T *w, double a, double b, const Geo<T> &g,
+hipError_t launch_axpby(int op, T *z, const T *x, const T *y, double a, double b, const Geo<T> &g,
                         const KState *st, hipStream_t s)
 {
-    (void)w;
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
     hipLaunchKernelGGL(k_axpby<T>, sh.grid, sh.block, 0, s, op, z, x, y, (T)a, (T)b, g, st, sh.KC);
     return hipGetLastError();
 }
 
-// PBiCGSTAB! p update with device scalars: iteration 1 mycopy!(p, r)
-// (NonUniform.jl:297), later BiCG1!(p, r, q, beta, omega) (:299-300).
-template <typename T>
-__global__ void __launch_bounds__(256) k_bicg_p(T *p, const T *r, const T *q, Geo<T> g, const KState *st, int KC)
-{
-    if (halted(st)) return;
-    INTERIOR_COLUMN(g, KC)
-    if (!active) return;
-    const bool first = (st->itr == 1);
-    const T beta = (T)st->beta, omg = (T)st->omega;
-    for (int k = kb; k < ke; ++k) {
-        const size_t c = c0_ + sz_ * k;
-        p[c] = first ? r[c] : r[c] + beta * (p[c] - omg * q[c]);
-    }
-}
-
-template <typename T>
-hipError_t launch_bicg_p(T *p, const T *r, const T *q, const Geo<T> &g, const KState *st, hipStream_t s)
-{
-    Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    hipLaunchKernelGGL(k_bicg_p<T>, sh.grid, sh.block, 0, s, p, r, q, g, st, sh.KC);
-    return hipGetLastError();
-}
-
-// s = (-alpha) q + r  (Triad!, NonUniform.jl:310-311)
-template <typename T>
-__global__ void __launch_bounds__(256) k_bicg_s(T *sv, const T *q, const T *r, Geo<T> g, const KState *st, int KC)
-{
-    if (halted(st)) return;
-    INTERIOR_COLUMN(g, KC)
-    if (!active) return;
-    const T r_alpha = (T)(-st->alpha);
-    for (int k = kb; k < ke; ++k) {
-        const size_t c = c0_ + sz_ * k;
-        sv[c] = r_alpha * q[c] + r[c];
-    }
-}
-
-template <typename T>
-hipError_t launch_bicg_s(T *s_, const T *q, const T *r, const Geo<T> &g, const KState *st, hipStream_t s)
-{
-    Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    hipLaunchKernelGGL(k_bicg_s<T>, sh.grid, sh.block, 0, s, s_, q, r, g, st, sh.KC);
-    return hipGetLastError();
-}
-
 // BICG2!(θ, p_, s_, alpha, omega); Triad!(r, t_, s, -omega); Σ r², and with
 // r0 also Σ r·r0 for the next iteration's ρ (NonUniform.jl:328-332, 289).
+// The iterate θ is always FP64: in the FP32 inner solve it accumulates the
+// correction, whose many cancelling updates FP32 storage would not resolve.
 template <typename T, int NS>
-__global__ void __launch_bounds__(256) k_bicg_update(T *th, T *r, const T *p_, const T *s_, const T *t, const T *sv,
-                                                     const T *r0, Geo<T> g, KState *st, double *hist, RedBuf rb,
-                                                     int fin, int KC)
+__global__ void __launch_bounds__(256) k_bicg_update(double *th, T *r, const T *p_, const T *s_, const T *t,
+                                                     const T *sv, const T *r0, Geo<T> g, KState *st, RedBuf rb,
+                                                     int KC)
 {
     if (halted(st)) return;
     INTERIOR_COLUMN(g, KC)
-    const T alpha = (T)st->alpha, omega = (T)st->omega, r_omega = (T)(-st->omega);
+    const double alpha = st->alpha, omega = st->omega;
+    const T r_omega = (T)(-st->omega);
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     if (active) {
         for (int k = kb; k < ke; ++k) {
             const size_t c = c0_ + sz_ * k;
-            th[c] = th[c] + (alpha * p_[c] + omega * s_[c]);
+            th[c] = th[c] + (alpha * (double)p_[c] + omega * (double)s_[c]);
             const T rv = r_omega * t[c] + sv[c];
             r[c] = rv;
             v[0] = dd_add(v[0], (double)rv * (double)rv);
             if (NS >= 2) v[1] = dd_add(v[1], (double)rv * (double)r0[c]);
         }
     }
-    block_finish<NS>(v, fin, st, hist, rb);
+    block_partials<NS>(v, rb);
 }
 
 template <typename T>
-hipError_t launch_bicg_update(T *th, T *r, const T *p_, const T *s_, const T *t, const T *sv, const T *r0,
+hipError_t launch_bicg_update(double *th, T *r, const T *p_, const T *s_, const T *t, const T *sv, const T *r0,
                               const Geo<T> &g, KState *st, double *hist, const RedBuf &rb, hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    if (r0 == nullptr)
-        hipLaunchKernelGGL((k_bicg_update<T, 1>), sh.grid, sh.block, 0, s, th, r, p_, s_, t, sv, r0, g, st, hist, rb,
-                           (int)FIN_BICG_RES, sh.KC);
-    else
-        hipLaunchKernelGGL((k_bicg_update<T, 2>), sh.grid, sh.block, 0, s, th, r, p_, s_, t, sv, r0, g, st, hist, rb,
-                           (int)FIN_BICG_RES_RHO, sh.KC);
-    return hipGetLastError();
+    hipError_t e;
+    if (r0 == nullptr) {
+        hipLaunchKernelGGL((k_bicg_update<T, 1>), sh.grid, sh.block, 0, s, th, r, p_, s_, t, sv, r0, g, st, rb,
+                           sh.KC);
+        e = hipGetLastError();
+        return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_BICG_RES, st, hist, rb, s);
+    }
+    hipLaunchKernelGGL((k_bicg_update<T, 2>), sh.grid, sh.block, 0, s, th, r, p_, s_, t, sv, r0, g, st, rb, sh.KC);
+    e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize<2>(sh, FIN_BICG_RES_RHO, st, hist, rb, s);
 }
 
 // CG!: θ += α p, r -= α q, Σ r² (NonUniform.jl:179-186)
 template <typename T>
-__global__ void __launch_bounds__(256) k_cg_update(T *th, T *r, const T *p, const T *q, Geo<T> g, KState *st,
-                                                   double *hist, RedBuf rb, int KC)
+__global__ void __launch_bounds__(256) k_cg_update(double *th, T *r, const T *p, const T *q, Geo<T> g, KState *st,
+                                                   RedBuf rb, int KC)
 {
     if (halted(st)) return;
     INTERIOR_COLUMN(g, KC)
+    const double alpha_d = st->alpha;
     const T alpha = (T)st->alpha;
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     if (active) {
         for (int k = kb; k < ke; ++k) {
             const size_t c = c0_ + sz_ * k;
-            th[c] = th[c] + alpha * p[c];
+            th[c] = th[c] + alpha_d * (double)p[c];
             const T rv = r[c] - alpha * q[c];
             r[c] = rv;
             v[0] = dd_add(v[0], (double)rv * (double)rv);
         }
     }
-    block_finish<1>(v, FIN_CG_RES, st, hist, rb);
+    block_partials<1>(v, rb);
 }
 
 template <typename T>
-hipError_t launch_cg_update(T *th, T *r, const T *p, const T *q, const Geo<T> &g, KState *st, double *hist,
+hipError_t launch_cg_update(double *th, T *r, const T *p, const T *q, const Geo<T> &g, KState *st, double *hist,
                             const RedBuf &rb, hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    hipLaunchKernelGGL(k_cg_update<T>, sh.grid, sh.block, 0, s, th, r, p, q, g, st, hist, rb, sh.KC);
-    return hipGetLastError();
+    hipLaunchKernelGGL(k_cg_update<T>, sh.grid, sh.block, 0, s, th, r, p, q, g, st, rb, sh.KC);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_CG_RES, st, hist, rb, s);
 }
 
 // CG!: p = z + β p (NonUniform.jl:218-220)
@@ -806,6 +987,29 @@ hipError_t launch_ap0(double *ap0, const double *rho, const double *cp, const Ge
     return hipGetLastError();
 }
 
+// class of a cell = index of its λ bit pattern in the sorted distinct list
+__global__ void __launch_bounds__(256) k_classify(uint8_t *cls, const double *lam, const uint64_t *vals, int K,
+                                                  size_t n)
+{
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t v = (uint64_t)__double_as_longlong(lam[c]);
+        int lo = 0, hi = K - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (vals[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        cls[c] = (uint8_t)lo;
+    }
+}
+
+hipError_t launch_classify(uint8_t *cls, const double *lam, const uint64_t *vals, int K, size_t n, hipStream_t s)
+{
+    const int nb = (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
+    hipLaunchKernelGGL(k_classify, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, cls, lam, vals, K, n);
+    return hipGetLastError();
+}
+
 template <typename D, typename S>
 __global__ void __launch_bounds__(256) k_convert(D *dst, const S *src, size_t n)
 {
@@ -821,17 +1025,17 @@ hipError_t launch_convert(D *dst, const S *src, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(256) k_add_correction(double *th, const float *d, Geo<double> g, int KC)
+__global__ void __launch_bounds__(256) k_add_correction(double *th, const double *d, Geo<double> g, int KC)
 {
     INTERIOR_COLUMN(g, KC)
     if (!active) return;
     for (int k = kb; k < ke; ++k) {
         const size_t c = c0_ + sz_ * k;
-        th[c] = th[c] + (double)d[c];
+        th[c] = th[c] + d[c];
     }
 }
 
-hipError_t launch_add_correction(double *th, const float *d, const Geo<double> &g, hipStream_t s)
+hipError_t launch_add_correction(double *th, const double *d, const Geo<double> &g, hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
     hipLaunchKernelGGL(k_add_correction, sh.grid, sh.block, 0, s, th, d, g, sh.KC);
@@ -842,21 +1046,22 @@ hipError_t launch_add_correction(double *th, const float *d, const Geo<double> &
 #define INST(T)                                                                                                   \
     template hipError_t launch_fill<T>(T *, T, size_t, hipStream_t);                                              \
     template hipError_t launch_copy_interior<T>(T *, const T *, const Geo<T> &, const KState *, hipStream_t);     \
-    template hipError_t launch_calc_ax<T>(T *, const T *, const T *, const Geo<T> &, const T *, const T *, int,   \
-                                          KState *, double *, const RedBuf &, hipStream_t);                      \
-    template hipError_t launch_calc_rk<T>(T *, T *, const T *, const T *, const T *, const Geo<T> &, int,        \
-                                          KState *, double *, const RedBuf &, hipStream_t);                      \
-    template hipError_t launch_rbsor<T>(T *, const T *, const T *, const Geo<T> &, int, const KState *,          \
-                                        hipStream_t);                                                            \
+    template hipError_t launch_calc_ax<T>(T *, const T *, const Geo<T> &, const T *, const T *, int, KState *,    \
+                                          double *, const RedBuf &, hipStream_t);                                \
+    template hipError_t launch_calc_rk<T>(T *, T *, const T *, const T *, const Geo<T> &, int, KState *,         \
+                                          double *, const RedBuf &, hipStream_t);                                \
+    template hipError_t launch_rbsor<T>(T *, const T *, const Geo<T> &, int, const KState *, hipStream_t);       \
+    template hipError_t launch_rbsor_first<T>(int, T *, T *, const T *, const T *, const Geo<T> &,               \
+                                              const KState *, hipStream_t);                                      \
+    template hipError_t launch_form_rhs<T>(int, T *, const T *, const T *, const Geo<T> &, const KState *,       \
+                                           hipStream_t);                                                         \
     template hipError_t launch_dot<T>(const T *, const T *, const T *, const Geo<T> &, int, KState *, double *,  \
                                       const RedBuf &, hipStream_t);                                              \
-    template hipError_t launch_axpby<T>(int, T *, const T *, const T *, const T *, double, double,               \
-                                        const Geo<T> &, const KState *, hipStream_t);                            \
-    template hipError_t launch_bicg_p<T>(T *, const T *, const T *, const Geo<T> &, const KState *, hipStream_t); \
-    template hipError_t launch_bicg_s<T>(T *, const T *, const T *, const Geo<T> &, const KState *, hipStream_t); \
-    template hipError_t launch_bicg_update<T>(T *, T *, const T *, const T *, const T *, const T *, const T *,   \
+    template hipError_t launch_axpby<T>(int, T *, const T *, const T *, double, double, const Geo<T> &,          \
+                                        const KState *, hipStream_t);                                            \
+    template hipError_t launch_bicg_update<T>(double *, T *, const T *, const T *, const T *, const T *, const T *, \
                                               const Geo<T> &, KState *, double *, const RedBuf &, hipStream_t);  \
-    template hipError_t launch_cg_update<T>(T *, T *, const T *, const T *, const Geo<T> &, KState *, double *,  \
+    template hipError_t launch_cg_update<T>(double *, T *, const T *, const T *, const Geo<T> &, KState *, double *, \
                                             const RedBuf &, hipStream_t);                                        \
     template hipError_t launch_cg_p<T>(T *, const T *, const Geo<T> &, const KState *, hipStream_t);
 INST(double)

@@ -241,6 +241,15 @@ class WorkBuffers:
     def bicg2(self, z, x, y, a, b):
         _check(self.ctx, self.lib.h3d_bicg2(self.ctx, self._fid(z), self._fid(x), self._fid(y), a, b))
 
+    def set_coef_form(self, form):
+        """L.COEF_AUTO (class tables when possible) or L.COEF_DIRECT."""
+        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_COEF_FORM, int(form)))
+
+    def num_classes(self):
+        v = ctypes.c_int()
+        _check(self.ctx, self.lib.h3d_get_option(self.ctx, L.OPT_NUM_CLASSES, ctypes.byref(v)))
+        return v.value
+
     def time_kernel(self, which, reps, dt=1.0, is_steady=True):
         ms = ctypes.c_double()
         _check(self.ctx, self.lib.h3d_time_kernel(self.ctx, int(which), int(reps), float(dt), int(bool(is_steady)),

@@ -77,6 +77,12 @@ extern "C" {
 #define H3D_F_HSRC 14
 #define H3D_NFIELDS 15
 
+/* options (h3d_set_option / h3d_get_option) */
+#define H3D_OPT_COEF_FORM 0    /* how the operator kernels obtain the face conductivities */
+#define H3D_COEF_AUTO 0        /* class tables when λ has at most 32 distinct values, else direct */
+#define H3D_COEF_DIRECT 1      /* always the reference's per-face harmonic mean of the λ field */
+#define H3D_OPT_NUM_CLASSES 1  /* read-only: distinct λ values in use by the table form (0 = direct) */
+
 typedef struct h3d_ctx h3d_ctx;
 
 /* WorkBuffers(mx,my,mz) (src/common.jl:71-89): all fields zero except
@@ -86,6 +92,13 @@ int h3d_destroy(h3d_ctx *ctx);
 const char *h3d_last_error(const h3d_ctx *ctx);
 /* message of the last h3d_create failure (no context exists then) */
 const char *h3d_create_error(void);
+
+/* Both coefficient forms give bitwise identical operators; the table form
+ * (a one-byte class per cell and a λf table per class pair) moves 1 byte
+ * per cell instead of 8 and drops the per-face division.  No reference
+ * counterpart: the reference has a single (direct) form. */
+int h3d_set_option(h3d_ctx *ctx, int option, int value);
+int h3d_get_option(const h3d_ctx *ctx, int option, int *value);
 
 /* Δh[1], Δh[2] and the Z metrics ZC, ΔZ (both MZ entries, genZ! layout,
  * src/Zcoord.jl:133-166). */

@@ -3,8 +3,11 @@
 Element-wise operators (boundary set-up, RHS, CalcAX!, CalcRK!'s residual
 field, one red-black colour, the preconditioner, BiCG1!/Triad!/BICG2!) must
 be bitwise equal to the oracle: both perform the reference's IEEE
-operations in the reference's order.  Grid sums (Fdot1/Fdot2, CalcRK!'s
-norm) use a different summation order and must agree to 1e-13 relative.
+operations in the reference's order.  The operators run in both coefficient
+forms of the library (per-face harmonic means of the λ field, and the
+class-table form used when λ takes few distinct values), which must agree
+bitwise.  Grid sums are exactly rounded: equal to the oracle's exact-sum
+mode and within 1e-13 of the reference's sequential sums.
 """
 import numpy as np
 import pytest
@@ -29,14 +32,18 @@ def mixed_bc():
                                 convection_bc(12.0, 290.0), isothermal_bc(300.0), heat_flux_bc(-2.0e3))
 
 
-def make_pair(oracle, NX, NY, NZ, seed, bc_set, random_lam=True, steady=True, dt=0.25):
-    """Identical random inputs on the device and in the oracle."""
+def make_pair(oracle, NX, NY, NZ, seed, bc_set, random_lam=True, steady=True, dt=0.25, form=L.COEF_AUTO):
+    """Identical random inputs on the device and in the oracle.  random_lam:
+    True = a different λ in every cell (the direct form), "classes" = λ
+    drawn from 9 random values (the table form), False = modelA's λ."""
     rng = np.random.default_rng(seed)
     MX, MY, MZ = NX + 2, NY + 2, NZ + 2
     shape = (MX, MY, MZ)
     dx, dy = h2_amd.pitch(NX), h2_amd.pitch(NY)
     Z, ZC, dZ = h2_amd.genZ(NZ)
-    if random_lam:
+    if random_lam == "classes":
+        lam = rng.choice(rng.uniform(0.3, 400.0, 9), size=shape)
+    elif random_lam:
         lam = rng.uniform(0.3, 400.0, shape)
     else:
         m = h2_amd.build_model(NX, NY, NZ)
@@ -45,6 +52,7 @@ def make_pair(oracle, NX, NY, NZ, seed, bc_set, random_lam=True, steady=True, dt
     cp = rng.uniform(300.0, 1100.0, shape)
     th = rng.uniform(290.0, 350.0, shape)
     wk = WorkBuffers(*shape)
+    wk.set_coef_form(form)
     wk.set_grid(dx, dy, ZC, dZ)
     wk.set_bc(bc_set)
     for name, a in (("lam", lam), ("rho", rho), ("cp", cp), ("theta", th)):
@@ -95,23 +103,32 @@ def test_calc_rhs_bitwise(oracle, size, steady):
     wk.close()
 
 
+FORMS = [L.COEF_AUTO, L.COEF_DIRECT]
+
+
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("steady", [True, False])
-@pytest.mark.parametrize("random_lam", [True, False])
+@pytest.mark.parametrize("random_lam", [True, "classes", False])
 @pytest.mark.parametrize("size", SIZES)
-def test_calc_ax_bitwise(oracle, size, random_lam, steady):
-    wk, c, rng = make_pair(oracle, *size, seed=3, bc_set=mixed_bc(), random_lam=random_lam, steady=steady)
+def test_calc_ax_bitwise(oracle, size, random_lam, steady, form):
+    wk, c, rng = make_pair(oracle, *size, seed=3, bc_set=mixed_bc(), random_lam=random_lam, steady=steady,
+                           form=form)
     x = rng.normal(0, 1, wk.shape)  # halo values too: the operator reads them
     put(wk, c, "pcg_p_", x)
     wk.calc_ax("pcg_q", "pcg_p_", c.grid.dt, steady)
     c.calc_ax("q", "p_")
     assert_bitwise(wk.download("pcg_q"), c.arr["q"], "ax")
+    # the table form is in use exactly when λ has few distinct values
+    expect_table = form == L.COEF_AUTO and random_lam is not True
+    assert (wk.num_classes() > 0) == expect_table
     wk.close()
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("steady", [True, False])
 @pytest.mark.parametrize("size", SIZES)
-def test_calc_rk(oracle, size, steady):
-    wk, c, rng = make_pair(oracle, *size, seed=4, bc_set=mixed_bc(), steady=steady)
+def test_calc_rk(oracle, size, steady, form):
+    wk, c, rng = make_pair(oracle, *size, seed=4, bc_set=mixed_bc(), steady=steady, random_lam="classes", form=form)
     put(wk, c, "b", rng.normal(0, 1e-3, wk.shape))
     res_d = wk.calc_rk("pcg_r", "theta", "b", c.grid.dt, steady)
     res_o = c.calc_rk("r", "theta", "b")
@@ -122,10 +139,11 @@ def test_calc_rk(oracle, size, steady):
     wk.close()
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("color", [0, 1])
 @pytest.mark.parametrize("size", SIZES)
-def test_rbsor_color_bitwise(oracle, size, color):
-    wk, c, rng = make_pair(oracle, *size, seed=5, bc_set=set_mode3_bc_parameters(), random_lam=False)
+def test_rbsor_color_bitwise(oracle, size, color, form):
+    wk, c, rng = make_pair(oracle, *size, seed=5, bc_set=set_mode3_bc_parameters(), random_lam=False, form=form)
     put(wk, c, "pcg_p", rng.normal(0, 1e-4, wk.shape))
     put(wk, c, "pcg_p_", rng.normal(0, 1e-2, wk.shape))
     wk.rbsor_color("pcg_p_", "pcg_p", color)
@@ -134,11 +152,12 @@ def test_rbsor_color_bitwise(oracle, size, color):
     wk.close()
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("steady", [True, False])
 @pytest.mark.parametrize("smoother", [0, 1])
 @pytest.mark.parametrize("size", SIZES)
-def test_precond_bitwise(oracle, size, smoother, steady):
-    wk, c, rng = make_pair(oracle, *size, seed=6, bc_set=mixed_bc(), steady=steady)
+def test_precond_bitwise(oracle, size, smoother, steady, form):
+    wk, c, rng = make_pair(oracle, *size, seed=6, bc_set=mixed_bc(), steady=steady, random_lam="classes", form=form)
     put(wk, c, "pcg_p", rng.normal(0, 1e-4, wk.shape))
     put(wk, c, "pcg_p_", np.zeros(wk.shape))
     wk.precond("pcg_p_", "pcg_p", smoother, c.grid.dt, steady)

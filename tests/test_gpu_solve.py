@@ -21,9 +21,10 @@ from h2_amd import _lib as L
 from h2_amd.heat3ds import WorkBuffers, load_model
 
 
-def device_case(NX, NY, NZ, power="modelA"):
+def device_case(NX, NY, NZ, power="modelA", form=L.COEF_AUTO):
     m = h2_amd.build_model(NX, NY, NZ, power=power)
     wk = WorkBuffers(*m.shape)
+    wk.set_coef_form(form)
     load_model(wk, m)
     return wk, m
 
@@ -36,11 +37,12 @@ def oracle_solve(oracle, NX, NY, NZ, solver, smoother, tol, maxItr=20000, steady
     return c, conv, itr, res0, hist
 
 
+@pytest.mark.parametrize("form", [L.COEF_AUTO, L.COEF_DIRECT])
 @pytest.mark.parametrize("solver,smoother,tol", [("pbicgstab", 1, 1e-6), ("cg", 1, 1e-6), ("pbicgstab", 0, 1e-6),
                                                  ("cg", 0, 1e-4)])
-def test_c1_solve_bitwise_vs_exact_oracle(oracle, solver, smoother, tol):
+def test_c1_solve_bitwise_vs_exact_oracle(oracle, solver, smoother, tol, form):
     c, conv, itr, res0, hist = oracle_solve(oracle, 40, 40, 30, solver, smoother, tol)
-    wk, m = device_case(40, 40, 30)
+    wk, m = device_case(40, 40, 30, form=form)
     wk.calc_rhs(10000.0, True)
     assert np.array_equal(wk.download("b"), c.arr["b"])
     s = L.SOLVER_CG if solver == "cg" else L.SOLVER_PBICGSTAB
