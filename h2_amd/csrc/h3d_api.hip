@@ -58,6 +58,7 @@ struct h3d_ctx {
     // FP32 shadows for the refinement inner solve (allocated on first use)
     float *g32[F32_N] = {};
     double *d_corr = nullptr;    // FP64 correction accumulated by the FP32 inner solve
+    double *d_best = nullptr;    // its iterate with the smallest inner residual
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evb[2] = {nullptr, nullptr};
     double last_ms = 0.0;
     int verbose = 0;             // H3D_VERBOSE=1: refinement diagnostics on stderr
@@ -196,12 +197,14 @@ int ensure_hist(h3d_ctx *c, int n)
     return H3D_OK;
 }
 
-int reset_state(h3d_ctx *c, double tol, int maxItr)
+int reset_state(h3d_ctx *c, double tol, int maxItr, bool guard = false)
 {
     KState s{};
     s.tol = tol;
     s.maxItr = maxItr;
     s.status = ST_RUNNING;
+    s.guard = guard ? 1 : 0;
+    s.res_min = 1.0;
     s.rho_old = 1.0; s.alpha = 0.0; s.omega = 1.0; s.beta = 0.0;  // NonUniform.jl:276-280
     // the stream is idle here (every entry point ends synchronised), so the
     // pinned mirror can stage the initial state
@@ -253,6 +256,7 @@ hipError_t precond_from_zero(int mode, T *xx, T *bb, const T *r, const T *q, con
 template <typename T>
 struct KVecs {
     double *x;
+    double *xbest;  // guarded solves: the iterate with the smallest residual (else nullptr)
     T *th, *b, *p, *p_, *r, *r0, *q, *s, *s_, *t;
 };
 
@@ -278,7 +282,7 @@ hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int
     if ((e = launch_calc_ax<T>(v.t, v.s_, g, nullptr, v.s, FIN_BICG_OMEGA, st, nullptr, c->rb, s)) != hipSuccess)
         return e;
     // θ += alpha p_ + omega s_; r = -omega t + s; res; next ρ = (r, r0)
-    return launch_bicg_update<T>(v.x, v.r, v.p_, v.s_, v.t, v.s, v.r0, g, st, c->d_hist, c->rb, s);
+    return launch_bicg_update<T>(v.x, v.xbest, v.r, v.p_, v.s_, v.t, v.s, v.r0, g, st, c->d_hist, c->rb, s);
 }
 
 // Enqueue one CG! iteration (NonUniform.jl:160-223).
@@ -290,7 +294,7 @@ hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int s
     hipError_t e;
     if ((e = launch_calc_ax<T>(v.q, v.p, g, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
         return e;
-    if ((e = launch_cg_update<T>(v.x, v.r, v.p, v.q, g, st, c->d_hist, c->rb, s)) != hipSuccess) return e;
+    if ((e = launch_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, g, st, c->d_hist, c->rb, s)) != hipSuccess) return e;
     if ((e = precond_from_zero<T>(FIRST_COPY, v.s, v.r, nullptr, nullptr, g, smoother, st, c->H, s)) != hipSuccess)
         return e;
     if ((e = launch_dot<T>(v.r, v.s, nullptr, g, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
@@ -305,7 +309,7 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Geo<T> &g, double to
     hipStream_t s = c->stream;
     int rc;
     if ((rc = ensure_hist(c, std::max(maxItr, 1))) != H3D_OK) return rc;
-    if ((rc = reset_state(c, tol, maxItr)) != H3D_OK) return rc;
+    if ((rc = reset_state(c, tol, maxItr, v.xbest != nullptr)) != H3D_OK) return rc;
     // The Krylov vectors keep a zero halo, as in the reference where only
     // their interiors are ever written (WorkBuffers starts them at zero).
     T *vecs[] = {v.p, v.p_, v.r, v.r0, v.q, v.s, v.s_, v.t};
@@ -373,6 +377,7 @@ KVecs<double> fp64_vecs(h3d_ctx *c)
 {
     KVecs<double> v;
     v.x = v.th = c->f[H3D_F_THETA]; v.b = c->f[H3D_F_B];
+    v.xbest = nullptr;
     v.p = c->f[H3D_F_PCG_P]; v.p_ = c->f[H3D_F_PCG_P_]; v.r = c->f[H3D_F_PCG_R];
     v.r0 = c->f[H3D_F_PCG_R0]; v.q = c->f[H3D_F_PCG_Q]; v.s = c->f[H3D_F_PCG_S];
     v.s_ = c->f[H3D_F_PCG_S_]; v.t = c->f[H3D_F_PCG_T_];
@@ -421,22 +426,19 @@ int ensure_fp32(h3d_ctx *c)
     for (int a = 0; a < F32_N; ++a)
         if (!c->g32[a]) HIPCHK(c, hipMalloc(&c->g32[a], c->H * sizeof(float)));
     if (!c->d_corr) HIPCHK(c, hipMalloc(&c->d_corr, c->H * sizeof(double)));
+    if (!c->d_best) HIPCHK(c, hipMalloc(&c->d_best, c->H * sizeof(double)));
     if (!c->d_ZC32) HIPCHK(c, hipMalloc(&c->d_ZC32, c->MZ * sizeof(float)));
     if (!c->d_dZ32) HIPCHK(c, hipMalloc(&c->d_dZ32, c->MZ * sizeof(float)));
     return H3D_OK;
 }
 
-int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
-                  double inner_tol, int *isconverged, int *itr, double *res0_out)
+// FP32 operator (flux form) from the FP64 one make_geo64 set up: λ, ZC,
+// ΔZ and ap0 rounded to float, the same classes with the FP32 λf table.
+int make_geo32(h3d_ctx *c, int is_steady, Geo<float> *out)
 {
-    Geo<double> g64;
-    int rc = make_geo64(c, dt, is_steady, &g64);
-    if (rc != H3D_OK) return rc;
+    int rc;
     if ((rc = ensure_fp32(c)) != H3D_OK) return rc;
     hipStream_t s = c->stream;
-    if (!(inner_tol > 0.0)) inner_tol = 1e-4;
-    HIPCHK(c, hipEventRecord(c->ev0, s));
-    // FP32 operator: λ, ZC, ΔZ, ap0 rounded to float; the same classes
     HIPCHK(c, launch_convert<float, double>(c->g32[F32_LAM], c->f[H3D_F_LAMBDA], c->H, s));
     HIPCHK(c, launch_convert<float, double>(c->d_ZC32, c->d_ZC, (size_t)c->MZ, s));
     HIPCHK(c, launch_convert<float, double>(c->d_dZ32, c->d_dZ, (size_t)c->MZ, s));
@@ -454,8 +456,25 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         HIPCHK(c, launch_convert<float, double>(c->d_ap032, c->d_ap0, c->H, s));
         g32.ap0 = c->d_ap032;
     }
+    *out = g32;
+    return H3D_OK;
+}
+
+int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
+                  double inner_tol, int *isconverged, int *itr, double *res0_out)
+{
+    Geo<double> g64;
+    int rc = make_geo64(c, dt, is_steady, &g64);
+    if (rc != H3D_OK) return rc;
+    if ((rc = ensure_fp32(c)) != H3D_OK) return rc;
+    hipStream_t s = c->stream;
+    if (!(inner_tol > 0.0)) inner_tol = 1e-4;
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    Geo<float> g32;
+    if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;
     KVecs<float> v;
     v.x = c->d_corr;
+    v.xbest = c->d_best;
     v.th = c->g32[F32_X]; v.b = c->g32[F32_B];
     v.p = c->g32[F32_P]; v.p_ = c->g32[F32_P_]; v.r = c->g32[F32_R]; v.r0 = c->g32[F32_R0];
     v.q = c->g32[F32_Q]; v.s = c->g32[F32_S]; v.s_ = c->g32[F32_S_]; v.t = c->g32[F32_T];
@@ -483,17 +502,19 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         if (!std::isfinite(rn)) { nan = true; break; }
         if (outer_it > 0 && rn / res0 < tol) { converged = true; break; }
         if (outer_it == max_outer || total_inner >= maxItr) break;
-        // inner FP32 solve of A d = r from d = 0
+        // inner FP32 solve of A d = r from d = 0, guarded: it stops on
+        // divergence or stagnation and the correction is its best iterate
         HIPCHK(c, launch_convert<float, double>(v.b, r64, c->H, s));
         HIPCHK(c, launch_fill<float>(v.th, 0.f, c->H, s));
         HIPCHK(c, launch_fill<double>(v.x, 0.0, c->H, s));
+        HIPCHK(c, launch_fill<double>(v.xbest, 0.0, c->H, s));
         KState st{};
         rc = krylov<float>(c, solver, v, g32, inner_tol, std::max(1, maxItr - total_inner), smoother, &st);
         if (rc != H3D_OK) return rc;
         total_inner += st.itr;
         if (c->verbose) {
-            std::fprintf(stderr, "h3d refine: outer %d true res %.6e, inner itr %d status %d inner res %.3e\n",
-                         outer_it, rn / res0, st.itr, st.status, st.res);
+            std::fprintf(stderr, "h3d refine: outer %d true res %.6e, inner itr %d status %d inner res %.3e min %.3e\n",
+                         outer_it, rn / res0, st.itr, st.status, st.res, st.res_min);
             if (c->verbose > 1 && st.itr > 0) {
                 std::vector<double> h((size_t)st.itr);
                 HIPCHK(c, hipMemcpy(h.data(), c->d_hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -501,24 +522,9 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
                     std::fprintf(stderr, "   inner %5d %.6e\n", k + 1, h[(size_t)k]);
             }
         }
-        if (c->verbose > 1) {
-            // FP32-operator residual of the accumulated correction, against the
-            // recurrence residual the inner solve stopped on
-            HIPCHK(c, launch_convert<float, double>(c->g32[F32_P], v.x, c->H, s));
-            HIPCHK(c, launch_calc_rk<float>(c->g32[F32_Q], nullptr, c->g32[F32_P], v.b, g32, FIN_STORE, c->d_unit,
-                                            nullptr, c->rb, s));
-            HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
-            HIPCHK(c, hipStreamSynchronize(s));
-            const double r32 = std::sqrt(c->h_state[0].out[0]);
-            HIPCHK(c, launch_dot<float>(v.b, nullptr, nullptr, g32, FIN_STORE, c->d_unit, nullptr, c->rb, s));
-            HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
-            HIPCHK(c, hipStreamSynchronize(s));
-            std::fprintf(stderr, "   FP32-operator true residual of the correction: %.3e (relative to |b32|)\n",
-                         r32 / std::sqrt(c->h_state[0].out[0]));
-        }
-        if (st.status == ST_NAN) { nan = true; break; }
-        HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], v.x, g64, s));
-        if (st.status == ST_BREAKDOWN && st.itr <= 1) break;  // no progress possible
+        if (!(st.res_min < 1.0)) break;  // the inner solve made no progress: stop refining
+        // st.improved: the last iterate is the best one; else the snapshot is
+        HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], st.improved ? v.x : v.xbest, g64, s));
     }
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -631,7 +637,7 @@ int h3d_destroy(h3d_ctx *c)
         if (c->g32[a]) (void)hipFree(c->g32[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_tab64, c->d_tab32, c->rb.partials,
-                  c->d_corr};
+                  c->d_corr, c->d_best};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
@@ -935,12 +941,25 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
 {
     if (!c) return H3D_ERR_ARG;
     if (!ms || reps < 1) return fail(c, H3D_ERR_ARG, "bad arguments");
-    if (which < 0 || which > 2) return fail(c, H3D_ERR_ARG, "unknown kernel id");
+    if (which < 0 || which > 4) return fail(c, H3D_ERR_ARG, "unknown kernel id");
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
     double *x = c->f[H3D_F_PCG_P_], *b = c->f[H3D_F_PCG_P], *q = c->f[H3D_F_PCG_Q];
+    Geo<float> g32{};
+    if (which >= 3) {
+        if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;
+        HIPCHK(c, launch_convert<float, double>(c->g32[F32_X], x, c->H, c->stream));
+        HIPCHK(c, launch_convert<float, double>(c->g32[F32_B], b, c->H, c->stream));
+    }
+    float *x32 = c->g32[F32_X], *b32 = c->g32[F32_B], *q32 = c->g32[F32_Q];
     auto once = [&]() -> hipError_t {
+        if (which == 3) {
+            hipError_t e = launch_rbsor<float>(x32, b32, g32, 0, c->d_unit, c->stream);
+            return e != hipSuccess ? e : launch_rbsor<float>(x32, b32, g32, 1, c->d_unit, c->stream);
+        }
+        if (which == 4)
+            return launch_calc_ax<float>(q32, x32, g32, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
         if (which == 0) {
             hipError_t e = launch_rbsor<double>(x, b, g, 0, c->d_unit, c->stream);
             return e != hipSuccess ? e : launch_rbsor<double>(x, b, g, 1, c->d_unit, c->stream);

@@ -16,7 +16,15 @@ enum : int {
     ST_BREAKDOWN = 2,   // |ρ|, |(t,t)|, |(p,q)| below FloatMin (src/common.jl:12)
     ST_MAXITER = 3,     // loop ran maxItr iterations without converging
     ST_NAN = 4,         // NaN/Inf residual (guard the reference lacks)
+    ST_DIVERGED = 5,    // guarded solve only: residual far above its minimum, NaN, or no new minimum for long
 };
+
+// Guarded solves (the FP32 inner solves of the refinement loop) stop when the
+// relative residual exceeds kDivergeFactor times the smallest one seen, or
+// when kStallIters iterations pass without a new minimum; the iterate with
+// the smallest residual is kept (see KState::improved).
+constexpr double kDivergeFactor = 1.0e4;
+constexpr int kStallIters = 500;
 
 // Scalar steps run by the finalize kernel once the grid sums s[0..2] of the
 // preceding reducing kernel are known.  Each one restates one scalar step of
@@ -38,8 +46,13 @@ enum : int {
 
 struct KState {
     double rho, rho_old, alpha, omega, beta, res0, res, tol;
+    double res_min;         // guarded solves: smallest relative residual so far (1 = the start)
     double out[4];          // FIN_STORE destination
-    int itr, status, maxItr, pad;
+    int itr, status, maxItr;
+    int guard;              // 1: divergence guard and best-iterate tracking on
+    int improved;           // guarded: the last residual was a new minimum, so the next update
+                            // kernel copies the iterate it is about to overwrite into the best-iterate buffer
+    int since_best;         // guarded: iterations since the last new minimum
 };
 
 // Per-workgroup partial sums of a reducing kernel (up to three sums per
@@ -128,14 +141,17 @@ enum : int {
 template <typename T>
 hipError_t launch_axpby(int op, T *z, const T *x, const T *y, double a, double b, const Geo<T> &g,
                         const KState *st, hipStream_t s);
-// BICG2!(θ,p_,s_,α,ω); Triad!(r,t,s,-ω); Σ r²; with r0 also Σ r·r0
+// BICG2!(θ,p_,s_,α,ω); Triad!(r,t,s,-ω); Σ r²; with r0 also Σ r·r0.
+// xbest (guarded solves, else nullptr) receives θ before the update when
+// the previous residual was a new minimum.
 template <typename T>
-hipError_t launch_bicg_update(double *th, T *r, const T *p_, const T *s_, const T *t, const T *sv, const T *r0,
-                              const Geo<T> &g, KState *st, double *hist, const RedBuf &rb, hipStream_t s);
+hipError_t launch_bicg_update(double *th, double *xbest, T *r, const T *p_, const T *s_, const T *t, const T *sv,
+                              const T *r0, const Geo<T> &g, KState *st, double *hist, const RedBuf &rb,
+                              hipStream_t s);
 // CG!: θ += α p, r -= α q, Σ r²
 template <typename T>
-hipError_t launch_cg_update(double *th, T *r, const T *p, const T *q, const Geo<T> &g, KState *st, double *hist,
-                            const RedBuf &rb, hipStream_t s);
+hipError_t launch_cg_update(double *th, double *xbest, T *r, const T *p, const T *q, const Geo<T> &g, KState *st,
+                            double *hist, const RedBuf &rb, hipStream_t s);
 // CG!: p = z + β p
 template <typename T>
 hipError_t launch_cg_p(T *p, const T *z, const Geo<T> &g, const KState *st, hipStream_t s);

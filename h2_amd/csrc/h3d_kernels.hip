@@ -252,12 +252,32 @@ __device__ void fin_bicg_rho(double rho, KState *st)  // NonUniform.jl:287-300
     if (st->itr > 1) st->beta = st->rho / st->rho_old * st->alpha / st->omega;
 }
 
+// Divergence guard of a guarded solve (no reference counterpart): returns
+// false after stopping the solve.
+__device__ bool guard_step(double res, KState *st)
+{
+    st->improved = 0;
+    if (!isfinite(res) || res > kDivergeFactor * st->res_min || st->since_best >= kStallIters) {
+        st->status = ST_DIVERGED;
+        return false;
+    }
+    if (res < st->res_min) {
+        st->res_min = res;
+        st->improved = 1;
+        st->since_best = 0;
+    } else {
+        st->since_best += 1;
+    }
+    return true;
+}
+
 __device__ void fin_bicg_res(double s0, KState *st, double *hist)  // NonUniform.jl:331-346
 {
     double res = sqrt(s0);
     res /= st->res0;
     st->res = res;
     if (hist) hist[st->itr - 1] = res;
+    if (st->guard && !guard_step(res, st)) return;
     if (!isfinite(res)) st->status = ST_NAN;
     else if (res < st->tol) st->status = ST_CONVERGED;
     else {
@@ -317,6 +337,7 @@ __device__ void finalize(int fin, const double *s, KState *st, double *hist)
         res /= st->res0;
         st->res = res;
         if (hist) hist[st->itr - 1] = res;
+        if (st->guard && !guard_step(res, st)) break;
         if (!isfinite(res)) st->status = ST_NAN;
         else if (res < st->tol) st->status = ST_CONVERGED;
         break;
@@ -768,19 +789,22 @@ hipError_t launch_axpby(int op, T *z, const T *x, const T *y, double a, double b
 // The iterate θ is always FP64: in the FP32 inner solve it accumulates the
 // correction, whose many cancelling updates FP32 storage would not resolve.
 template <typename T, int NS>
-__global__ void __launch_bounds__(256) k_bicg_update(double *th, T *r, const T *p_, const T *s_, const T *t,
-                                                     const T *sv, const T *r0, Geo<T> g, KState *st, RedBuf rb,
-                                                     int KC)
+__global__ void __launch_bounds__(256) k_bicg_update(double *th, double *xbest, T *r, const T *p_, const T *s_,
+                                                     const T *t, const T *sv, const T *r0, Geo<T> g, KState *st,
+                                                     RedBuf rb, int KC)
 {
     if (halted(st)) return;
     INTERIOR_COLUMN(g, KC)
     const double alpha = st->alpha, omega = st->omega;
     const T r_omega = (T)(-st->omega);
+    const bool snap = xbest != nullptr && st->improved != 0;
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     if (active) {
         for (int k = kb; k < ke; ++k) {
             const size_t c = c0_ + sz_ * k;
-            th[c] = th[c] + (alpha * (double)p_[c] + omega * (double)s_[c]);
+            const double xo = th[c];
+            if (snap) xbest[c] = xo;
+            th[c] = xo + (alpha * (double)p_[c] + omega * (double)s_[c]);
             const T rv = r_omega * t[c] + sv[c];
             r[c] = rv;
             v[0] = dd_add(v[0], (double)rv * (double)rv);
@@ -791,36 +815,41 @@ __global__ void __launch_bounds__(256) k_bicg_update(double *th, T *r, const T *
 }
 
 template <typename T>
-hipError_t launch_bicg_update(double *th, T *r, const T *p_, const T *s_, const T *t, const T *sv, const T *r0,
-                              const Geo<T> &g, KState *st, double *hist, const RedBuf &rb, hipStream_t s)
+hipError_t launch_bicg_update(double *th, double *xbest, T *r, const T *p_, const T *s_, const T *t, const T *sv,
+                              const T *r0, const Geo<T> &g, KState *st, double *hist, const RedBuf &rb,
+                              hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
     hipError_t e;
     if (r0 == nullptr) {
-        hipLaunchKernelGGL((k_bicg_update<T, 1>), sh.grid, sh.block, 0, s, th, r, p_, s_, t, sv, r0, g, st, rb,
-                           sh.KC);
+        hipLaunchKernelGGL((k_bicg_update<T, 1>), sh.grid, sh.block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st,
+                           rb, sh.KC);
         e = hipGetLastError();
         return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_BICG_RES, st, hist, rb, s);
     }
-    hipLaunchKernelGGL((k_bicg_update<T, 2>), sh.grid, sh.block, 0, s, th, r, p_, s_, t, sv, r0, g, st, rb, sh.KC);
+    hipLaunchKernelGGL((k_bicg_update<T, 2>), sh.grid, sh.block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st, rb,
+                       sh.KC);
     e = hipGetLastError();
     return e != hipSuccess ? e : launch_finalize<2>(sh, FIN_BICG_RES_RHO, st, hist, rb, s);
 }
 
 // CG!: θ += α p, r -= α q, Σ r² (NonUniform.jl:179-186)
 template <typename T>
-__global__ void __launch_bounds__(256) k_cg_update(double *th, T *r, const T *p, const T *q, Geo<T> g, KState *st,
-                                                   RedBuf rb, int KC)
+__global__ void __launch_bounds__(256) k_cg_update(double *th, double *xbest, T *r, const T *p, const T *q, Geo<T> g,
+                                                   KState *st, RedBuf rb, int KC)
 {
     if (halted(st)) return;
     INTERIOR_COLUMN(g, KC)
     const double alpha_d = st->alpha;
     const T alpha = (T)st->alpha;
+    const bool snap = xbest != nullptr && st->improved != 0;
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     if (active) {
         for (int k = kb; k < ke; ++k) {
             const size_t c = c0_ + sz_ * k;
-            th[c] = th[c] + alpha_d * (double)p[c];
+            const double xo = th[c];
+            if (snap) xbest[c] = xo;
+            th[c] = xo + alpha_d * (double)p[c];
             const T rv = r[c] - alpha * q[c];
             r[c] = rv;
             v[0] = dd_add(v[0], (double)rv * (double)rv);
@@ -830,11 +859,11 @@ __global__ void __launch_bounds__(256) k_cg_update(double *th, T *r, const T *p,
 }
 
 template <typename T>
-hipError_t launch_cg_update(double *th, T *r, const T *p, const T *q, const Geo<T> &g, KState *st, double *hist,
-                            const RedBuf &rb, hipStream_t s)
+hipError_t launch_cg_update(double *th, double *xbest, T *r, const T *p, const T *q, const Geo<T> &g, KState *st,
+                            double *hist, const RedBuf &rb, hipStream_t s)
 {
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
-    hipLaunchKernelGGL(k_cg_update<T>, sh.grid, sh.block, 0, s, th, r, p, q, g, st, rb, sh.KC);
+    hipLaunchKernelGGL(k_cg_update<T>, sh.grid, sh.block, 0, s, th, xbest, r, p, q, g, st, rb, sh.KC);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_CG_RES, st, hist, rb, s);
 }
@@ -1059,9 +1088,11 @@ hipError_t launch_add_correction(double *th, const double *d, const Geo<double> 
                                       const RedBuf &, hipStream_t);                                              \
     template hipError_t launch_axpby<T>(int, T *, const T *, const T *, double, double, const Geo<T> &,          \
                                         const KState *, hipStream_t);                                            \
-    template hipError_t launch_bicg_update<T>(double *, T *, const T *, const T *, const T *, const T *, const T *, \
+    template hipError_t launch_bicg_update<T>(double *, double *, T *, const T *, const T *, const T *, const T *, \
+                                              const T *, \
                                               const Geo<T> &, KState *, double *, const RedBuf &, hipStream_t);  \
-    template hipError_t launch_cg_update<T>(double *, T *, const T *, const T *, const Geo<T> &, KState *, double *, \
+    template hipError_t launch_cg_update<T>(double *, double *, T *, const T *, const T *, const Geo<T> &, KState *, \
+                                            double *, \
                                             const RedBuf &, hipStream_t);                                        \
     template hipError_t launch_cg_p<T>(T *, const T *, const Geo<T> &, const KState *, hipStream_t);
 INST(double)

@@ -165,7 +165,8 @@ int h3d_bicg2(h3d_ctx *ctx, int z, int x, int y, double a, double b);
 /* Timing of one kernel family on the context stream (HIP events around
  * reps back-to-back launches after one warm-up launch; *ms = mean per
  * launch).  which: 0 = one full red-black GS sweep (both colours),
- * 1 = CalcAX!, 2 = CalcAX! with its fused dot. */
+ * 1 = CalcAX!, 2 = CalcAX! with its fused dot; 3 and 4 = the FP32 operator's
+ * sweep and CalcAX! (the kernels of the H3D_FP32_REFINED inner solve). */
 int h3d_time_kernel(h3d_ctx *ctx, int which, int reps, double dt, int is_steady, double *ms);
 
 #ifdef __cplusplus
