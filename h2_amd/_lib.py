@@ -18,7 +18,8 @@ ISOTHERMAL, HEAT_FLUX, CONVECTION = 0, 1, 2
 SOLVER_PBICGSTAB, SOLVER_CG = 0, 1
 SMOOTHER_NONE, SMOOTHER_GS = 0, 1
 FP64, FP32_REFINED = 0, 1
-OPT_COEF_FORM, OPT_NUM_CLASSES = 0, 1
+OPT_COEF_FORM, OPT_NUM_CLASSES, OPT_LAYOUT = 0, 1, 2
+LAYOUT_AUTO, LAYOUT_STANDARD = 0, 1
 COEF_AUTO, COEF_DIRECT = 0, 1
 
 FIELDS = ["theta", "b", "mask", "rho", "lam", "cp", "pcg_p", "pcg_p_", "pcg_r", "pcg_r0",

@@ -59,6 +59,36 @@ struct h3d_ctx {
     float *g32[F32_N] = {};
     double *d_corr = nullptr;    // FP64 correction accumulated by the FP32 inner solve
     double *d_best = nullptr;    // its iterate with the smallest inner residual
+    // colour-split (RB) layout of the Krylov vectors: the solver's layout
+    // whenever the class table form is active (H3D_OPT_LAYOUT)
+    int layout = H3D_LAYOUT_AUTO;
+    unsigned long grid_version = 1;       // bumped by h3d_set_grid / h3d_set_bc
+    unsigned long rb_lam = 0, rb_grid = 0; // versions the RB tables were built for
+    std::vector<double> hZC, hdZ;         // host copies of the Z metrics
+    std::vector<double> h_tab64;          // host copies of the λf tables
+    std::vector<float> h_tab32;
+    int HX = 0;
+    size_t HS = 0;
+    uint8_t *d_cls_rb = nullptr;
+    double *d_ZF64 = nullptr;
+    float *d_ZF32 = nullptr;
+    double *d_ap0_rb = nullptr;
+    float *d_ap0_rb32 = nullptr;
+    double *rb64[8] = {};                 // p, p_, r, r0, q, s, s_, t in RB layout
+    float *rb32[8] = {};
+    // instantiated iteration graphs, reused while nothing they captured changed
+    struct GraphKey {
+        int solver, smoother, tsize, steady, rb;
+        unsigned long lam_v, grid_v;
+        const void *hist, *partials, *x;
+        bool operator==(const GraphKey &o) const
+        {
+            return solver == o.solver && smoother == o.smoother && tsize == o.tsize && steady == o.steady &&
+                   rb == o.rb && lam_v == o.lam_v && grid_v == o.grid_v && hist == o.hist &&
+                   partials == o.partials && x == o.x;
+        }
+    };
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evb[2] = {nullptr, nullptr};
     double last_ms = 0.0;
     int verbose = 0;             // H3D_VERBOSE=1: refinement diagnostics on stderr
@@ -152,6 +182,8 @@ int ensure_classes(h3d_ctx *c)
     HIPCHK(c, hipMemcpy(c->d_vals, vals.data(), K * sizeof(uint64_t), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+    c->h_tab64 = t64;
+    c->h_tab32 = t32;
     HIPCHK(c, launch_classify(c->d_cls, c->f[H3D_F_LAMBDA], c->d_vals, K, c->H, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->K = K;
@@ -232,24 +264,6 @@ hipError_t precond(T *xx, const T *bb, const Geo<T> &g, int smoother, const KSta
 // myfill!(xx, 0) followed by Preconditioner!: the fill and colour 0 of the
 // first sweep run in one kernel, which also forms bb first when mode asks
 // (BiCG1!/Triad!, see launch_rbsor_first).
-template <typename T>
-hipError_t precond_from_zero(int mode, T *xx, T *bb, const T *r, const T *q, const Geo<T> &g, int smoother,
-                             const KState *st, size_t H, hipStream_t s)
-{
-    hipError_t e;
-    if (smoother == H3D_SMOOTHER_NONE) {
-        if (mode != FIRST_COPY && (e = launch_form_rhs<T>(mode, bb, r, q, g, st, s)) != hipSuccess) return e;
-        return hipMemcpyAsync(xx, bb, H * sizeof(T), hipMemcpyDeviceToDevice, s);
-    }
-    if ((e = launch_rbsor_first<T>(mode, xx, bb, r, q, g, st, s)) != hipSuccess) return e;
-    if ((e = launch_rbsor<T>(xx, bb, g, 1, st, s)) != hipSuccess) return e;
-    for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
-        e = launch_rbsor<T>(xx, bb, g, 0, st, s);
-        if (e == hipSuccess) e = launch_rbsor<T>(xx, bb, g, 1, st, s);
-    }
-    return e;
-}
-
 // Buffers of one Krylov solve in type T.  x is the FP64 iterate the
 // updates accumulate into; th is the starting iterate CalcRK! reads (x
 // itself in FP64, a zero field in the FP32 inner solve, which starts at 0).
@@ -260,75 +274,147 @@ struct KVecs {
     T *th, *b, *p, *p_, *r, *r0, *q, *s, *s_, *t;
 };
 
+// The operator of a solve in the layout it runs in: the reference's layout
+// (Geo) or the colour-split one (RbGeo); the p_* helpers dispatch.
+template <typename T>
+struct Path {
+    Geo<T> g{};
+    RbGeo<T> rg{};
+    bool rb = false;
+    size_t n = 0;  // Krylov vector length
+};
+
+template <typename T>
+hipError_t p_calc_ax(const Path<T> &P, T *ax, const T *x, const T *dy, const T *dz, int fin, KState *st, double *hist,
+                     const RedBuf &rb, hipStream_t s)
+{
+    return P.rb ? launch_rb_calc_ax<T>(ax, x, P.rg, dy, dz, fin, st, hist, rb, s)
+                : launch_calc_ax<T>(ax, x, P.g, dy, dz, fin, st, hist, rb, s);
+}
+
+template <typename T>
+hipError_t p_calc_rk(const Path<T> &P, T *r, T *r0copy, const T *th, const T *b, int fin, KState *st, double *hist,
+                     const RedBuf &rb, hipStream_t s)
+{
+    return P.rb ? launch_rb_calc_rk<T>(r, r0copy, th, b, P.rg, fin, st, hist, rb, s)
+                : launch_calc_rk<T>(r, r0copy, th, b, P.g, fin, st, hist, rb, s);
+}
+
+template <typename T>
+hipError_t p_dot(const Path<T> &P, const T *x, const T *y, const T *z, int fin, KState *st, double *hist,
+                 const RedBuf &rb, hipStream_t s)
+{
+    return P.rb ? launch_rb_dot<T>(x, y, z, P.rg, fin, st, hist, rb, s)
+                : launch_dot<T>(x, y, z, P.g, fin, st, hist, rb, s);
+}
+
+template <typename T>
+hipError_t p_copy_interior(const Path<T> &P, T *dst, const T *src, const KState *st, hipStream_t s)
+{
+    return P.rb ? launch_rb_copy_interior<T>(dst, src, P.rg, st, s) : launch_copy_interior<T>(dst, src, P.g, st, s);
+}
+
+template <typename T>
+hipError_t p_sweep(const Path<T> &P, T *x, const T *b, int color, const KState *st, hipStream_t s)
+{
+    return P.rb ? launch_rb_sweep<T>(x, b, P.rg, color, st, s) : launch_rbsor<T>(x, b, P.g, color, st, s);
+}
+
+// Preconditioner! applied to the vector the mode forms (BiCG1!, Triad! or
+// bb itself), starting from xx = 0 as the callers' myfill!(xx, 0) does.
+template <typename T>
+hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T *r, const T *q, int smoother,
+                               const KState *st, hipStream_t s)
+{
+    hipError_t e;
+    if (smoother == H3D_SMOOTHER_NONE) {
+        if (mode != FIRST_COPY) {
+            e = P.rb ? launch_rb_form_rhs<T>(mode, bb, r, q, P.rg, st, s)
+                     : launch_form_rhs<T>(mode, bb, r, q, P.g, st, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipMemcpyAsync(xx, bb, P.n * sizeof(T), hipMemcpyDeviceToDevice, s);
+    }
+    e = P.rb ? launch_rb_first<T>(mode, xx, bb, r, q, P.rg, st, s)
+             : launch_rbsor_first<T>(mode, xx, bb, r, q, P.g, st, s);
+    if (e != hipSuccess) return e;
+    if ((e = p_sweep<T>(P, xx, bb, 1, st, s)) != hipSuccess) return e;
+    for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
+        e = p_sweep<T>(P, xx, bb, 0, st, s);
+        if (e == hipSuccess) e = p_sweep<T>(P, xx, bb, 1, st, s);
+    }
+    return e;
+}
+
 // Enqueue one PBiCGSTAB! iteration (NonUniform.jl:287-347).  ρ of the
 // iteration was formed by the previous iteration's update (or by CalcRK!
 // for iteration 1) and its FIN_BICG_*RHO step.
 template <typename T>
-hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int smoother)
+hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, int smoother)
 {
     hipStream_t s = c->stream;
     KState *st = c->d_state;
     hipError_t e;
     // p = r or BiCG1!; myfill!(p_, 0); Preconditioner!(p_, p)
-    if ((e = precond_from_zero<T>(FIRST_BICG_P, v.p_, v.p, v.r, v.q, g, smoother, st, c->H, s)) != hipSuccess)
-        return e;
+    if ((e = p_precond_from_zero<T>(P, FIRST_BICG_P, v.p_, v.p, v.r, v.q, smoother, st, s)) != hipSuccess) return e;
     // q = A p_; alpha = rho / (q, r0)
-    if ((e = launch_calc_ax<T>(v.q, v.p_, g, v.r0, nullptr, FIN_BICG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
+    if ((e = p_calc_ax<T>(P, v.q, v.p_, v.r0, nullptr, FIN_BICG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
         return e;
     // s = -alpha q + r; myfill!(s_, 0); Preconditioner!(s_, s)
-    if ((e = precond_from_zero<T>(FIRST_BICG_S, v.s_, v.s, v.r, v.q, g, smoother, st, c->H, s)) != hipSuccess)
-        return e;
+    if ((e = p_precond_from_zero<T>(P, FIRST_BICG_S, v.s_, v.s, v.r, v.q, smoother, st, s)) != hipSuccess) return e;
     // t = A s_; omega = (t, s) / (t, t)
-    if ((e = launch_calc_ax<T>(v.t, v.s_, g, nullptr, v.s, FIN_BICG_OMEGA, st, nullptr, c->rb, s)) != hipSuccess)
+    if ((e = p_calc_ax<T>(P, v.t, v.s_, nullptr, v.s, FIN_BICG_OMEGA, st, nullptr, c->rb, s)) != hipSuccess)
         return e;
     // θ += alpha p_ + omega s_; r = -omega t + s; res; next ρ = (r, r0)
-    return launch_bicg_update<T>(v.x, v.xbest, v.r, v.p_, v.s_, v.t, v.s, v.r0, g, st, c->d_hist, c->rb, s);
+    return P.rb ? launch_rb_bicg_update<T>(v.x, v.xbest, v.r, v.p_, v.s_, v.t, v.s, v.r0, P.rg, st, c->d_hist, c->rb,
+                                           s)
+                : launch_bicg_update<T>(v.x, v.xbest, v.r, v.p_, v.s_, v.t, v.s, v.r0, P.g, st, c->d_hist, c->rb, s);
 }
 
 // Enqueue one CG! iteration (NonUniform.jl:160-223).
 template <typename T>
-hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Geo<T> &g, int smoother)
+hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, int smoother)
 {
     hipStream_t s = c->stream;
     KState *st = c->d_state;
     hipError_t e;
-    if ((e = launch_calc_ax<T>(v.q, v.p, g, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess)
+    if ((e = p_calc_ax<T>(P, v.q, v.p, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess) return e;
+    e = P.rb ? launch_rb_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.rg, st, c->d_hist, c->rb, s)
+             : launch_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.g, st, c->d_hist, c->rb, s);
+    if (e != hipSuccess) return e;
+    if ((e = p_precond_from_zero<T>(P, FIRST_COPY, v.s, v.r, nullptr, nullptr, smoother, st, s)) != hipSuccess)
         return e;
-    if ((e = launch_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, g, st, c->d_hist, c->rb, s)) != hipSuccess) return e;
-    if ((e = precond_from_zero<T>(FIRST_COPY, v.s, v.r, nullptr, nullptr, g, smoother, st, c->H, s)) != hipSuccess)
-        return e;
-    if ((e = launch_dot<T>(v.r, v.s, nullptr, g, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
-    return launch_cg_p<T>(v.p, v.s, g, st, s);
+    if ((e = p_dot<T>(P, v.r, v.s, nullptr, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
+    return P.rb ? launch_rb_cg_p<T>(v.p, v.s, P.rg, st, s) : launch_cg_p<T>(v.p, v.s, P.g, st, s);
 }
 
-// Run a Krylov solve whose vectors are in v, starting from the current θ.
-template <typename T>
-int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Geo<T> &g, double tol, int maxItr, int smoother,
-           KState *fin_state)
+void drop_graphs(h3d_ctx *c)
 {
+    for (auto &e : c->graphs) (void)hipGraphExecDestroy(e.second);
+    c->graphs.clear();
+}
+
+// The instantiated graph of one iteration for this configuration, captured
+// on first use.  Everything a captured launch argument depends on is in
+// the key, so a cached graph is only replayed with identical arguments.
+template <typename T>
+int iteration_graph(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, int smoother,
+                    hipGraphExec_t *out)
+{
+    h3d_ctx::GraphKey key{solver, smoother, (int)sizeof(T), P.g.ap0 != nullptr, P.rb ? 1 : 0,
+                          c->lam_version,   c->grid_version, c->d_hist, c->rb.partials, v.x};
+    for (auto &e : c->graphs)
+        if (e.first == key) {
+            *out = e.second;
+            return H3D_OK;
+        }
+    if (c->graphs.size() >= 8) drop_graphs(c);
     hipStream_t s = c->stream;
-    int rc;
-    if ((rc = ensure_hist(c, std::max(maxItr, 1))) != H3D_OK) return rc;
-    if ((rc = reset_state(c, tol, maxItr, v.xbest != nullptr)) != H3D_OK) return rc;
-    // The Krylov vectors keep a zero halo, as in the reference where only
-    // their interiors are ever written (WorkBuffers starts them at zero).
-    T *vecs[] = {v.p, v.p_, v.r, v.r0, v.q, v.s, v.s_, v.t};
-    for (T *a : vecs) HIPCHK(c, launch_fill<T>(a, T(0), c->H, s));
-    // res0 = CalcRK!(r, θ, b, ...); PBiCGSTAB also copies r0 = r
-    const bool bicg = (solver == H3D_SOLVER_PBICGSTAB);
-    HIPCHK(c, launch_calc_rk<T>(v.r, bicg ? v.r0 : nullptr, v.th, v.b, g, bicg ? FIN_RES0_BICG : FIN_RES0,
-                                c->d_state, nullptr, c->rb, s));
-    if (!bicg) {
-        // z = M⁻¹ r; p = z; rho_old = (r, z)   (NonUniform.jl:147-155)
-        HIPCHK(c, precond_from_zero<T>(FIRST_COPY, v.s, v.r, nullptr, nullptr, g, smoother, c->d_state, c->H, s));
-        HIPCHK(c, launch_copy_interior<T>(v.p, v.s, g, c->d_state, s));
-        HIPCHK(c, launch_dot<T>(v.r, v.s, nullptr, g, FIN_CG_RHO0, c->d_state, nullptr, c->rb, s));
-    }
-    // capture one iteration as a graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t ce = bicg ? enqueue_bicg_iter<T>(c, v, g, smoother) : enqueue_cg_iter<T>(c, v, g, smoother);
+    const bool bicg = (solver == H3D_SOLVER_PBICGSTAB);
+    hipError_t ce = bicg ? enqueue_bicg_iter<T>(c, v, P, smoother) : enqueue_cg_iter<T>(c, v, P, smoother);
     hipError_t ee = hipStreamEndCapture(s, &graph);
     if (ce != hipSuccess || ee != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -338,21 +424,47 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Geo<T> &g, double to
     hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(c, H3D_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    c->graphs.emplace_back(key, exec);
+    *out = exec;
+    return H3D_OK;
+}
+
+// Run a Krylov solve whose vectors are in v, starting from the current θ.
+template <typename T>
+int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double tol, int maxItr, int smoother,
+           KState *fin_state)
+{
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = ensure_hist(c, std::max(maxItr, 1))) != H3D_OK) return rc;
+    if ((rc = reset_state(c, tol, maxItr, v.xbest != nullptr)) != H3D_OK) return rc;
+    // The Krylov vectors keep a zero halo, as in the reference where only
+    // their interiors are ever written (WorkBuffers starts them at zero).
+    T *vecs[] = {v.p, v.p_, v.r, v.r0, v.q, v.s, v.s_, v.t};
+    for (T *a : vecs) HIPCHK(c, launch_fill<T>(a, T(0), P.n, s));
+    // res0 = CalcRK!(r, θ, b, ...); PBiCGSTAB also copies r0 = r
+    const bool bicg = (solver == H3D_SOLVER_PBICGSTAB);
+    HIPCHK(c, p_calc_rk<T>(P, v.r, bicg ? v.r0 : nullptr, v.th, v.b, bicg ? FIN_RES0_BICG : FIN_RES0, c->d_state,
+                           nullptr, c->rb, s));
+    if (!bicg) {
+        // z = M⁻¹ r; p = z; rho_old = (r, z)   (NonUniform.jl:147-155)
+        HIPCHK(c, p_precond_from_zero<T>(P, FIRST_COPY, v.s, v.r, nullptr, nullptr, smoother, c->d_state, s));
+        HIPCHK(c, p_copy_interior<T>(P, v.p, v.s, c->d_state, s));
+        HIPCHK(c, p_dot<T>(P, v.r, v.s, nullptr, FIN_CG_RHO0, c->d_state, nullptr, c->rb, s));
+    }
+    hipGraphExec_t exec = nullptr;
+    if ((rc = iteration_graph<T>(c, solver, v, P, smoother, &exec)) != H3D_OK) return rc;
     // Replay in batches.  After each batch the state is copied into a pinned
     // slot and an event recorded; the host checks the previous batch's slot
     // while the next batch runs.
     int launched = 0, nbatch = 0;
     const int batch = 4;
-    KState st{};
     bool done = false;
     while (!done) {
         const int nb = std::min(batch, maxItr - launched);
         for (int it = 0; it < nb; ++it) {
             hipError_t e = hipGraphLaunch(exec, s);
-            if (e != hipSuccess) {
-                (void)hipGraphExecDestroy(exec);
-                return fail(c, H3D_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
-            }
+            if (e != hipSuccess) return fail(c, H3D_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
         }
         launched += nb;
         const int slot = 1 + (nbatch & 1);
@@ -366,22 +478,156 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Geo<T> &g, double to
         if (launched >= maxItr) done = true;
         ++nbatch;
     }
-    (void)hipGraphExecDestroy(exec);
     HIPCHK(c, hipStreamSynchronize(s));
-    st = c->h_state[1 + ((nbatch - 1) & 1)];
-    *fin_state = st;
+    *fin_state = c->h_state[1 + ((nbatch - 1) & 1)];
     return H3D_OK;
 }
 
-KVecs<double> fp64_vecs(h3d_ctx *c)
+// Colour-split set-up for the current λ classes and grid: the class field
+// in RB layout and the z-face conductance tables (FP64 and FP32), formed on
+// the host with the same IEEE operations, in the same order, as the
+// kernels of the standard layout form them per cell.
+int ensure_rb(h3d_ctx *c)
 {
-    KVecs<double> v;
-    v.x = v.th = c->f[H3D_F_THETA]; v.b = c->f[H3D_F_B];
-    v.xbest = nullptr;
-    v.p = c->f[H3D_F_PCG_P]; v.p_ = c->f[H3D_F_PCG_P_]; v.r = c->f[H3D_F_PCG_R];
-    v.r0 = c->f[H3D_F_PCG_R0]; v.q = c->f[H3D_F_PCG_Q]; v.s = c->f[H3D_F_PCG_S];
-    v.s_ = c->f[H3D_F_PCG_S_]; v.t = c->f[H3D_F_PCG_T_];
-    return v;
+    if (c->K <= 0) return H3D_ERR_UNSUPPORTED;
+    if (c->rb_lam == c->lam_version && c->rb_grid == c->grid_version && c->d_cls_rb) return H3D_OK;
+    hipStream_t s = c->stream;
+    c->HX = (c->MX + 1) / 2;
+    c->HS = (size_t)c->HX * c->MY * c->MZ;
+    if (!c->d_cls_rb) {
+        HIPCHK(c, hipMalloc(&c->d_cls_rb, 2 * c->HS));
+        HIPCHK(c, hipMemsetAsync(c->d_cls_rb, 0, 2 * c->HS, s));
+    }
+    HIPCHK(c, launch_to_rb<uint8_t>(c->d_cls_rb, c->d_cls, c->MX, c->MY, c->MZ, s));
+    const int K = c->K, NZ = c->NZ;
+    const size_t nzf = (size_t)(NZ + 1) * K * K;
+    std::vector<double> z64(nzf);
+    std::vector<float> z32(nzf);
+    const double dx0 = c->dx, dy0 = c->dy;
+    const float dx0f = (float)c->dx, dy0f = (float)c->dy;
+    for (int kf = 0; kf <= NZ; ++kf) {
+        const int mb = (kf == 0 || kf == NZ) ? 0 : K;
+        const double dzc = c->hZC[(size_t)kf + 1] - c->hZC[(size_t)kf];
+        const float dzc32 = (float)c->hZC[(size_t)kf + 1] - (float)c->hZC[(size_t)kf];
+        for (int a = 0; a < K; ++a)
+            for (int b = 0; b < K; ++b) {
+                const size_t o = ((size_t)kf * K + a) * K + b;
+                z64[o] = c->h_tab64[(size_t)(mb + a) * K + b] * dx0 * dy0 / dzc;
+                z32[o] = c->h_tab32[(size_t)(mb + a) * K + b] * dx0f * dy0f / dzc32;
+            }
+    }
+    const size_t cap = (size_t)(NZ + 1) * kMaxClasses * kMaxClasses;
+    if (!c->d_ZF64) HIPCHK(c, hipMalloc(&c->d_ZF64, cap * sizeof(double)));
+    if (!c->d_ZF32) HIPCHK(c, hipMalloc(&c->d_ZF32, cap * sizeof(float)));
+    HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipMemcpy(c->d_ZF64, z64.data(), nzf * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ZF32, z32.data(), nzf * sizeof(float), hipMemcpyHostToDevice));
+    c->rb_lam = c->lam_version;
+    c->rb_grid = c->grid_version;
+    return H3D_OK;
+}
+
+// The colour-split kernels index with 32-bit element offsets.
+bool use_rb(const h3d_ctx *c)
+{
+    const size_t cells = 2 * (size_t)((c->MX + 1) / 2) * c->MY * c->MZ;
+    return c->layout == H3D_LAYOUT_AUTO && c->K > 0 && cells < ((size_t)1 << 31);
+}
+
+// RB operator in type T from the standard one (g), for the FP64 or FP32 path.
+template <typename T>
+int make_rbgeo(h3d_ctx *c, const Geo<T> &g, RbGeo<T> *out)
+{
+    int rc;
+    if ((rc = ensure_rb(c)) != H3D_OK) return rc;
+    RbGeo<T> r{};
+    r.MX = g.MX; r.MY = g.MY; r.MZ = g.MZ; r.NX = g.NX; r.NY = g.NY; r.NZ = g.NZ;
+    r.HX = c->HX; r.HS = c->HS;
+    r.dx0 = g.dx0; r.dy0 = g.dy0; r.ddx = g.ddx; r.ddy = g.ddy;
+    for (int f = 0; f < 6; ++f) r.HC[f] = g.HC[f];
+    r.dZ = g.dZ;
+    r.cls = c->d_cls_rb;
+    r.tab = g.tab;
+    r.K = g.K;
+    r.flux = g.flux;
+    const bool dbl = sizeof(T) == sizeof(double);
+    r.ZF = dbl ? reinterpret_cast<const T *>(c->d_ZF64) : reinterpret_cast<const T *>(c->d_ZF32);
+    r.ap0 = nullptr;
+    if (g.ap0) {
+        T **buf = dbl ? reinterpret_cast<T **>(&c->d_ap0_rb) : reinterpret_cast<T **>(&c->d_ap0_rb32);
+        if (!*buf) {
+            HIPCHK(c, hipMalloc(buf, 2 * c->HS * sizeof(T)));
+            HIPCHK(c, launch_fill<T>(*buf, T(0), 2 * c->HS, c->stream));
+        }
+        HIPCHK(c, launch_to_rb<T>(*buf, g.ap0, c->MX, c->MY, c->MZ, c->stream));
+        r.ap0 = *buf;
+    }
+    rb_shape(r);
+    if (r.nb > (unsigned)c->rb.nblocks_max) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->rb.partials);
+        c->rb.partials = nullptr;
+        HIPCHK(c, hipMalloc(&c->rb.partials, (size_t)r.nb * 6 * sizeof(double)));
+        c->rb.nblocks_max = (int)r.nb;
+    }
+    *out = r;
+    return H3D_OK;
+}
+
+// The path a solve runs on: RB when the class table form is active.
+template <typename T>
+int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
+{
+    P->g = g;
+    P->rb = use_rb(c);
+    P->n = c->H;
+    if (P->rb) {
+        int rc = make_rbgeo<T>(c, g, &P->rg);
+        if (rc != H3D_OK) return rc;
+        P->n = 2 * c->HS;
+    }
+    return H3D_OK;
+}
+
+// Krylov vectors of type T for the path: RB buffers or the fields.
+template <typename T>
+int path_vecs(h3d_ctx *c, const Path<T> &P, T *out[8]);
+
+template <>
+int path_vecs<double>(h3d_ctx *c, const Path<double> &P, double *out[8])
+{
+    if (!P.rb) {
+        const int f[8] = {H3D_F_PCG_P, H3D_F_PCG_P_, H3D_F_PCG_R, H3D_F_PCG_R0,
+                          H3D_F_PCG_Q, H3D_F_PCG_S,  H3D_F_PCG_S_, H3D_F_PCG_T_};
+        for (int a = 0; a < 8; ++a) out[a] = c->f[f[a]];
+        return H3D_OK;
+    }
+    for (int a = 0; a < 8; ++a) {
+        if (!c->rb64[a]) HIPCHK(c, hipMalloc(&c->rb64[a], P.n * sizeof(double)));
+        out[a] = c->rb64[a];
+    }
+    return H3D_OK;
+}
+
+template <>
+int path_vecs<float>(h3d_ctx *c, const Path<float> &P, float *out[8])
+{
+    if (!P.rb) {
+        const int f[8] = {F32_P, F32_P_, F32_R, F32_R0, F32_Q, F32_S, F32_S_, F32_T};
+        for (int a = 0; a < 8; ++a) out[a] = c->g32[f[a]];
+        return H3D_OK;
+    }
+    for (int a = 0; a < 8; ++a) {
+        if (!c->rb32[a]) HIPCHK(c, hipMalloc(&c->rb32[a], P.n * sizeof(float)));
+        out[a] = c->rb32[a];
+    }
+    return H3D_OK;
+}
+
+template <typename T>
+void set_krylov(KVecs<T> &v, T *const w[8])
+{
+    v.p = w[0]; v.p_ = w[1]; v.r = w[2]; v.r0 = w[3]; v.q = w[4]; v.s = w[5]; v.s_ = w[6]; v.t = w[7];
 }
 
 int finish_solve(h3d_ctx *c, const KState &st, int *isconverged, int *itr, double *res0)
@@ -403,9 +649,18 @@ int solve_fp64(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int sm
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
+    Path<double> P;
+    if ((rc = make_path<double>(c, g, &P)) != H3D_OK) return rc;
+    double *w[8];
+    if ((rc = path_vecs<double>(c, P, w)) != H3D_OK) return rc;
+    KVecs<double> v;
+    v.x = v.th = c->f[H3D_F_THETA];
+    v.b = c->f[H3D_F_B];
+    v.xbest = nullptr;
+    set_krylov(v, w);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     KState st{};
-    rc = krylov<double>(c, solver, fp64_vecs(c), g, tol, maxItr, smoother, &st);
+    rc = krylov<double>(c, solver, v, P, tol, maxItr, smoother, &st);
     if (rc != H3D_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -472,12 +727,15 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
     HIPCHK(c, hipEventRecord(c->ev0, s));
     Geo<float> g32;
     if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;
+    Path<float> P32;
+    if ((rc = make_path<float>(c, g32, &P32)) != H3D_OK) return rc;
+    float *w[8];
+    if ((rc = path_vecs<float>(c, P32, w)) != H3D_OK) return rc;
     KVecs<float> v;
     v.x = c->d_corr;
     v.xbest = c->d_best;
     v.th = c->g32[F32_X]; v.b = c->g32[F32_B];
-    v.p = c->g32[F32_P]; v.p_ = c->g32[F32_P_]; v.r = c->g32[F32_R]; v.r0 = c->g32[F32_R0];
-    v.q = c->g32[F32_Q]; v.s = c->g32[F32_S]; v.s_ = c->g32[F32_S_]; v.t = c->g32[F32_T];
+    set_krylov(v, w);
     HIPCHK(c, launch_fill<float>(v.b, 0.f, c->H, s));
 
     double *r64 = c->f[H3D_F_PCG_R];
@@ -509,7 +767,7 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         HIPCHK(c, launch_fill<double>(v.x, 0.0, c->H, s));
         HIPCHK(c, launch_fill<double>(v.xbest, 0.0, c->H, s));
         KState st{};
-        rc = krylov<float>(c, solver, v, g32, inner_tol, std::max(1, maxItr - total_inner), smoother, &st);
+        rc = krylov<float>(c, solver, v, P32, inner_tol, std::max(1, maxItr - total_inner), smoother, &st);
         if (rc != H3D_OK) return rc;
         total_inner += st.itr;
         if (c->verbose) {
@@ -635,6 +893,14 @@ int h3d_destroy(h3d_ctx *c)
         if (c->f[f]) (void)hipFree(c->f[f]);
     for (int a = 0; a < F32_N; ++a)
         if (c->g32[a]) (void)hipFree(c->g32[a]);
+    drop_graphs(c);
+    for (int a = 0; a < 8; ++a) {
+        if (c->rb64[a]) (void)hipFree(c->rb64[a]);
+        if (c->rb32[a]) (void)hipFree(c->rb32[a]);
+    }
+    void *rbs[] = {c->d_cls_rb, c->d_ZF64, c->d_ZF32, c->d_ap0_rb, c->d_ap0_rb32};
+    for (void *p : rbs)
+        if (p) (void)hipFree(p);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_tab64, c->d_tab32, c->rb.partials,
                   c->d_corr, c->d_best};
@@ -661,6 +927,11 @@ int h3d_set_option(h3d_ctx *c, int option, int value)
         c->lam_version++;  // rebuild (or drop) the class tables at the next use
         return H3D_OK;
     }
+    if (option == H3D_OPT_LAYOUT) {
+        if (value != H3D_LAYOUT_AUTO && value != H3D_LAYOUT_STANDARD) return fail(c, H3D_ERR_ARG, "bad layout");
+        c->layout = value;
+        return H3D_OK;
+    }
     return fail(c, H3D_ERR_ARG, "unknown option");
 }
 
@@ -675,6 +946,10 @@ int h3d_get_option(const h3d_ctx *c, int option, int *value)
         *value = c->K;
         return H3D_OK;
     }
+    if (option == H3D_OPT_LAYOUT) {
+        *value = c->layout;
+        return H3D_OK;
+    }
     return H3D_ERR_ARG;
 }
 
@@ -687,7 +962,10 @@ int h3d_set_grid(h3d_ctx *c, double dx, double dy, const double *ZC, const doubl
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->d_ZC, ZC, mz * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_dZ, dZ, mz * sizeof(double), hipMemcpyHostToDevice));
+    c->hZC.assign(ZC, ZC + mz);
+    c->hdZ.assign(dZ, dZ + mz);
     c->grid_set = true;
+    c->grid_version++;
     return H3D_OK;
 }
 
@@ -708,6 +986,7 @@ int h3d_set_bc(h3d_ctx *c, const int type[6], const double temperature[6], const
         c->HC[f] = (type[f] == H3D_CONVECTION) ? htc[f] : 0.0;
     }
     c->bc_set = true;
+    c->grid_version++;
     return H3D_OK;
 }
 
@@ -945,29 +1224,35 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
-    double *x = c->f[H3D_F_PCG_P_], *b = c->f[H3D_F_PCG_P], *q = c->f[H3D_F_PCG_Q];
-    Geo<float> g32{};
+    // the kernels of the layout the solver runs in, on its Krylov buffers
+    Path<double> P;
+    if ((rc = make_path<double>(c, g, &P)) != H3D_OK) return rc;
+    double *w[8];
+    if ((rc = path_vecs<double>(c, P, w)) != H3D_OK) return rc;
+    double *x = w[1], *b = w[0], *q = w[4], *r0 = w[3];
+    Path<float> P32;
+    float *w32[8] = {};
     if (which >= 3) {
+        Geo<float> g32{};
         if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;
-        HIPCHK(c, launch_convert<float, double>(c->g32[F32_X], x, c->H, c->stream));
-        HIPCHK(c, launch_convert<float, double>(c->g32[F32_B], b, c->H, c->stream));
+        if ((rc = make_path<float>(c, g32, &P32)) != H3D_OK) return rc;
+        if ((rc = path_vecs<float>(c, P32, w32)) != H3D_OK) return rc;
+        HIPCHK(c, launch_convert<float, double>(w32[1], x, P.n, c->stream));
+        HIPCHK(c, launch_convert<float, double>(w32[0], b, P.n, c->stream));
     }
-    float *x32 = c->g32[F32_X], *b32 = c->g32[F32_B], *q32 = c->g32[F32_Q];
     auto once = [&]() -> hipError_t {
         if (which == 3) {
-            hipError_t e = launch_rbsor<float>(x32, b32, g32, 0, c->d_unit, c->stream);
-            return e != hipSuccess ? e : launch_rbsor<float>(x32, b32, g32, 1, c->d_unit, c->stream);
+            hipError_t e = p_sweep<float>(P32, w32[1], w32[0], 0, c->d_unit, c->stream);
+            return e != hipSuccess ? e : p_sweep<float>(P32, w32[1], w32[0], 1, c->d_unit, c->stream);
         }
         if (which == 4)
-            return launch_calc_ax<float>(q32, x32, g32, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
+            return p_calc_ax<float>(P32, w32[4], w32[1], nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
         if (which == 0) {
-            hipError_t e = launch_rbsor<double>(x, b, g, 0, c->d_unit, c->stream);
-            return e != hipSuccess ? e : launch_rbsor<double>(x, b, g, 1, c->d_unit, c->stream);
+            hipError_t e = p_sweep<double>(P, x, b, 0, c->d_unit, c->stream);
+            return e != hipSuccess ? e : p_sweep<double>(P, x, b, 1, c->d_unit, c->stream);
         }
-        if (which == 1)
-            return launch_calc_ax<double>(q, x, g, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
-        return launch_calc_ax<double>(q, x, g, c->f[H3D_F_PCG_R0], nullptr, FIN_STORE, c->d_unit, nullptr, c->rb,
-                                      c->stream);
+        if (which == 1) return p_calc_ax<double>(P, q, x, nullptr, nullptr, -1, c->d_unit, nullptr, c->rb, c->stream);
+        return p_calc_ax<double>(P, q, x, r0, nullptr, FIN_STORE, c->d_unit, nullptr, c->rb, c->stream);
     };
     HIPCHK(c, once());
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -95,7 +95,49 @@ struct Geo {
 
 constexpr int kMaxClasses = 32;
 
+// Colour-split ("RB") layout of the solver's Krylov vectors.  Cell
+// (i,j,k) of the (MX,MY,MZ) array lives in half h = (i+j+k) & 1 at
+//   h*HS + (i >> 1) + HX*(j + MY*k),   HX = (MX+1)/2,  HS = HX*MY*MZ,
+// so the cells one Gauss-Seidel colour updates are contiguous and every
+// neighbour of a cell is in the other half at the same (j,k) row: the
+// i-neighbours at m-1+s and m+s (m = i>>1, s = i&1), the others at m.
+// Colour c of rbsor_core! updates half 1-c.  Entries with i >= MX are
+// padding and never read.  The operator takes its coefficients from the
+// class table form only: the class field in RB layout, the λf table, and a
+// per-face table of the z conductances,
+//   ZF[kf][a][b] = ((λf(v_a, v_b, m) * dx0) * dy0) / (ZC[kf+1] - ZC[kf]),
+// for the face between planes kf and kf+1 (m = 0 on the two boundary faces,
+// 1 inside), which is the value CalcAX! forms for the zm term of plane
+// kf+1 and the zp term of plane kf (λf is symmetric in its two values).
+template <typename T>
+struct RbGeo {
+    int MX, MY, MZ, NX, NY, NZ;
+    int HX;                 // half-row length
+    size_t HS;              // half size
+    T dx0, dy0, ddx, ddy;
+    T HC[6];
+    const T *dZ;            // device, MZ entries
+    const T *ZF;            // device, (NZ+1)*K*K face conductances
+    const T *ap0;           // device, RB layout; nullptr when steady
+    const uint8_t *cls;     // device, RB layout class field
+    const T *tab;           // device [2][K][K] λf table
+    int K;
+    int flux;               // as Geo::flux
+    // launch grids (rb_shape): reducing kernels march KC planes per thread
+    // so that at most ~4k workgroups write partials; the others take one
+    // plane per thread.  Workgroup counts are rounded up to multiples of 8.
+    int gx, gy, gz, KC;
+    unsigned nb;            // reducing kernels: gx*gy*gz workgroups
+    unsigned nb1;           // other kernels: gx*gy*NZ workgroups
+};
+
+// fills the launch grid fields of g
+template <typename T> void rb_shape(RbGeo<T> &g);
+
 // ---- launchers (h3d_kernels.hip); all return hipError_t of the launch -----
+// Sum nb workgroup partials of ns sums each and run finalizer fin.
+hipError_t launch_finalize_nb(int ns, unsigned nb, int fin, KState *st, double *hist, const RedBuf &rb,
+                              hipStream_t s);
 template <typename T> hipError_t launch_fill(T *a, T v, size_t n, hipStream_t s);
 template <typename T>
 hipError_t launch_copy_interior(T *dst, const T *src, const Geo<T> &g, const KState *st, hipStream_t s);
@@ -169,5 +211,41 @@ hipError_t launch_classify(uint8_t *cls, const double *lam, const uint64_t *vals
 template <typename D, typename S> hipError_t launch_convert(D *dst, const S *src, size_t n, hipStream_t s);
 // θ += d on the interior: the refinement update
 hipError_t launch_add_correction(double *th, const double *d, const Geo<double> &g, hipStream_t s);
+
+// ---- RB-layout launchers (h3d_rb.hip) ---------------------------------------
+// dst (RB) = src (standard layout), every cell including the halo
+template <typename D> hipError_t launch_to_rb(D *dst, const D *src, int MX, int MY, int MZ, hipStream_t s);
+// dst (standard) = src (RB), every cell including the halo
+template <typename D> hipError_t launch_from_rb(D *dst, const D *src, int MX, int MY, int MZ, hipStream_t s);
+template <typename T>
+hipError_t launch_rb_copy_interior(T *dst, const T *src, const RbGeo<T> &g, const KState *st, hipStream_t s);
+template <typename T>
+hipError_t launch_rb_calc_ax(T *ax, const T *x, const RbGeo<T> &g, const T *dot_y, const T *dot_z, int fin,
+                             KState *st, double *hist, const RedBuf &rb, hipStream_t s);
+// r (RB) = b - A θ with θ and b in standard layout
+template <typename T>
+hipError_t launch_rb_calc_rk(T *r, T *r0copy, const T *th, const T *b, const RbGeo<T> &g, int fin, KState *st,
+                             double *hist, const RedBuf &rb, hipStream_t s);
+template <typename T>
+hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s);
+template <typename T>
+hipError_t launch_rb_first(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const KState *st,
+                           hipStream_t s);
+template <typename T>
+hipError_t launch_rb_form_rhs(int mode, T *bb, const T *r, const T *q, const RbGeo<T> &g, const KState *st,
+                              hipStream_t s);
+template <typename T>
+hipError_t launch_rb_dot(const T *x, const T *y, const T *z, const RbGeo<T> &g, int fin, KState *st,
+                         double *hist, const RedBuf &rb, hipStream_t s);
+// θ and xbest in standard layout (FP64), the rest RB
+template <typename T>
+hipError_t launch_rb_bicg_update(double *th, double *xbest, T *r, const T *p_, const T *s_, const T *t,
+                                 const T *sv, const T *r0, const RbGeo<T> &g, KState *st, double *hist,
+                                 const RedBuf &rb, hipStream_t s);
+template <typename T>
+hipError_t launch_rb_cg_update(double *th, double *xbest, T *r, const T *p, const T *q, const RbGeo<T> &g,
+                               KState *st, double *hist, const RedBuf &rb, hipStream_t s);
+template <typename T>
+hipError_t launch_rb_cg_p(T *p, const T *z, const RbGeo<T> &g, const KState *st, hipStream_t s);
 
 }  // namespace h3d

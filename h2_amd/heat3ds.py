@@ -245,6 +245,11 @@ class WorkBuffers:
         """L.COEF_AUTO (class tables when possible) or L.COEF_DIRECT."""
         _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_COEF_FORM, int(form)))
 
+    def set_layout(self, layout):
+        """L.LAYOUT_AUTO (colour-split Krylov vectors with the table form) or
+        L.LAYOUT_STANDARD (the reference's layout); results are identical."""
+        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_LAYOUT, int(layout)))
+
     def num_classes(self):
         v = ctypes.c_int()
         _check(self.ctx, self.lib.h3d_get_option(self.ctx, L.OPT_NUM_CLASSES, ctypes.byref(v)))

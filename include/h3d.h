@@ -82,6 +82,9 @@ extern "C" {
 #define H3D_COEF_AUTO 0        /* class tables when λ has at most 32 distinct values, else direct */
 #define H3D_COEF_DIRECT 1      /* always the reference's per-face harmonic mean of the λ field */
 #define H3D_OPT_NUM_CLASSES 1  /* read-only: distinct λ values in use by the table form (0 = direct) */
+#define H3D_OPT_LAYOUT 2       /* device layout of the solver's Krylov vectors */
+#define H3D_LAYOUT_AUTO 0      /* colour-split (each red-black colour contiguous) when the table form is active */
+#define H3D_LAYOUT_STANDARD 1  /* the reference's column-major layout */
 
 typedef struct h3d_ctx h3d_ctx;
 
@@ -95,8 +98,10 @@ const char *h3d_create_error(void);
 
 /* Both coefficient forms give bitwise identical operators; the table form
  * (a one-byte class per cell and a λf table per class pair) moves 1 byte
- * per cell instead of 8 and drops the per-face division.  No reference
- * counterpart: the reference has a single (direct) form. */
+ * per cell instead of 8 and drops the per-face division.  Both layouts give
+ * bitwise identical solves: the colour-split layout only changes where the
+ * Krylov vectors' elements live (fields keep the reference's layout).  No
+ * reference counterpart: the reference has one form and one layout. */
 int h3d_set_option(h3d_ctx *ctx, int option, int value);
 int h3d_get_option(const h3d_ctx *ctx, int option, int *value);
 

@@ -1,0 +1,50 @@
+"""The colour-split layout of the solver's Krylov vectors against the
+reference's layout: whole solves must be bitwise identical (same iteration
+count, residual history and θ), for both solvers and smoothers, steady and
+transient, FP64 and the FP32 refinement, on grids with an even and an odd
+number of stored cells per row (the odd one has padding entries)."""
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no HIP device")]
+
+import h2_amd
+from h2_amd import _lib as L
+from h2_amd.heat3ds import WorkBuffers, load_model
+
+
+def run(shape, layout, solver, smoother, steady, precision, tol):
+    m = h2_amd.build_model(*shape)
+    wk = WorkBuffers(*m.shape)
+    wk.set_layout(layout)
+    load_model(wk, m)
+    wk.calc_rhs(10000.0, steady)
+    res = wk.solve(solver, 10000.0, tol, 3000, smoother, steady, precision=precision)
+    out = (res, wk.history(), wk.download("theta"), wk.num_classes())
+    wk.close()
+    return out
+
+
+@pytest.mark.parametrize("shape", [(40, 40, 30), (17, 9, 30), (13, 70, 60)])
+@pytest.mark.parametrize("solver,smoother", [(L.SOLVER_PBICGSTAB, L.SMOOTHER_GS), (L.SOLVER_CG, L.SMOOTHER_GS),
+                                             (L.SOLVER_PBICGSTAB, L.SMOOTHER_NONE)])
+@pytest.mark.parametrize("steady", [True, False])
+def test_fp64_layouts_bitwise(shape, solver, smoother, steady):
+    tol = 1e-8 if smoother == L.SMOOTHER_GS else 1e-5
+    a = run(shape, L.LAYOUT_AUTO, solver, smoother, steady, L.FP64, tol)
+    b = run(shape, L.LAYOUT_STANDARD, solver, smoother, steady, L.FP64, tol)
+    assert a[3] > 0, "the table form (and so the colour-split layout) must be active"
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("shape", [(40, 40, 30), (17, 9, 30)])
+def test_fp32_refined_layouts_bitwise(shape):
+    a = run(shape, L.LAYOUT_AUTO, L.SOLVER_PBICGSTAB, L.SMOOTHER_GS, True, L.FP32_REFINED, 1e-8)
+    b = run(shape, L.LAYOUT_STANDARD, L.SOLVER_PBICGSTAB, L.SMOOTHER_GS, True, L.FP32_REFINED, 1e-8)
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
