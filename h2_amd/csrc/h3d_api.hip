@@ -114,6 +114,16 @@ int fail(h3d_ctx *c, int code, const std::string &msg)
 
 bool field_ok(int f) { return f >= 0 && f < H3D_NFIELDS; }
 
+// reduction scratch for up to nb workgroups, with the pre-reduction area after it
+hipError_t alloc_partials(RedBuf *rb, long nb)
+{
+    rb->nblocks_max = (int)std::min<long>(nb, 1L << 30);
+    const size_t n1 = (size_t)rb->nblocks_max * 6, n2 = ((size_t)rb->nblocks_max / 1024 + 2) * 6;
+    hipError_t e = hipMalloc(&rb->partials, (n1 + n2) * sizeof(double));
+    rb->partials2 = e == hipSuccess ? rb->partials + n1 : nullptr;
+    return e;
+}
+
 template <typename T>
 Geo<T> base_geo(const h3d_ctx *c)
 {
@@ -527,11 +537,11 @@ int ensure_rb(h3d_ctx *c)
     return H3D_OK;
 }
 
-// The colour-split kernels index with 32-bit element offsets.
+// The colour-split kernels address each half with 32-bit byte offsets.
 bool use_rb(const h3d_ctx *c)
 {
-    const size_t cells = 2 * (size_t)((c->MX + 1) / 2) * c->MY * c->MZ;
-    return c->layout == H3D_LAYOUT_AUTO && c->K > 0 && cells < ((size_t)1 << 31);
+    const size_t half = (size_t)((c->MX + 1) / 2) * c->MY * c->MZ;
+    return c->layout == H3D_LAYOUT_AUTO && c->K > 0 && 2 * half * sizeof(double) < ((size_t)1 << 32);
 }
 
 // RB operator in type T from the standard one (g), for the FP64 or FP32 path.
@@ -563,12 +573,12 @@ int make_rbgeo(h3d_ctx *c, const Geo<T> &g, RbGeo<T> *out)
         r.ap0 = *buf;
     }
     rb_shape(r);
-    if (r.nb > (unsigned)c->rb.nblocks_max) {
+    // CalcAX! launches two workgroups per (m-block, row block, plane)
+    if (2 * r.nb1 > (unsigned)c->rb.nblocks_max) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         (void)hipFree(c->rb.partials);
-        c->rb.partials = nullptr;
-        HIPCHK(c, hipMalloc(&c->rb.partials, (size_t)r.nb * 6 * sizeof(double)));
-        c->rb.nblocks_max = (int)r.nb;
+        c->rb.partials = c->rb.partials2 = nullptr;
+        if (alloc_partials(&c->rb, 2 * (long)r.nb1) != hipSuccess) return fail(c, H3D_ERR_HIP, "hipMalloc partials");
     }
     *out = r;
     return H3D_OK;
@@ -873,9 +883,7 @@ int h3d_create(h3d_ctx **out, int MX, int MY, int MZ, int device)
         return bail("hipHostMalloc", e);
     // reduction scratch: enough partials for any interior launch shape
     const long maxb = ((long)(c->NX + 63) / 64) * ((c->NY + 3) / 4) * (long)c->NZ + 64;
-    c->rb.nblocks_max = (int)std::min<long>(maxb, 1L << 30);
-    if ((e = hipMalloc(&c->rb.partials, (size_t)c->rb.nblocks_max * 6 * sizeof(double))) != hipSuccess)
-        return bail("hipMalloc partials", e);
+    if ((e = alloc_partials(&c->rb, maxb)) != hipSuccess) return bail("hipMalloc partials", e);
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail("event", e);
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
     for (int a = 0; a < 2; ++a)

@@ -51,45 +51,79 @@ __device__ __forceinline__ dd dd_add(dd x, dd y)
     return quick_two_sum(u.hi, u.lo + t.lo);
 }
 
-__device__ __forceinline__ dd wave_sum(dd v)
+// v of lane l-N of the same 16-lane row, 0 where there is none (DPP
+// row_shr, a VALU operation, on both 32-bit halves).
+template <int N>
+__device__ __forceinline__ double row_shr(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        dd o;
-        o.hi = __shfl_xor(v.hi, off, 64);
-        o.lo = __shfl_xor(v.lo, off, 64);
-        v = dd_add(v, o);
-    }
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u & 0xffffffffull), 0x110 + N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x110 + N, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Inclusive scan within each 16-lane row: lane 15 of a row ends with the
+// row's sum.  All lanes of the wave must be active.
+__device__ __forceinline__ dd row_sum(dd v)
+{
+    v = dd_add(v, dd{row_shr<1>(v.hi), row_shr<1>(v.lo)});
+    v = dd_add(v, dd{row_shr<2>(v.hi), row_shr<2>(v.lo)});
+    v = dd_add(v, dd{row_shr<4>(v.hi), row_shr<4>(v.lo)});
+    v = dd_add(v, dd{row_shr<8>(v.hi), row_shr<8>(v.lo)});
     return v;
 }
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also orders
+// global memory, which makes every wave wait for its outstanding global
+// stores first; the reductions exchange data through LDS alone.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Sum up to three per-thread values over the NT = 256-thread workgroup:
+// row sums (16 of them) through LDS into the first row of wave 0, whose
+// lane 15 ends with the totals (the return value is meaningful there only:
+// leader() is true).  Every thread must call it.
+template <int NS>
+__device__ __forceinline__ void block_sum(const dd (&v)[3], dd (&out)[3])
+{
+    static_assert(NT == 256, "16 row sums fill one row");
+    __shared__ dd sh[3][16];
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const dd w = row_sum(v[s]);
+        if ((lane & 15) == 15) sh[s][wave * 4 + (lane >> 4)] = w;
+    }
+    lds_barrier();
+    if (wave == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) out[s] = row_sum(lane < 16 ? sh[s][lane] : dd{0.0, 0.0});
+    }
+}
+
+__device__ __forceinline__ bool block_leader() { return threadIdx.x + BX * threadIdx.y == 15; }
 
 // Reduce up to three per-thread sums over the workgroup and store the
 // workgroup's partials.  Every thread must call it.
 template <int NS>
 __device__ __forceinline__ void block_partials(const dd (&v)[3], const RedBuf &rb)
 {
-    __shared__ dd sh[3][NWAVE];
-    const int tid = threadIdx.x + BX * threadIdx.y;
-    const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const dd w = wave_sum(v[s]);
-        if (lane == 0) sh[s][wave] = w;
-    }
-    __syncthreads();
-    if (tid == 0) {
+    dd t[3];
+    block_sum<NS>(v, t);
+    if (block_leader()) {
         const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            dd b = sh[s][0];
-#pragma unroll
-            for (int w = 1; w < NWAVE; ++w) b = dd_add(b, sh[s][w]);
-            rb.partials[6 * (size_t)bid + 2 * s] = b.hi;
-            rb.partials[6 * (size_t)bid + 2 * s + 1] = b.lo;
+            rb.partials[6 * (size_t)bid + 2 * s] = t[s].hi;
+            rb.partials[6 * (size_t)bid + 2 * s + 1] = t[s].lo;
         }
     }
 }
-
 
 __device__ __forceinline__ bool halted(const KState *st) { return st != nullptr && st->status != ST_RUNNING; }
 

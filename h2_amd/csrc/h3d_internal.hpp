@@ -57,10 +57,14 @@ struct KState {
 
 // Per-workgroup partial sums of a reducing kernel (up to three sums per
 // workgroup, each a double-double hi/lo pair), read by the finalize kernel.
+// More than kFinalizeMax partials are first pre-reduced 1024 to one into
+// partials2 by a many-workgroup kernel.
 struct RedBuf {
     double *partials;       // [nblocks_max][3][2]
+    double *partials2;      // [nblocks_max/1024 + 1][3][2]
     int nblocks_max;
 };
+constexpr unsigned kFinalizeMax = 4096;
 
 // Per-solve operator constants.  T is the storage/arithmetic type of the
 // stencil (double for the reference iteration, float for the FP32 inner
@@ -123,12 +127,10 @@ struct RbGeo {
     const T *tab;           // device [2][K][K] λf table
     int K;
     int flux;               // as Geo::flux
-    // launch grids (rb_shape): reducing kernels march KC planes per thread
-    // so that at most ~4k workgroups write partials; the others take one
-    // plane per thread.  Workgroup counts are rounded up to multiples of 8.
-    int gx, gy, gz, KC;
-    unsigned nb;            // reducing kernels: gx*gy*gz workgroups
-    unsigned nb1;           // other kernels: gx*gy*NZ workgroups
+    // launch grid (rb_shape): one plane per thread, gx*gy*NZ workgroups
+    // rounded up to a multiple of 8
+    int gx, gy;
+    unsigned nb1;
 };
 
 // fills the launch grid fields of g

@@ -251,37 +251,77 @@ __device__ void finalize(int fin, const double *s, KState *st, double *hist)
 
 
 // Sum nb workgroup partials in a fixed order and run the finaliser.  One
-// workgroup of NT threads.
+// workgroup of NT threads; each thread issues its loads four at a time
+// before adding them.
 template <int NS>
 __global__ void __launch_bounds__(256) k_finalize(const double *partials, unsigned nb, int fin, KState *st,
                                                   double *hist)
 {
     if (halted(st)) return;
-    __shared__ dd sh[3][NWAVE];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
     dd acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-    for (unsigned b = tid; b < nb; b += NT) {
+    unsigned b = tid;
+    for (; b + 3 * NT < nb; b += 4 * NT) {
+        double v[4][3][2];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                v[u][s][0] = partials[6 * (size_t)(b + u * NT) + 2 * s];
+                v[u][s][1] = partials[6 * (size_t)(b + u * NT) + 2 * s + 1];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc[s] = dd_add(acc[s], dd{v[u][s][0], v[u][s][1]});
+    }
+    for (; b < nb; b += NT) {
 #pragma unroll
         for (int s = 0; s < NS; ++s)
             acc[s] = dd_add(acc[s], dd{partials[6 * (size_t)b + 2 * s], partials[6 * (size_t)b + 2 * s + 1]});
     }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const dd w = wave_sum(acc[s]);
-        if (lane == 0) sh[s][wave] = w;
-    }
-    __syncthreads();
-    if (tid == 0) {
+    dd t[3];
+    block_sum<NS>(acc, t);
+    if (block_leader()) {
         double tot[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            dd t = sh[s][0];
-#pragma unroll
-            for (int w = 1; w < NWAVE; ++w) t = dd_add(t, sh[s][w]);
-            tot[s] = t.hi + t.lo;
-        }
+        for (int s = 0; s < NS; ++s) tot[s] = t[s].hi + t[s].lo;
         finalize(fin, tot, st, hist);
+    }
+}
+
+// Pre-reduction for large grids: workgroup w sums partials [1024w, 1024w+1024)
+// (four per thread) into partial w of out.
+template <int NS>
+__global__ void __launch_bounds__(256) k_prereduce(const double *partials, unsigned nb, double *out,
+                                                   const KState *st)
+{
+    if (halted(st)) return;
+    const int tid = threadIdx.x;
+    dd acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    double v[4][3][2] = {};
+    const unsigned b0 = blockIdx.x * 1024u + tid;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const unsigned b = b0 + u * NT;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            v[u][s][0] = b < nb ? partials[6 * (size_t)b + 2 * s] : 0.0;
+            v[u][s][1] = b < nb ? partials[6 * (size_t)b + 2 * s + 1] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[s] = dd_add(acc[s], dd{v[u][s][0], v[u][s][1]});
+    dd t[3];
+    block_sum<NS>(acc, t);
+    if (block_leader()) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            out[6 * (size_t)blockIdx.x + 2 * s] = t[s].hi;
+            out[6 * (size_t)blockIdx.x + 2 * s + 1] = t[s].lo;
+        }
     }
 }
 
@@ -289,9 +329,21 @@ hipError_t launch_finalize_nb(int ns, unsigned nb, int fin, KState *st, double *
                               hipStream_t s)
 {
     if (nb > (unsigned)rb.nblocks_max) return hipErrorInvalidValue;
-    if (ns == 1) hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(NT), 0, s, rb.partials, nb, fin, st, hist);
-    else if (ns == 2) hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(NT), 0, s, rb.partials, nb, fin, st, hist);
-    else hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(NT), 0, s, rb.partials, nb, fin, st, hist);
+    const double *src = rb.partials;
+    if (nb > kFinalizeMax) {
+        const unsigned nb2 = (nb + 1023) / 1024;
+        if (ns == 1) hipLaunchKernelGGL(k_prereduce<1>, dim3(nb2), dim3(NT), 0, s, rb.partials, nb, rb.partials2, st);
+        else if (ns == 2)
+            hipLaunchKernelGGL(k_prereduce<2>, dim3(nb2), dim3(NT), 0, s, rb.partials, nb, rb.partials2, st);
+        else hipLaunchKernelGGL(k_prereduce<3>, dim3(nb2), dim3(NT), 0, s, rb.partials, nb, rb.partials2, st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        src = rb.partials2;
+        nb = nb2;
+    }
+    if (ns == 1) hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(NT), 0, s, src, nb, fin, st, hist);
+    else if (ns == 2) hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(NT), 0, s, src, nb, fin, st, hist);
+    else hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(NT), 0, s, src, nb, fin, st, hist);
     return hipGetLastError();
 }
 

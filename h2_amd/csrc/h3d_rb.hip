@@ -16,13 +16,15 @@
 // which does not change a sum's bits.
 //
 // Work decomposition: a workgroup is 64 (half-row index m) by 4 (rows j)
-// threads; each thread of a reducing kernel marches over KC planes, of the
-// others it takes one plane.  "Pair" kernels handle the
-// two cells i = 2m and 2m+1 of a thread (one in each half); the sweep
-// handles the one cell of the colour being updated.  Workgroups are
-// renumbered so that each XCD (workgroups are dealt to the eight XCDs
-// round-robin) receives a contiguous range of rows and planes, keeping the
-// rows and planes neighbouring workgroups share in one L2.
+// threads of one plane.  "Pair" kernels handle the two cells i = 2m and
+// 2m+1 of a thread (one in each half); the sweep handles the one cell of
+// the colour being updated.  Every thread computes from clamped, always
+// in-bounds indices and only its stores and sums are predicated on the
+// cell being an interior cell, so no branch separates a thread's loads and
+// they are all in flight together.  Workgroups are renumbered so that each
+// XCD (workgroups are dealt to the eight XCDs round-robin) receives a
+// contiguous range of rows and planes, keeping the rows and planes
+// neighbouring workgroups share in one L2.
 
 #include "h3d_device.hpp"
 
@@ -33,59 +35,62 @@ void rb_shape(RbGeo<T> &g)
 {
     g.gx = (g.HX + BX - 1) / BX;
     g.gy = (g.NY + BY - 1) / BY;
-    const long nxy = (long)g.gx * g.gy;
-    // planes per thread: keep the grid at no more than ~4k workgroups (the
-    // finalize kernel reads one partial per workgroup), then balance
-    int kc = (int)((nxy * g.NZ + 4095) / 4096);
-    if (kc < 1) kc = 1;
-    if (kc > g.NZ) kc = g.NZ;
-    g.gz = (g.NZ + kc - 1) / kc;
-    g.KC = (g.NZ + g.gz - 1) / g.gz;
-    const unsigned nb = (unsigned)(nxy * g.gz);
-    g.nb = (nb + 7u) & ~7u;
-    const unsigned nb1 = (unsigned)(nxy * g.NZ);
+    const unsigned nb1 = (unsigned)((long)g.gx * g.gy * g.NZ);
     g.nb1 = (nb1 + 7u) & ~7u;
 }
 template void rb_shape<double>(RbGeo<double> &);
 template void rb_shape<float>(RbGeo<float> &);
 
-struct RbCoord {
-    int m, j, kb, ke;
+// The thread's plane k and row j, and its half-row index m clamped to a
+// loadable cell; live = the thread owns cells (inside the grid, j <= NY,
+// m < HX).  Workgroup p is on XCD p mod 8; logical workgroup
+// L = (nb/8)*(p mod 8) + p/8 gives each XCD a contiguous range of L,
+// decoded m-block fastest, then the plane, then the row block.
+struct RbThread {
+    int m, j, k;
+    bool live;
 };
 
-// Workgroup p is on XCD p mod 8; logical workgroup L = (nb/8)*(p mod 8) + p/8
-// gives each XCD a contiguous range of L, decoded m-block fastest, then the
-// plane block, then the row block.  RED selects the reducing kernels' grid
-// (KC planes per thread) over the one-plane grid.
-template <bool RED, typename T>
-__device__ __forceinline__ RbCoord rb_coord(const RbGeo<T> &g)
+template <typename T>
+__device__ __forceinline__ RbThread rb_thread(const RbGeo<T> &g)
 {
-    const unsigned nb = RED ? g.nb : g.nb1;
-    const unsigned gz = RED ? (unsigned)g.gz : (unsigned)g.NZ;
-    const int KC = RED ? g.KC : 1;
+    const unsigned nb = g.nb1;
     const unsigned p = blockIdx.x;
     const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
+    const unsigned gz = (unsigned)g.NZ;
     const unsigned nreal = (unsigned)g.gx * (unsigned)g.gy * gz;
     const unsigned bx = L % (unsigned)g.gx, t = L / (unsigned)g.gx;
     const unsigned bz = t % gz, by = t / gz;
-    RbCoord c;
-    c.m = (int)bx * BX + threadIdx.x;
-    c.j = (int)by * BY + threadIdx.y + 1;
-    c.kb = 1 + (int)bz * KC;
-    c.ke = min(c.kb + KC, g.NZ + 1);
-    if (L >= nreal || c.j > g.NY) c.ke = c.kb;  // no cells
-    return c;
+    const int m = (int)bx * BX + threadIdx.x;
+    const int j = (int)by * BY + threadIdx.y + 1;
+    RbThread r;
+    r.live = L < nreal && j <= g.NY && m < g.HX;
+    r.m = min(m, g.HX - 1);
+    r.j = min(j, g.NY);
+    r.k = (int)bz + 1;
+    return r;
 }
 
-// λf table into LDS (all threads must call)
+// Stage into LDS the conductance tables of the workgroup's plane k, each
+// 2K² entries indexed [(m ? K : 0) + neighbour class][centre class]:
+//   XT = ((λf * dy0) * ΔZ[k]) * ddx,   YT = ((λf * dx0) * ΔZ[k]) * ddy,
+// the products CalcAX! forms per face, in its order; then the z-face
+// conductances of faces k-1 and k (K² each).  All threads must call.
 template <typename T>
-__device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g)
+__device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g, int k)
 {
     extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
     T *t = reinterpret_cast<T *>(h3d_dyn_smem);
     const int n = 2 * g.K * g.K;
+    const T dz_k = g.dZ[k];
+    const T *zf = g.ZF + (size_t)(k - 1) * g.K * g.K;
     const int tid = threadIdx.x + BX * threadIdx.y;
-    for (int a = tid; a < n; a += NT) t[a] = g.tab[a];
+    for (int a = tid; a < n; a += NT) {
+        const T f = g.tab[a];
+        t[a] = f * g.dy0 * dz_k * g.ddx;
+        t[n + a] = f * g.dx0 * dz_k * g.ddy;
+        t[2 * n + a] = zf[a];
+    }
     __syncthreads();
     return t;
 }
@@ -93,7 +98,22 @@ __device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g)
 template <typename T>
 static size_t rb_lds(const RbGeo<T> &g)
 {
-    return (size_t)2 * g.K * g.K * sizeof(T);
+    return (size_t)6 * g.K * g.K * sizeof(T);
+}
+
+// Byte offsets of a whole RB array (both halves) fit in 32 bits (the host
+// checks): loads take a uniform base and one 32-bit offset register, and a
+// neighbour's offset is one integer add.
+template <typename T>
+__device__ __forceinline__ T ldb(const T *__restrict__ p, unsigned boff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + boff);
+}
+
+template <typename T>
+__device__ __forceinline__ void stb(T *__restrict__ p, unsigned boff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff) = v;
 }
 
 // The six neighbour values of a cell, all in the other half.
@@ -102,55 +122,63 @@ struct Nb {
     T xm, xp, ym, yp, zm, zp;
 };
 
-// Offsets of a cell in the RB layout: own (its half), oth (same row, other
-// half); s = i & 1.
+// A cell in the RB layout: co = its offset (in its half's array), no = the
+// offset of the same (m, j, k) in the other half's array; s = i & 1.
 struct RbCell {
     unsigned co, no;
-    int s;
+    int s, i;
+    bool valid;
 };
 
+// Neighbour values from the other half's array xt (element offset no)
 template <typename T>
-__device__ __forceinline__ Nb<T> rb_nb(const T *__restrict__ x, const RbCell &q, unsigned HX, unsigned HXY)
+__device__ __forceinline__ Nb<T> rb_nb(const T *__restrict__ xt, unsigned no, int s, unsigned HX, unsigned HXY)
 {
+    const unsigned b = no * (unsigned)sizeof(T);
+    const unsigned sx = (unsigned)sizeof(T) * (unsigned)s;
+    const unsigned bx = HX * (unsigned)sizeof(T), bz = HXY * (unsigned)sizeof(T);
     Nb<T> n;
-    n.xm = x[q.no - 1 + q.s];
-    n.xp = x[q.no + q.s];
-    n.ym = x[q.no - HX];
-    n.yp = x[q.no + HX];
-    n.zm = x[q.no - HXY];
-    n.zp = x[q.no + HXY];
+    n.xm = ldb(xt, b - (unsigned)sizeof(T) + sx);
+    n.xp = ldb(xt, b + sx);
+    n.ym = ldb(xt, b - bx);
+    n.yp = ldb(xt, b + bx);
+    n.zm = ldb(xt, b - bz);
+    n.zp = ldb(xt, b + bz);
     return n;
 }
 
 // The conductances of cell (i,j,k) from the class field and tables; the
-// same values conductances() forms in h3d_kernels.hip.
+// same values conductances() forms in h3d_kernels.hip.  clo/clt: the class
+// arrays of the cell's half and of the other half; ap: ap0 of the cell's
+// half or nullptr.
 template <typename T>
-__device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, int i, int j, int k, const RbCell &q,
-                                           unsigned HX, unsigned HXY)
+__device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, const uint8_t *__restrict__ clo,
+                                           const uint8_t *__restrict__ clt, const T *__restrict__ ap, int i, int j,
+                                           int k, unsigned co, unsigned no, int s, unsigned HX, unsigned HXY)
 {
-    const int K = g.K;
-    const uint8_t *cl = g.cls;
-    const int b = cl[q.co];
+    const int K = g.K, n2 = 2 * K * K;
+    const int b = clo[co];
     const bool ixm = i > 1, ixp = i < g.NX, iym = j > 1, iyp = j < g.NY;
-    const T dz_k = g.dZ[k];
-    const T fxm = stab[((ixm ? K : 0) + cl[q.no - 1 + q.s]) * K + b];
-    const T fxp = stab[((ixp ? K : 0) + cl[q.no + q.s]) * K + b];
-    const T fym = stab[((iym ? K : 0) + cl[q.no - HX]) * K + b];
-    const T fyp = stab[((iyp ? K : 0) + cl[q.no + HX]) * K + b];
+    // stab: XT, YT (plane k) and the two z faces, staged by rb_stage
     Cond<T> o;
-    o.xm = fxm * g.dy0 * dz_k * g.ddx;
-    o.xp = fxp * g.dy0 * dz_k * g.ddx;
-    o.ym = fym * g.dx0 * dz_k * g.ddy;
-    o.yp = fyp * g.dx0 * dz_k * g.ddy;
-    o.zm = g.ZF[((unsigned)(k - 1) * K + cl[q.no - HXY]) * K + b];
-    o.zp = g.ZF[((unsigned)k * K + cl[q.no + HXY]) * K + b];
-    o.cxm = ixm ? T(0) : g.HC[0] * g.dy0 * dz_k * T(1);
-    o.cxp = ixp ? T(0) : g.HC[1] * g.dy0 * dz_k * T(1);
-    o.cym = iym ? T(0) : g.HC[2] * g.dx0 * dz_k * T(1);
-    o.cyp = iyp ? T(0) : g.HC[3] * g.dx0 * dz_k * T(1);
-    o.czm = k > 1 ? T(0) : g.HC[4] * g.dx0 * g.dy0 * T(1);
-    o.czp = k < g.NZ ? T(0) : g.HC[5] * g.dx0 * g.dy0 * T(1);
-    o.ap0 = g.ap0 ? g.ap0[q.co] : T(0);
+    o.xm = stab[((ixm ? K : 0) + clt[no - 1 + s]) * K + b];
+    o.xp = stab[((ixp ? K : 0) + clt[no + s]) * K + b];
+    o.ym = stab[n2 + ((iym ? K : 0) + clt[no - HX]) * K + b];
+    o.yp = stab[n2 + ((iyp ? K : 0) + clt[no + HX]) * K + b];
+    o.zm = stab[2 * n2 + clt[no - HXY] * K + b];
+    o.zp = stab[2 * n2 + K * K + clt[no + HXY] * K + b];
+    // convective terms: exact zeros away from the boundary (skipped there)
+    o.cxm = o.cxp = o.cym = o.cyp = o.czm = o.czp = T(0);
+    if (!(ixm && ixp && iym && iyp && k > 1 && k < g.NZ)) {
+        const T dz_k = g.dZ[k];
+        o.cxm = ixm ? T(0) : g.HC[0] * g.dy0 * dz_k * T(1);
+        o.cxp = ixp ? T(0) : g.HC[1] * g.dy0 * dz_k * T(1);
+        o.cym = iym ? T(0) : g.HC[2] * g.dx0 * dz_k * T(1);
+        o.cyp = iyp ? T(0) : g.HC[3] * g.dx0 * dz_k * T(1);
+        o.czm = k > 1 ? T(0) : g.HC[4] * g.dx0 * g.dy0 * T(1);
+        o.czp = k < g.NZ ? T(0) : g.HC[5] * g.dx0 * g.dy0 * T(1);
+    }
+    o.ap0 = ap ? ldb(ap, co * (unsigned)sizeof(T)) : T(0);
     return o;
 }
 
@@ -179,32 +207,22 @@ __device__ __forceinline__ T rb_apply(const RbGeo<T> &g, const Cond<T> &o, const
     return (rb_offdiag(o, n) - diag(o) * x0) * T(1);
 }
 
-// For pair kernels: the two cells of thread (m, j, k).  Cell 0 is i = 2m
-// (half (j+k)&1), cell 1 is i = 2m+1 (the other half); valid[] says which
-// are interior cells.
-struct RbPair {
-    RbCell c[2];
-    int i[2];
-    bool valid[2];
-};
-
+// The two cells of a pair thread: cell a is i = 2m + a, in half h0 ^ a with
+// h0 = (j + k) & 1; offsets relative to each half's array.
 template <typename T>
-__device__ __forceinline__ RbPair rb_pair(const RbGeo<T> &g, int m, int j, int k)
+__device__ __forceinline__ void rb_pair(const RbGeo<T> &g, const RbThread &t, RbCell (&c)[2], int (&h)[2])
 {
-    RbPair p;
-    const unsigned r = (unsigned)m + (unsigned)g.HX * ((unsigned)j + (unsigned)g.MY * (unsigned)k);
-    const unsigned HS = (unsigned)g.HS;
-    const int h0 = (j + k) & 1;
+    const unsigned r = (unsigned)t.m + (unsigned)g.HX * ((unsigned)t.j + (unsigned)g.MY * (unsigned)t.k);
+    const int h0 = (t.j + t.k) & 1;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-        const int h = h0 ^ a;
-        p.i[a] = 2 * m + a;
-        p.c[a].co = (h ? HS : 0u) + r;
-        p.c[a].no = (h ? 0u : HS) + r;
-        p.c[a].s = a;
-        p.valid[a] = p.i[a] >= 1 && p.i[a] <= g.NX;
+        h[a] = h0 ^ a;
+        c[a].i = 2 * t.m + a;
+        c[a].co = r;
+        c[a].no = r;
+        c[a].s = a;
+        c[a].valid = t.live && c[a].i >= 1 && c[a].i <= g.NX;
     }
-    return p;
 }
 
 // ---------------------------------------------------------------------------
@@ -242,14 +260,27 @@ hipError_t launch_from_rb(D *dst, const D *src, int MX, int MY, int MZ, hipStrea
     return hipGetLastError();
 }
 
+// Pointers of the two halves of an RB array
+template <typename T>
+struct Halves {
+    T *h[2];
+};
+
+template <typename T>
+static Halves<T> halves(T *a, size_t HS)
+{
+    return Halves<T>{{a, a ? a + HS : nullptr}};
+}
+
 // ---------------------------------------------------------------------------
 // red-black Gauss-Seidel
 // ---------------------------------------------------------------------------
 // One colour of rbsor_core! (src/NonUniform.jl:848-915), ω = 1, in place:
 // colour c updates half h = 1-c.  The kernel gets the two halves of x as
 // separate (non-aliasing) pointers: xo = the half it updates, xt = the half
-// it reads the neighbours from; bo = b's half h; co = cls's half h, ct =
-// the other; ap = ap0's half h (or nullptr).  Same arithmetic as gs_update().
+// it reads the neighbours from; bo = b's half h; clo/clt = the class
+// field's halves; ap = ap0's half h (or nullptr).  Same arithmetic as
+// gs_update().
 template <typename T>
 __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *__restrict__ xt,
                                                   const T *__restrict__ bo, const uint8_t *__restrict__ clo,
@@ -257,56 +288,27 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
                                                   int h, const KState *st)
 {
     if (halted(st)) return;
-    const T *stab = rb_stage(g);
-    const RbCoord q = rb_coord<false>(g);
+    const RbThread t = rb_thread(g);
+    const T *stab = rb_stage(g, t.k);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
-    const int K = g.K;
-    for (int k = q.kb; k < q.ke; ++k) {
-        const int s = (h + q.j + k) & 1;
-        const int i = 2 * q.m + s;
-        if (i < 1 || i > g.NX) continue;
-        const unsigned r = (unsigned)q.m + HX * (unsigned)q.j + HXY * (unsigned)k;
-        Nb<T> n;
-        n.xm = xt[r - 1 + s];
-        n.xp = xt[r + s];
-        n.ym = xt[r - HX];
-        n.yp = xt[r + HX];
-        n.zm = xt[r - HXY];
-        n.zp = xt[r + HXY];
-        // conductances (as rb_cond)
-        const int b = clo[r];
-        const bool ixm = i > 1, ixp = i < g.NX, iym = q.j > 1, iyp = q.j < g.NY;
-        const T dz_k = g.dZ[k];
-        const T fxm = stab[((ixm ? K : 0) + clt[r - 1 + s]) * K + b];
-        const T fxp = stab[((ixp ? K : 0) + clt[r + s]) * K + b];
-        const T fym = stab[((iym ? K : 0) + clt[r - HX]) * K + b];
-        const T fyp = stab[((iyp ? K : 0) + clt[r + HX]) * K + b];
-        Cond<T> o;
-        o.xm = fxm * g.dy0 * dz_k * g.ddx;
-        o.xp = fxp * g.dy0 * dz_k * g.ddx;
-        o.ym = fym * g.dx0 * dz_k * g.ddy;
-        o.yp = fyp * g.dx0 * dz_k * g.ddy;
-        o.zm = g.ZF[((unsigned)(k - 1) * K + clt[r - HXY]) * K + b];
-        o.zp = g.ZF[((unsigned)k * K + clt[r + HXY]) * K + b];
-        o.cxm = ixm ? T(0) : g.HC[0] * g.dy0 * dz_k * T(1);
-        o.cxp = ixp ? T(0) : g.HC[1] * g.dy0 * dz_k * T(1);
-        o.cym = iym ? T(0) : g.HC[2] * g.dx0 * dz_k * T(1);
-        o.cyp = iyp ? T(0) : g.HC[3] * g.dx0 * dz_k * T(1);
-        o.czm = k > 1 ? T(0) : g.HC[4] * g.dx0 * g.dy0 * T(1);
-        o.czp = k < g.NZ ? T(0) : g.HC[5] * g.dx0 * g.dy0 * T(1);
-        o.ap0 = ap ? ap[r] : T(0);
-        const T pp = xo[r];
-        const T bc = bo[r];
-        const T dd = diag(o);
-        T xn;
-        if (g.flux) {
-            xn = pp + (rb_flux(o, n, pp) - bc) / dd;
-        } else {
-            const T dp = (((rb_offdiag(o, n) - bc) / dd - pp)) * T(1);
-            xn = pp + T(1) * dp;
-        }
-        xo[r] = xn;
+    const int s = (h + t.j + t.k) & 1;
+    const int i = 2 * t.m + s;
+    const bool valid = t.live && i >= 1 && i <= g.NX;
+    const unsigned r = (unsigned)t.m + HX * (unsigned)t.j + HXY * (unsigned)t.k;
+    const unsigned rb = r * (unsigned)sizeof(T);
+    const Nb<T> n = rb_nb(xt, r, s, HX, HXY);
+    const T pp = ldb(xo, rb);
+    const T bc = ldb(bo, rb);
+    const Cond<T> o = rb_cond(g, stab, clo, clt, ap, i, t.j, t.k, r, r, s, HX, HXY);
+    const T dd = diag(o);
+    T xn;
+    if (g.flux) {
+        xn = pp + (rb_flux(o, n, pp) - bc) / dd;
+    } else {
+        const T dp = (((rb_offdiag(o, n) - bc) / dd - pp)) * T(1);
+        xn = pp + T(1) * dp;
     }
+    if (valid) stb(xo, rb, xn);
 }
 
 template <typename T>
@@ -321,16 +323,19 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
 
 // Colour 0 of the first sweep from x = 0, fused with forming its right-hand
 // side bb (BiCG1! / Triad! / given) and with the fill: the colour-0 cell of
-// the pair gets the Gauss-Seidel value from all-zero neighbours, the
-// colour-1 cell 0.  Same arithmetic as k_rbsor_first.
+// the pair (half 1) gets the Gauss-Seidel value from all-zero neighbours,
+// the colour-1 cell 0.  Same arithmetic as k_rbsor_first.
 template <typename T>
-__global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x, T *__restrict__ bb,
-                                                  const T *__restrict__ r, const T *__restrict__ q, RbGeo<T> g,
-                                                  const KState *st)
+__global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x0, T *__restrict__ x1,
+                                                  T *__restrict__ b0, T *__restrict__ b1, const T *__restrict__ r0,
+                                                  const T *__restrict__ r1, const T *__restrict__ q0,
+                                                  const T *__restrict__ q1, const uint8_t *__restrict__ c0,
+                                                  const uint8_t *__restrict__ c1, const T *__restrict__ ap1,
+                                                  RbGeo<T> g, const KState *st)
 {
     if (halted(st)) return;
-    const T *stab = rb_stage(g);
-    const RbCoord cq = rb_coord<false>(g);
+    const RbThread t = rb_thread(g);
+    const T *stab = rb_stage(g, t.k);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
     T beta = T(0), omg = T(0), r_alpha = T(0);
     bool first_itr = false;
@@ -341,48 +346,54 @@ __global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x, T
     } else if (mode == FIRST_BICG_S) {
         r_alpha = (T)(-st->alpha);
     }
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair p = rb_pair(g, cq.m, cq.j, k);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, t, c, h);
+    // right-hand side of both cells: half 0 and half 1 entries at the same offset
+    const unsigned o = c[0].co;
+    const unsigned ob = o * (unsigned)sizeof(T);
+    T bv[2];
+    const T *rr[2] = {r0, r1}, *qq[2] = {q0, q1};
+    T *bb[2] = {b0, b1};
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!p.valid[a]) continue;
-            const RbCell &cell = p.c[a];
-            const unsigned c = cell.co;
-            T bv;
-            if (mode == FIRST_BICG_P) {
-                bv = first_itr ? r[c] : r[c] + beta * (bb[c] - omg * q[c]);
-                bb[c] = bv;
-            } else if (mode == FIRST_BICG_S) {
-                bv = r_alpha * q[c] + r[c];
-                bb[c] = bv;
-            } else {
-                bv = bb[c];
-            }
-            const bool red = c >= (unsigned)g.HS;  // half 1: updated by colour 0
-            T xn = T(0);
-            if (red) {
-                const Cond<T> o = rb_cond(g, stab, p.i[a], cq.j, k, cell, HX, HXY);
-                const T pp = T(0);
-                const T dd = diag(o);
-                if (g.flux) {
-                    xn = pp + (T(0) - bv) / dd;
-                } else {
-                    const T ss =
-                        (o.xp * T(0) + o.xm * T(0) + o.yp * T(0) + o.ym * T(0) + o.zp * T(0) + o.zm * T(0));
-                    const T dp = (((ss - bv) / dd - pp)) * T(1);
-                    xn = pp + T(1) * dp;
-                }
-            }
-            x[c] = xn;
-        }
+    for (int hh = 0; hh < 2; ++hh) {
+        if (mode == FIRST_BICG_P)
+            bv[hh] = first_itr ? ldb(rr[hh], ob) : ldb(rr[hh], ob) + beta * (ldb(bb[hh], ob) - omg * ldb(qq[hh], ob));
+        else if (mode == FIRST_BICG_S) bv[hh] = r_alpha * ldb(qq[hh], ob) + ldb(rr[hh], ob);
+        else bv[hh] = ldb(bb[hh], ob);
     }
+    // the half-1 cell (colour 0) of the pair
+    const int a1 = h[0] == 1 ? 0 : 1;
+    const RbCell &cr = c[a1];
+    const Cond<T> cd = rb_cond(g, stab, c1, c0, ap1, cr.i, t.j, t.k, o, o, cr.s, HX, HXY);
+    const T pp = T(0);
+    const T dd = diag(cd);
+    T xn;
+    if (g.flux) {
+        xn = pp + (T(0) - bv[1]) / dd;
+    } else {
+        const T ss = (cd.xp * T(0) + cd.xm * T(0) + cd.yp * T(0) + cd.ym * T(0) + cd.zp * T(0) + cd.zm * T(0));
+        const T dp = (((ss - bv[1]) / dd - pp)) * T(1);
+        xn = pp + T(1) * dp;
+    }
+    const bool v1 = c[a1].valid, v0 = c[1 - a1].valid;
+    if (mode != FIRST_COPY) {
+        if (v0) stb(b0, ob, bv[0]);
+        if (v1) stb(b1, ob, bv[1]);
+    }
+    if (v0) stb(x0, ob, T(0));
+    if (v1) stb(x1, ob, xn);
 }
 
 template <typename T>
 hipError_t launch_rb_first(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const KState *st,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rb_first<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, mode, x, bb, r, q, g, st);
+    const size_t HS = g.HS;
+    // q is not read in FIRST_COPY mode (nullptr); r neither
+    const T *r1 = r ? r + HS : nullptr, *q1 = q ? q + HS : nullptr;
+    hipLaunchKernelGGL(k_rb_first<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, mode, x, x + HS, bb, bb + HS, r, r1, q,
+                       q1, g.cls, g.cls + HS, g.ap0 ? g.ap0 + HS : nullptr, g, st);
     return hipGetLastError();
 }
 
@@ -392,18 +403,17 @@ __global__ void __launch_bounds__(256) k_rb_form_rhs(int mode, T *__restrict__ b
                                                      const T *__restrict__ q, RbGeo<T> g, const KState *st)
 {
     if (halted(st)) return;
-    const RbCoord cq = rb_coord<false>(g);
+    const RbThread t = rb_thread(g);
     const bool first_itr = (st->itr == 1);
     const T beta = (T)st->beta, omg = (T)st->omega, r_alpha = (T)(-st->alpha);
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair p = rb_pair(g, cq.m, cq.j, k);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, t, c, h);
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!p.valid[a]) continue;
-            const unsigned c = p.c[a].co;
-            if (mode == FIRST_BICG_P) bb[c] = first_itr ? r[c] : r[c] + beta * (bb[c] - omg * q[c]);
-            else bb[c] = r_alpha * q[c] + r[c];
-        }
+    for (int a = 0; a < 2; ++a) {
+        const size_t o = (h[a] ? g.HS : 0) + c[a].co;
+        const T v = mode == FIRST_BICG_P ? (first_itr ? r[o] : r[o] + beta * (bb[o] - omg * q[o])) : r_alpha * q[o] + r[o];
+        if (c[a].valid) bb[o] = v;
     }
 }
 
@@ -418,32 +428,47 @@ hipError_t launch_rb_form_rhs(int mode, T *bb, const T *r, const T *q, const RbG
 // ---------------------------------------------------------------------------
 // operator, residual, BLAS-1 with fused sums
 // ---------------------------------------------------------------------------
+// CalcAX! (src/NonUniform.jl:454-511): ax = A x; with NS >= 1 also
+// s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz.  Pair mapping; offsets are into
+// the whole arrays (both halves).
 template <typename T, int NS>
-__global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T *__restrict__ x, RbGeo<T> g,
-                                                    const T *__restrict__ dy, const T *__restrict__ dz, KState *st,
-                                                    RedBuf rb)
+__global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T *__restrict__ x,
+                                                    const T *__restrict__ dy, const T *__restrict__ dz, RbGeo<T> g,
+                                                    KState *st, RedBuf rb)
 {
     if (halted(st)) return;
-    const T *stab = rb_stage(g);
-    const RbCoord cq = NS > 0 ? rb_coord<true>(g) : rb_coord<false>(g);
-    const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
-    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll 2
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair p = rb_pair(g, cq.m, cq.j, k);
+    const RbThread t = rb_thread(g);
+    const T *stab = rb_stage(g, t.k);
+    const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY, HS = (unsigned)g.HS;
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, t, c, h);
+    const unsigned sz = (unsigned)sizeof(T);
+    T av[2], dyv[2], dzv[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!p.valid[a]) continue;
-            const RbCell &cell = p.c[a];
-            const Nb<T> n = rb_nb(x, cell, HX, HXY);
-            const Cond<T> o = rb_cond(g, stab, p.i[a], cq.j, k, cell, HX, HXY);
-            const T av = rb_apply(g, o, n, x[cell.co]);
-            ax[cell.co] = av;
-            if (NS >= 1) v[0] = dd_add(v[0], (double)av * (double)(dy ? dy[cell.co] : av));
-            if (NS >= 2) v[1] = dd_add(v[1], (double)av * (double)dz[cell.co]);
+    for (int a = 0; a < 2; ++a) {
+        const unsigned co = (h[a] ? HS : 0u) + c[a].co, no = (h[a] ? 0u : HS) + c[a].co;
+        const Nb<T> n = rb_nb(x, no, c[a].s, HX, HXY);
+        const T xv = ldb(x, co * sz);
+        const Cond<T> cd = rb_cond(g, stab, g.cls, g.cls, g.ap0, c[a].i, t.j, t.k, co, no, c[a].s, HX, HXY);
+        av[a] = rb_apply(g, cd, n, xv);
+        dyv[a] = (NS >= 1 && dy) ? ldb(dy, co * sz) : av[a];
+        dzv[a] = NS >= 2 ? ldb(dz, co * sz) : T(0);
+    }
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (c[a].valid) {
+            if (NS >= 1) v[0] = dd_add(v[0], (double)av[a] * (double)dyv[a]);
+            if (NS >= 2) v[1] = dd_add(v[1], (double)av[a] * (double)dzv[a]);
         }
     }
+    // the workgroup sum first, the stores last: a store before the
+    // reduction's barrier would hold the wave until it completes
     if (NS > 0) block_partials<NS>(v, rb);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+        if (c[a].valid) stb(ax, ((h[a] ? HS : 0u) + c[a].co) * sz, av[a]);
 }
 
 template <typename T>
@@ -451,15 +476,14 @@ hipError_t launch_rb_calc_ax(T *ax, const T *x, const RbGeo<T> &g, const T *dot_
                              KState *st, double *hist, const RedBuf &rb, hipStream_t s)
 {
     const int ns = fin < 0 ? 0 : (dot_z ? 2 : 1);
-    const dim3 grid(g.nb), block(BX, BY);
-    if (ns == 0)
-        hipLaunchKernelGGL((k_rb_calc_ax<T, 0>), dim3(g.nb1), block, rb_lds(g), s, ax, x, g, dot_y, dot_z, st, rb);
+    const dim3 grid(g.nb1), block(BX, BY);
+    if (ns == 0) hipLaunchKernelGGL((k_rb_calc_ax<T, 0>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     else if (ns == 1)
-        hipLaunchKernelGGL((k_rb_calc_ax<T, 1>), grid, block, rb_lds(g), s, ax, x, g, dot_y, dot_z, st, rb);
-    else hipLaunchKernelGGL((k_rb_calc_ax<T, 2>), grid, block, rb_lds(g), s, ax, x, g, dot_y, dot_z, st, rb);
+        hipLaunchKernelGGL((k_rb_calc_ax<T, 1>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+    else hipLaunchKernelGGL((k_rb_calc_ax<T, 2>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ns == 0) return e;
-    return launch_finalize_nb(ns, g.nb, fin, st, hist, rb, s);
+    return launch_finalize_nb(ns, g.nb1, fin, st, hist, rb, s);
 }
 
 // CalcRK! (src/NonUniform.jl:372-434) with θ, b in standard layout; r (and
@@ -470,45 +494,58 @@ __global__ void __launch_bounds__(256) k_rb_calc_rk(T *__restrict__ r, T *__rest
                                                     KState *st, RedBuf rb)
 {
     if (halted(st)) return;
-    const T *stab = rb_stage(g);
-    const RbCoord cq = rb_coord<true>(g);
+    const RbThread t = rb_thread(g);
+    const T *stab = rb_stage(g, t.k);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
     const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
-    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair p = rb_pair(g, cq.m, cq.j, k);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, t, c, h);
+    const unsigned o = c[0].co;
+    const uint8_t *cl = g.cls;
+    const size_t HS = g.HS;
+    T rs[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!p.valid[a]) continue;
-            const RbCell &cell = p.c[a];
-            const size_t c = (size_t)p.i[a] + sy * (size_t)cq.j + sz * (size_t)k;
-            Nb<T> n;
-            n.xm = th[c - 1];
-            n.xp = th[c + 1];
-            n.ym = th[c - sy];
-            n.yp = th[c + sy];
-            n.zm = th[c - sz];
-            n.zp = th[c + sz];
-            const Cond<T> o = rb_cond(g, stab, p.i[a], cq.j, k, cell, HX, HXY);
-            const T x0 = th[c];
-            T rs;
-            if (g.flux) rs = b[c] - rb_flux(o, n, x0);
-            else rs = (b[c] - (rb_offdiag(o, n) - diag(o) * x0)) * T(1);
-            r[cell.co] = rs;
-            if (r0copy) r0copy[cell.co] = rs;
-            v[0] = dd_add(v[0], (double)rs * (double)rs);
+    for (int a = 0; a < 2; ++a) {
+        // standard-layout offset of the cell (i is at most MX-1 for a loadable m)
+        const size_t cs = (size_t)min(c[a].i, g.MX - 2 + a) + sy * (size_t)t.j + sz * (size_t)t.k;
+        Nb<T> n;
+        n.xm = th[cs - 1];
+        n.xp = th[cs + 1];
+        n.ym = th[cs - sy];
+        n.yp = th[cs + sy];
+        n.zm = th[cs - sz];
+        n.zp = th[cs + sz];
+        const T x0 = th[cs];
+        const T bv = b[cs];
+        const int hh = h[a];
+        const Cond<T> cd = rb_cond(g, stab, cl + (hh ? HS : 0), cl + (hh ? 0 : HS), g.ap0 ? g.ap0 + (hh ? HS : 0) : nullptr,
+                                   c[a].i, t.j, t.k, o, o, c[a].s, HX, HXY);
+        if (g.flux) rs[a] = bv - rb_flux(cd, n, x0);
+        else rs[a] = (bv - (rb_offdiag(cd, n) - diag(cd) * x0)) * T(1);
+    }
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+        if (c[a].valid) v[0] = dd_add(v[0], (double)rs[a] * (double)rs[a]);
+    block_partials<1>(v, rb);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (c[a].valid) {
+            const size_t ro = (h[a] ? HS : 0) + o;
+            r[ro] = rs[a];
+            if (r0copy) r0copy[ro] = rs[a];
         }
     }
-    block_partials<1>(v, rb);
 }
 
 template <typename T>
 hipError_t launch_rb_calc_rk(T *r, T *r0copy, const T *th, const T *b, const RbGeo<T> &g, int fin, KState *st,
                              double *hist, const RedBuf &rb, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rb_calc_rk<T>, dim3(g.nb), dim3(BX, BY), rb_lds(g), s, r, r0copy, th, b, g, st, rb);
+    hipLaunchKernelGGL(k_rb_calc_rk<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, r, r0copy, th, b, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb, fin, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, fin, st, hist, rb, s);
 }
 
 // Fdot1/Fdot2 (src/NonUniform.jl:567-592): s0 = Σ x·(y ? y : x), s1 = Σ x·z
@@ -517,18 +554,24 @@ __global__ void __launch_bounds__(256) k_rb_dot(const T *__restrict__ x, const T
                                                 const T *__restrict__ z, RbGeo<T> g, KState *st, RedBuf rb)
 {
     if (halted(st)) return;
-    const RbCoord cq = rb_coord<true>(g);
-    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll 2
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair p = rb_pair(g, cq.m, cq.j, k);
+    const RbThread t = rb_thread(g);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, t, c, h);
+    double xv[2], yv[2], zv[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!p.valid[a]) continue;
-            const unsigned c = p.c[a].co;
-            const double xv = (double)x[c];
-            v[0] = dd_add(v[0], xv * (double)(y ? y[c] : x[c]));
-            if (NS >= 2) v[1] = dd_add(v[1], xv * (double)z[c]);
+    for (int a = 0; a < 2; ++a) {
+        const size_t o = (h[a] ? g.HS : 0) + c[a].co;
+        xv[a] = (double)x[o];
+        yv[a] = (double)(y ? y[o] : x[o]);
+        zv[a] = NS >= 2 ? (double)z[o] : 0.0;
+    }
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (c[a].valid) {
+            v[0] = dd_add(v[0], xv[a] * yv[a]);
+            if (NS >= 2) v[1] = dd_add(v[1], xv[a] * zv[a]);
         }
     }
     block_partials<NS>(v, rb);
@@ -538,11 +581,11 @@ template <typename T>
 hipError_t launch_rb_dot(const T *x, const T *y, const T *z, const RbGeo<T> &g, int fin, KState *st, double *hist,
                          const RedBuf &rb, hipStream_t s)
 {
-    const dim3 grid(g.nb), block(BX, BY);
+    const dim3 grid(g.nb1), block(BX, BY);
     if (z) hipLaunchKernelGGL((k_rb_dot<T, 2>), grid, block, 0, s, x, y, z, g, st, rb);
     else hipLaunchKernelGGL((k_rb_dot<T, 1>), grid, block, 0, s, x, y, z, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(z ? 2 : 1, g.nb, fin, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize_nb(z ? 2 : 1, g.nb1, fin, st, hist, rb, s);
 }
 
 // BICG2!(θ, p_, s_, alpha, omega); Triad!(r, t_, s, -omega); Σ r², and with
@@ -556,30 +599,46 @@ __global__ void __launch_bounds__(256) k_rb_bicg_update(double *__restrict__ th,
                                                         RbGeo<T> g, KState *st, RedBuf rb)
 {
     if (halted(st)) return;
-    const RbCoord cq = rb_coord<true>(g);
+    const RbThread tt = rb_thread(g);
     const double alpha = st->alpha, omega = st->omega;
     const T r_omega = (T)(-st->omega);
     const bool snap = xbest != nullptr && st->improved != 0;
     const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
-    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll 2
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair p = rb_pair(g, cq.m, cq.j, k);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, tt, c, h);
+    // the pair's two standard-layout cells are adjacent: i = 2m, 2m+1
+    const size_t cs0 = (size_t)(2 * tt.m) + sy * (size_t)tt.j + sz * (size_t)tt.k;
+    double xo[2], xn[2];
+    T rv[2], r0v[2];
+    const unsigned HS = (unsigned)g.HS, esz = (unsigned)sizeof(T);
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!p.valid[a]) continue;
-            const unsigned c = p.c[a].co;
-            const size_t cs = (size_t)p.i[a] + sy * (size_t)cq.j + sz * (size_t)k;
-            const double xo = th[cs];
-            if (snap) xbest[cs] = xo;
-            th[cs] = xo + (alpha * (double)p_[c] + omega * (double)s_[c]);
-            const T rv = r_omega * t[c] + sv[c];
-            r[c] = rv;
-            v[0] = dd_add(v[0], (double)rv * (double)rv);
-            if (NS >= 2) v[1] = dd_add(v[1], (double)rv * (double)r0[c]);
+    for (int a = 0; a < 2; ++a) {
+        const unsigned ob = ((h[a] ? HS : 0u) + c[a].co) * esz;
+        const size_t cs = min(cs0 + a, cs0 + (size_t)(g.MX - 1 - 2 * tt.m));
+        xo[a] = th[cs];
+        xn[a] = xo[a] + (alpha * (double)ldb(p_, ob) + omega * (double)ldb(s_, ob));
+        rv[a] = r_omega * ldb(t, ob) + ldb(sv, ob);
+        r0v[a] = NS >= 2 ? ldb(r0, ob) : T(0);
+    }
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (c[a].valid) {
+            v[0] = dd_add(v[0], (double)rv[a] * (double)rv[a]);
+            if (NS >= 2) v[1] = dd_add(v[1], (double)rv[a] * (double)r0v[a]);
         }
     }
     block_partials<NS>(v, rb);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (c[a].valid) {
+            const size_t cs = cs0 + a;
+            if (snap) xbest[cs] = xo[a];
+            th[cs] = xn[a];
+            stb(r, ((h[a] ? HS : 0u) + c[a].co) * esz, rv[a]);
+        }
+    }
 }
 
 template <typename T>
@@ -587,15 +646,15 @@ hipError_t launch_rb_bicg_update(double *th, double *xbest, T *r, const T *p_, c
                                  const T *sv, const T *r0, const RbGeo<T> &g, KState *st, double *hist,
                                  const RedBuf &rb, hipStream_t s)
 {
-    const dim3 grid(g.nb), block(BX, BY);
+    const dim3 grid(g.nb1), block(BX, BY);
     if (r0 == nullptr) {
         hipLaunchKernelGGL((k_rb_bicg_update<T, 1>), grid, block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st, rb);
         hipError_t e = hipGetLastError();
-        return e != hipSuccess ? e : launch_finalize_nb(1, g.nb, FIN_BICG_RES, st, hist, rb, s);
+        return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, FIN_BICG_RES, st, hist, rb, s);
     }
     hipLaunchKernelGGL((k_rb_bicg_update<T, 2>), grid, block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(2, g.nb, FIN_BICG_RES_RHO, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize_nb(2, g.nb1, FIN_BICG_RES_RHO, st, hist, rb, s);
 }
 
 // CG!: θ += α p, r -= α q, Σ r² (NonUniform.jl:179-186)
@@ -605,37 +664,49 @@ __global__ void __launch_bounds__(256) k_rb_cg_update(double *__restrict__ th, d
                                                       const T *__restrict__ q, RbGeo<T> g, KState *st, RedBuf rb)
 {
     if (halted(st)) return;
-    const RbCoord cq = rb_coord<true>(g);
+    const RbThread tt = rb_thread(g);
     const double alpha_d = st->alpha;
     const T alpha = (T)st->alpha;
     const bool snap = xbest != nullptr && st->improved != 0;
     const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
-    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair pr = rb_pair(g, cq.m, cq.j, k);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, tt, c, h);
+    const size_t cs0 = (size_t)(2 * tt.m) + sy * (size_t)tt.j + sz * (size_t)tt.k;
+    double xo[2], xn[2];
+    T rv[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!pr.valid[a]) continue;
-            const unsigned c = pr.c[a].co;
-            const size_t cs = (size_t)pr.i[a] + sy * (size_t)cq.j + sz * (size_t)k;
-            const double xo = th[cs];
-            if (snap) xbest[cs] = xo;
-            th[cs] = xo + alpha_d * (double)p[c];
-            const T rv = r[c] - alpha * q[c];
-            r[c] = rv;
-            v[0] = dd_add(v[0], (double)rv * (double)rv);
+    for (int a = 0; a < 2; ++a) {
+        const size_t o = (h[a] ? g.HS : 0) + c[a].co;
+        const size_t cs = min(cs0 + a, cs0 + (size_t)(g.MX - 1 - 2 * tt.m));
+        xo[a] = th[cs];
+        xn[a] = xo[a] + alpha_d * (double)p[o];
+        rv[a] = r[o] - alpha * q[o];
+    }
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+        if (c[a].valid) v[0] = dd_add(v[0], (double)rv[a] * (double)rv[a]);
+    block_partials<1>(v, rb);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (c[a].valid) {
+            const size_t o = (h[a] ? g.HS : 0) + c[a].co;
+            const size_t cs = cs0 + a;
+            if (snap) xbest[cs] = xo[a];
+            th[cs] = xn[a];
+            r[o] = rv[a];
         }
     }
-    block_partials<1>(v, rb);
 }
 
 template <typename T>
 hipError_t launch_rb_cg_update(double *th, double *xbest, T *r, const T *p, const T *q, const RbGeo<T> &g,
                                KState *st, double *hist, const RedBuf &rb, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rb_cg_update<T>, dim3(g.nb), dim3(BX, BY), 0, s, th, xbest, r, p, q, g, st, rb);
+    hipLaunchKernelGGL(k_rb_cg_update<T>, dim3(g.nb1), dim3(BX, BY), 0, s, th, xbest, r, p, q, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb, FIN_CG_RES, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, FIN_CG_RES, st, hist, rb, s);
 }
 
 // CG!: p = z + β p (NonUniform.jl:218-220)
@@ -644,16 +715,16 @@ __global__ void __launch_bounds__(256) k_rb_cg_p(T *__restrict__ p, const T *__r
                                                  const KState *st)
 {
     if (halted(st)) return;
-    const RbCoord cq = rb_coord<false>(g);
+    const RbThread t = rb_thread(g);
     const T beta = (T)st->beta;
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair pr = rb_pair(g, cq.m, cq.j, k);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, t, c, h);
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (!pr.valid[a]) continue;
-            const unsigned c = pr.c[a].co;
-            p[c] = z[c] + beta * p[c];
-        }
+    for (int a = 0; a < 2; ++a) {
+        const size_t o = (h[a] ? g.HS : 0) + c[a].co;
+        const T v = z[o] + beta * p[o];
+        if (c[a].valid) p[o] = v;
     }
 }
 
@@ -669,12 +740,15 @@ __global__ void __launch_bounds__(256) k_rb_copy_interior(T *__restrict__ dst, c
                                                           const KState *st)
 {
     if (halted(st)) return;
-    const RbCoord cq = rb_coord<false>(g);
-    for (int k = cq.kb; k < cq.ke; ++k) {
-        const RbPair pr = rb_pair(g, cq.m, cq.j, k);
+    const RbThread t = rb_thread(g);
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, t, c, h);
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-            if (pr.valid[a]) dst[pr.c[a].co] = src[pr.c[a].co];
+    for (int a = 0; a < 2; ++a) {
+        const size_t o = (h[a] ? g.HS : 0) + c[a].co;
+        const T v = src[o];
+        if (c[a].valid) dst[o] = v;
     }
 }
 
