@@ -1,16 +1,20 @@
 """Benchmark of the Heat3ds linear-solve path on one MI355X.
 
 A step is one pass of the hot path: calRHS! + PBiCGSTAB! with the 5-sweep
-red-black Gauss-Seidel preconditioner to ε = 1e-4 (208 iterations), FP64,
-on the modelA stack at 240 x 240 x 30 interior cells (the reference's
-README/benchmark case), with all fields resident in HBM.  θ is reset to the
-initial state before every step (untimed).
+red-black Gauss-Seidel preconditioner to ε = 1e-4, FP64, on the modelA
+stack at 240 x 240 x 30 interior cells (the reference's README/benchmark
+case: 208 iterations with the reference's sequential sums, 223 with the
+exactly rounded sums of this implementation), with all fields resident in
+HBM.  θ is reset to the initial state before every step (untimed).
 
 Printed JSON line (one, on stdout):
   value      algorithmic HBM GB/s of the whole solve:
              560 B/cell/iteration (FP64) x cells x iterations / step time
-  roofline   dominant kernel (one red-black GS colour sweep): algorithmic
-             bytes per launch / mean launch time (HIP events, context stream)
+  roofline   dominant kernel (one red-black GS colour, k_rb_sweep):
+             algorithmic bytes per launch / mean launch time (HIP events
+             around 200 launches on the context stream); traffic = HBM
+             bytes per launch from the committed rocprofv3 FETCH_SIZE and
+             WRITE_SIZE figures (profiles/traffic.json)
   cpu_baseline  the oracle (OpenMP C restatement of the reference
              algorithm) on a bounded sample of the same workload
   fp32_refined  the same case solved to a true relative residual of 1e-8
@@ -41,7 +45,7 @@ def parse():
     ap.add_argument("--eps", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-refined", action="store_true", help="skip the FP32-refinement comparison")
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-iters", type=int, default=200)
     return ap.parse_args()
 
 
@@ -122,7 +126,7 @@ def main():
     itr = itrs[-1]
     value = BYTES_PER_CELL_ITER["fp64"] * cells * itr / (ms_per_step * 1e-3) / 1e9
 
-    # dominant kernel: one red-black colour sweep (k_rbsor), HIP events on the context stream
+    # dominant kernel: one red-black colour sweep (k_rb_sweep), HIP events on the context stream
     sweep_ms = wk.time_kernel(0, 200, dt, True)  # both colours
     half_ms = sweep_ms / 2.0
     per_launch_bytes = SWEEP_BYTES_PER_CELL / 2.0 * cells
@@ -131,8 +135,9 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("k_rbsor_240x240x30")
-    roofline = {"bound": "hbm", "kernel": "k_rbsor<double> (one red-black colour)", "achieved": round(achieved, 1),
+            traffic = json.load(f).get("k_rb_sweep_%dx%dx%d" % (nx, nx, nz), {}).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": "k_rb_sweep<double> (one red-black colour, colour-split layout)",
+                "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "launch_ms": round(half_ms, 5),
                 "algorithmic_bytes_per_launch": per_launch_bytes}
