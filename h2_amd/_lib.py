@@ -20,6 +20,7 @@ SMOOTHER_NONE, SMOOTHER_GS = 0, 1
 FP64, FP32_REFINED = 0, 1
 OPT_COEF_FORM, OPT_NUM_CLASSES, OPT_LAYOUT = 0, 1, 2
 LAYOUT_AUTO, LAYOUT_STANDARD = 0, 1
+OPT_OPERATOR_FORM, FORM_REFERENCE, FORM_FLUX = 3, 0, 1
 COEF_AUTO, COEF_DIRECT = 0, 1
 
 FIELDS = ["theta", "b", "mask", "rho", "lam", "cp", "pcg_p", "pcg_p_", "pcg_r", "pcg_r0",
@@ -33,6 +34,7 @@ EXPORTS = [
     "h3d_pbicgstab", "h3d_cg", "h3d_get_history", "h3d_last_solve_ms", "h3d_calc_ax", "h3d_calc_rk",
     "h3d_rbsor_color", "h3d_precond", "h3d_fdot1", "h3d_fdot2", "h3d_bicg1", "h3d_triad", "h3d_bicg2",
     "h3d_time_kernel", "h3d_set_option", "h3d_get_option",
+    "h3d_objective", "h3d_adjoint_solve", "h3d_gradients", "h3d_get_adjoint",
     "h3d_model_pitch", "h3d_model_genz", "h3d_model_fill_id", "h3d_model_properties",
     "h3d_model_heat_src", "h3d_model_mode3_bc", "h3d_model_power_map",
 ]
@@ -71,6 +73,10 @@ _SIGS = {
     "h3d_triad": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_double]),
     "h3d_bicg2": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_double, _c_double]),
     "h3d_time_kernel": (_c_int, [_vp, _c_int, _c_int, _c_double, _c_int, _dp]),
+    "h3d_objective": (_c_int, [_vp, _vp, _c_size, _c_double, _dp]),
+    "h3d_adjoint_solve": (_c_int, [_vp, _c_int, _c_double, _c_double, _c_int, _c_int, _c_int, _ip, _ip, _dp]),
+    "h3d_gradients": (_c_int, [_vp, _vp, _vp, _c_size]),
+    "h3d_get_adjoint": (_c_int, [_vp, _vp, _vp, _c_size]),
     "h3d_set_option": (_c_int, [_vp, _c_int, _c_int]),
     "h3d_get_option": (_c_int, [_vp, _c_int, _ip]),
     "h3d_model_pitch": (_c_int, [_c_int, _dp]),

@@ -62,6 +62,7 @@ struct h3d_ctx {
     // colour-split (RB) layout of the Krylov vectors: the solver's layout
     // whenever the class table form is active (H3D_OPT_LAYOUT)
     int layout = H3D_LAYOUT_AUTO;
+    int op_form = H3D_FORM_REFERENCE;     // H3D_OPT_OPERATOR_FORM of the FP64 operator
     unsigned long grid_version = 1;       // bumped by h3d_set_grid / h3d_set_bc
     unsigned long rb_lam = 0, rb_grid = 0; // versions the RB tables were built for
     std::vector<double> hZC, hdZ;         // host copies of the Z metrics
@@ -89,6 +90,11 @@ struct h3d_ctx {
         }
     };
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+    // sensitivity extension: region, seed ∂J/∂θ, adjoint μ, saved θ and b
+    uint8_t *d_region = nullptr;
+    double *d_seed = nullptr, *d_mu = nullptr, *d_save[2] = {nullptr, nullptr};
+    double *d_bmax = nullptr, *d_scalar = nullptr;
+    bool seed_valid = false, mu_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evb[2] = {nullptr, nullptr};
     double last_ms = 0.0;
     int verbose = 0;             // H3D_VERBOSE=1: refinement diagnostics on stderr
@@ -214,7 +220,7 @@ int make_geo64(h3d_ctx *c, double dt, int is_steady, Geo<double> *out)
         g.tab = c->d_tab64;
         g.K = c->K;
     }
-    g.flux = 0;
+    g.flux = c->op_form == H3D_FORM_FLUX ? 1 : 0;
     g.ap0 = nullptr;
     if (!is_steady) {
         if (!(dt > 0.0) && !(dt < 0.0)) return fail(c, H3D_ERR_ARG, "transient operator needs dt != 0");
@@ -906,7 +912,8 @@ int h3d_destroy(h3d_ctx *c)
         if (c->rb64[a]) (void)hipFree(c->rb64[a]);
         if (c->rb32[a]) (void)hipFree(c->rb32[a]);
     }
-    void *rbs[] = {c->d_cls_rb, c->d_ZF64, c->d_ZF32, c->d_ap0_rb, c->d_ap0_rb32};
+    void *rbs[] = {c->d_cls_rb, c->d_ZF64,  c->d_ZF32,      c->d_ap0_rb,    c->d_ap0_rb32, c->d_region,
+                   c->d_seed,   c->d_mu,    c->d_save[0],   c->d_save[1],   c->d_bmax,     c->d_scalar};
     for (void *p : rbs)
         if (p) (void)hipFree(p);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_state,
@@ -940,6 +947,12 @@ int h3d_set_option(h3d_ctx *c, int option, int value)
         c->layout = value;
         return H3D_OK;
     }
+    if (option == H3D_OPT_OPERATOR_FORM) {
+        if (value != H3D_FORM_REFERENCE && value != H3D_FORM_FLUX) return fail(c, H3D_ERR_ARG, "bad form");
+        c->op_form = value;
+        c->grid_version++;  // captured iterations carry the form
+        return H3D_OK;
+    }
     return fail(c, H3D_ERR_ARG, "unknown option");
 }
 
@@ -956,6 +969,10 @@ int h3d_get_option(const h3d_ctx *c, int option, int *value)
     }
     if (option == H3D_OPT_LAYOUT) {
         *value = c->layout;
+        return H3D_OK;
+    }
+    if (option == H3D_OPT_OPERATOR_FORM) {
+        *value = c->op_form;
         return H3D_OK;
     }
     return H3D_ERR_ARG;
@@ -1270,6 +1287,104 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
     float t = 0.f;
     HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
     *ms = (double)t / reps;
+    return H3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sensitivity extension
+// ---------------------------------------------------------------------------
+int h3d_objective(h3d_ctx *c, const uint8_t *region, size_t n, double beta, double *J)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!region || !J || n != c->H) return fail(c, H3D_ERR_ARG, "region must have MX*MY*MZ entries");
+    if (!(beta > 0.0)) return fail(c, H3D_ERR_ARG, "beta must be > 0");
+    if (!c->grid_set) return fail(c, H3D_ERR_STATE, "h3d_set_grid has not been called");
+    hipStream_t s = c->stream;
+    if (!c->d_region) HIPCHK(c, hipMalloc(&c->d_region, c->H));
+    if (!c->d_seed) HIPCHK(c, hipMalloc(&c->d_seed, c->H * sizeof(double)));
+    if (!c->d_bmax) HIPCHK(c, hipMalloc(&c->d_bmax, (size_t)c->rb.nblocks_max * sizeof(double)));
+    if (!c->d_scalar) HIPCHK(c, hipMalloc(&c->d_scalar, sizeof(double)));
+    HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipMemcpy(c->d_region, region, c->H, hipMemcpyHostToDevice));
+    Geo<double> g = base_geo<double>(c);
+    g.ZC = c->d_ZC; g.dZ = c->d_dZ; g.lam = c->f[H3D_F_LAMBDA];
+    HIPCHK(c, launch_fill<double>(c->d_seed, 0.0, c->H, s));
+    HIPCHK(c, launch_objective_max(c->f[H3D_F_THETA], c->d_region, g, c->d_bmax, c->d_scalar, s));
+    HIPCHK(c, launch_objective_exp(c->f[H3D_F_THETA], c->d_region, g, beta, c->d_scalar, c->d_seed, c->d_unit, c->rb,
+                                   s));
+    double m = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipMemcpy(&m, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
+    const double S = c->h_state[0].out[0];
+    if (!(S > 0.0)) return fail(c, H3D_ERR_ARG, "the region has no interior cell");
+    HIPCHK(c, launch_scale_interior(c->d_seed, g, 1.0 / S, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    *J = m + std::log(S) / beta;
+    if (!std::isfinite(*J)) return fail(c, H3D_ERR_NAN, "non-finite objective");
+    c->seed_valid = true;
+    c->mu_valid = false;
+    return H3D_OK;
+}
+
+int h3d_adjoint_solve(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
+                      int *isconverged, int *itr, double *res0)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!c->seed_valid) return fail(c, H3D_ERR_STATE, "h3d_objective has not been called");
+    hipStream_t s = c->stream;
+    const size_t bytes = c->H * sizeof(double);
+    for (int a = 0; a < 2; ++a)
+        if (!c->d_save[a]) HIPCHK(c, hipMalloc(&c->d_save[a], bytes));
+    if (!c->d_mu) HIPCHK(c, hipMalloc(&c->d_mu, bytes));
+    double *th = c->f[H3D_F_THETA], *b = c->f[H3D_F_B];
+    HIPCHK(c, hipMemcpyAsync(c->d_save[0], th, bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d_save[1], b, bytes, hipMemcpyDeviceToDevice, s));
+    // μ starts at 0 with a zero halo; the right-hand side is the seed
+    HIPCHK(c, launch_fill<double>(th, 0.0, c->H, s));
+    HIPCHK(c, hipMemcpyAsync(b, c->d_seed, bytes, hipMemcpyDeviceToDevice, s));
+    const int rc = solve(c, solver, dt, tol, maxItr, smoother, is_steady, H3D_FP64, 0.0, isconverged, itr, res0);
+    HIPCHK(c, hipMemcpyAsync(c->d_mu, th, bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(th, c->d_save[0], bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(b, c->d_save[1], bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->mu_valid = (rc == H3D_OK);
+    return rc;
+}
+
+int h3d_gradients(h3d_ctx *c, double *dJ_dlambda, double *dJ_dhsrc, size_t n)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (n != c->H) return fail(c, H3D_ERR_ARG, "outputs must have MX*MY*MZ entries");
+    if (!c->mu_valid) return fail(c, H3D_ERR_STATE, "h3d_adjoint_solve has not completed");
+    hipStream_t s = c->stream;
+    Geo<double> g = base_geo<double>(c);
+    g.ZC = c->d_ZC; g.dZ = c->d_dZ; g.lam = c->f[H3D_F_LAMBDA];
+    int iso = 0;
+    for (int f = 0; f < 6; ++f)
+        if (c->bc_type[f] == H3D_ISOTHERMAL) iso |= 1 << f;
+    HIPCHK(c, launch_fill<double>(c->d_save[0], 0.0, c->H, s));
+    HIPCHK(c, launch_fill<double>(c->d_save[1], 0.0, c->H, s));
+    HIPCHK(c, launch_gradients(c->f[H3D_F_THETA], c->d_mu, g, iso, c->d_save[0], c->d_save[1], s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (dJ_dlambda) HIPCHK(c, hipMemcpy(dJ_dlambda, c->d_save[0], n * sizeof(double), hipMemcpyDeviceToHost));
+    if (dJ_dhsrc) HIPCHK(c, hipMemcpy(dJ_dhsrc, c->d_save[1], n * sizeof(double), hipMemcpyDeviceToHost));
+    return H3D_OK;
+}
+
+int h3d_get_adjoint(h3d_ctx *c, double *mu, double *seed, size_t n)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (n != c->H) return fail(c, H3D_ERR_ARG, "outputs must have MX*MY*MZ entries");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (mu) {
+        if (!c->mu_valid) return fail(c, H3D_ERR_STATE, "no adjoint");
+        HIPCHK(c, hipMemcpy(mu, c->d_mu, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (seed) {
+        if (!c->seed_valid) return fail(c, H3D_ERR_STATE, "no objective");
+        HIPCHK(c, hipMemcpy(seed, c->d_seed, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
     return H3D_OK;
 }
 

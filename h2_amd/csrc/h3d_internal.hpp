@@ -214,6 +214,19 @@ template <typename D, typename S> hipError_t launch_convert(D *dst, const S *src
 // θ += d on the interior: the refinement update
 hipError_t launch_add_correction(double *th, const double *d, const Geo<double> &g, hipStream_t s);
 
+// ---- objective and adjoint gradients (h3d_adjoint.hip) ---------------------
+// out[0] = max of θ over the region's interior cells (-DBL_MAX if none);
+// bmax holds one value per workgroup of the cell grid
+hipError_t launch_objective_max(const double *th, const uint8_t *region, const Geo<double> &g, double *bmax,
+                                double *out, hipStream_t s);
+// w = exp(β(θ - *mptr)) on the region (0 elsewhere); st->out[0] = Σ w
+hipError_t launch_objective_exp(const double *th, const uint8_t *region, const Geo<double> &g, double beta,
+                                const double *mptr, double *w, KState *st, const RedBuf &rb, hipStream_t s);
+hipError_t launch_scale_interior(double *w, const Geo<double> &g, double sc, hipStream_t s);
+// per-cell dJ/dλ and dJ/dhsrc from θ and the adjoint μ; iso: isothermal faces (bit f)
+hipError_t launch_gradients(const double *th, const double *mu, const Geo<double> &g, int iso, double *glam,
+                            double *ghs, hipStream_t s);
+
 // ---- RB-layout launchers (h3d_rb.hip) ---------------------------------------
 // dst (RB) = src (standard layout), every cell including the halo
 template <typename D> hipError_t launch_to_rb(D *dst, const D *src, int MX, int MY, int MZ, hipStream_t s);

@@ -250,6 +250,10 @@ class WorkBuffers:
         L.LAYOUT_STANDARD (the reference's layout); results are identical."""
         _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_LAYOUT, int(layout)))
 
+    def set_operator_form(self, form):
+        """L.FORM_REFERENCE (the reference's expression, default) or L.FORM_FLUX."""
+        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_OPERATOR_FORM, int(form)))
+
     def num_classes(self):
         v = ctypes.c_int()
         _check(self.ctx, self.lib.h3d_get_option(self.ctx, L.OPT_NUM_CLASSES, ctypes.byref(v)))
@@ -260,6 +264,39 @@ class WorkBuffers:
         _check(self.ctx, self.lib.h3d_time_kernel(self.ctx, int(which), int(reps), float(dt), int(bool(is_steady)),
                                                   ctypes.byref(ms)))
         return ms.value
+
+    # sensitivity extension (no reference counterpart) ----------------------
+    def objective(self, region, beta):
+        """Smooth maximum J of θ over the interior cells where region != 0;
+        keeps the seed ∂J/∂θ on the device."""
+        r = np.asfortranarray(np.asarray(region, dtype=np.uint8))
+        if r.shape != self.shape:
+            raise ValueError("expected shape %s, got %s" % (self.shape, r.shape))
+        J = ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_objective(self.ctx, r.ctypes.data_as(ctypes.c_void_p), self.n, float(beta),
+                                                ctypes.byref(J)))
+        return J.value
+
+    def adjoint_solve(self, solver, dt, tol, max_itr=ITR_MAX, smoother=L.SMOOTHER_GS, is_steady=True):
+        conv, itr, res0 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_adjoint_solve(self.ctx, int(solver), float(dt), float(tol), int(max_itr),
+                                                    int(smoother), int(bool(is_steady)), ctypes.byref(conv),
+                                                    ctypes.byref(itr), ctypes.byref(res0)))
+        return bool(conv.value), itr.value, res0.value
+
+    def gradients(self):
+        """(dJ/dλ, dJ/dhsrc) per cell for the current θ and adjoint."""
+        gl = np.zeros(self.shape, order="F")
+        gh = np.zeros(self.shape, order="F")
+        _check(self.ctx, self.lib.h3d_gradients(self.ctx, L.ptr(gl), L.ptr(gh), self.n))
+        return gl, gh
+
+    def adjoint_fields(self):
+        """(μ, seed ∂J/∂θ) as device-held."""
+        mu = np.zeros(self.shape, order="F")
+        seed = np.zeros(self.shape, order="F")
+        _check(self.ctx, self.lib.h3d_get_adjoint(self.ctx, L.ptr(mu), L.ptr(seed), self.n))
+        return mu, seed
 
 
 # ---------------------------------------------------------------------------

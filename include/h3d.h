@@ -85,6 +85,11 @@ extern "C" {
 #define H3D_OPT_LAYOUT 2       /* device layout of the solver's Krylov vectors */
 #define H3D_LAYOUT_AUTO 0      /* colour-split (each red-black colour contiguous) when the table form is active */
 #define H3D_LAYOUT_STANDARD 1  /* the reference's column-major layout */
+#define H3D_OPT_OPERATOR_FORM 3 /* how the FP64 operator evaluates A·x */
+#define H3D_FORM_REFERENCE 0   /* ss - dd·x0, the reference's expression (bitwise the reference) */
+#define H3D_FORM_FLUX 1        /* Σ c_n (x_n - x0) - (Σconv + ap0)·x0: the same operator with rounding
+                                  relative to temperature differences instead of temperatures (used for
+                                  finite-difference checks of the adjoint) */
 
 typedef struct h3d_ctx h3d_ctx;
 
@@ -166,6 +171,27 @@ int h3d_fdot2(h3d_ctx *ctx, int x, int y, double *out);
 int h3d_bicg1(h3d_ctx *ctx, int p, int r, int q, double beta, double omg);
 int h3d_triad(h3d_ctx *ctx, int z, int x, int y, double a);
 int h3d_bicg2(h3d_ctx *ctx, int z, int x, int y, double a, double b);
+
+/* Sensitivity extension (no counterpart in the reference): a smooth maximum
+ * of θ, its adjoint, and the gradients the adjoint gives.
+ *
+ * h3d_objective: J = m + log(Σ_R exp(β(θ_c - m)))/β over the interior cells
+ * whose region byte is non-zero (region: MX*MY*MZ bytes, reference layout),
+ * m = max_R θ; keeps the seed ∂J/∂θ (the softmax weights) on the device.
+ * H3D_ERR_ARG when the region has no interior cell. */
+int h3d_objective(h3d_ctx *ctx, const uint8_t *region, size_t n, double beta, double *J);
+/* Solves Aᵀ μ = ∂J/∂θ for the adjoint μ (A is the operator of the last
+ * forward solve's dt and is_steady; it is symmetric, so the same PBiCGSTAB!
+ * or CG! runs on it with a zero halo).  θ and b are left as they were.
+ * Outputs as h3d_pbicgstab. */
+int h3d_adjoint_solve(h3d_ctx *ctx, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
+                      int *isconverged, int *itr, double *res0);
+/* Per-cell gradients of J for the current θ and μ (MX*MY*MZ each, zero on
+ * the halo): dJ/dλ (including the isothermal halo copy of λ) and dJ/dhsrc.
+ * Either pointer may be NULL. */
+int h3d_gradients(h3d_ctx *ctx, double *dJ_dlambda, double *dJ_dhsrc, size_t n);
+/* The adjoint μ and the seed ∂J/∂θ (either may be NULL). */
+int h3d_get_adjoint(h3d_ctx *ctx, double *mu, double *seed, size_t n);
 
 /* Timing of one kernel family on the context stream (HIP events around
  * reps back-to-back launches after one warm-up launch; *ms = mean per
