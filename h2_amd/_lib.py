@@ -35,6 +35,8 @@ EXPORTS = [
     "h3d_rbsor_color", "h3d_precond", "h3d_fdot1", "h3d_fdot2", "h3d_bicg1", "h3d_triad", "h3d_bicg2",
     "h3d_time_kernel", "h3d_set_option", "h3d_get_option",
     "h3d_objective", "h3d_adjoint_solve", "h3d_gradients", "h3d_get_adjoint",
+    "h3d_adjoint_step", "h3d_gradients_accumulate", "h3d_gradients_accumulated",
+    "h3d_store_field", "h3d_load_field", "h3d_release_slots", "h3d_mem_info",
     "h3d_model_pitch", "h3d_model_genz", "h3d_model_fill_id", "h3d_model_properties",
     "h3d_model_heat_src", "h3d_model_mode3_bc", "h3d_model_power_map",
 ]
@@ -77,6 +79,14 @@ _SIGS = {
     "h3d_adjoint_solve": (_c_int, [_vp, _c_int, _c_double, _c_double, _c_int, _c_int, _c_int, _ip, _ip, _dp]),
     "h3d_gradients": (_c_int, [_vp, _vp, _vp, _c_size]),
     "h3d_get_adjoint": (_c_int, [_vp, _vp, _vp, _c_size]),
+    "h3d_adjoint_step": (_c_int, [_vp, _c_int, _c_double, _c_double, _c_int, _c_int, _c_int, _c_double, _c_int,
+                                  _ip, _ip, _dp]),
+    "h3d_gradients_accumulate": (_c_int, [_vp, _c_double, _c_double]),
+    "h3d_gradients_accumulated": (_c_int, [_vp, _vp, _vp, _c_size, _c_int]),
+    "h3d_store_field": (_c_int, [_vp, _c_int, _c_int]),
+    "h3d_load_field": (_c_int, [_vp, _c_int, _c_int, _c_double]),
+    "h3d_release_slots": (_c_int, [_vp]),
+    "h3d_mem_info": (_c_int, [_vp, ctypes.POINTER(_c_size), ctypes.POINTER(_c_size)]),
     "h3d_set_option": (_c_int, [_vp, _c_int, _c_int]),
     "h3d_get_option": (_c_int, [_vp, _c_int, _ip]),
     "h3d_model_pitch": (_c_int, [_c_int, _dp]),

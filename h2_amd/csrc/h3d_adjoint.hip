@@ -138,6 +138,51 @@ __global__ void __launch_bounds__(256) k_gradients(const double *th, const doubl
     ghs[c] = -m0 * g.dx0 * g.dy0 * dzk;
 }
 
+// Right-hand side of one adjoint step (interior; halo 0):
+//   rhs = w·seed + D∘μ_next,  D_c = ∂b_c/∂θ_c^old of calRHS! =
+//   -ap0_c + Σ_{convective faces f of c} (f is a plus face ? +1 : -1)·h_f·area_f
+// (k_bc2rhs_face adds b ∓= h·area·(θ∞ - θ) on minus faces, b ±= on plus
+// faces, before b = -(ap0·θ + hsrc·V + b); src/RHS.jl:151-190,
+// src/NonUniform.jl:85-90).  hconv[f] = h_f for convective faces, else 0.
+__global__ void __launch_bounds__(256) k_adjoint_rhs(double *rhs, const double *seed, const double *mu_next,
+                                                     Geo<double> g, double w, double hx0, double hx1, double hy0,
+                                                     double hy1, double hz0, double hz1)
+{
+    const Cell q = interior_cell(g);
+    if (!q.active) return;
+    const size_t c = q.c;
+    double v = w * seed[c];
+    if (mu_next) {
+        const double dzk = g.dZ[q.k];
+        double d = g.ap0 ? -g.ap0[c] : 0.0;
+        if (q.i == 1) d -= hx0 * g.dy0 * dzk;
+        if (q.i == g.NX) d += hx1 * g.dy0 * dzk;
+        if (q.j == 1) d -= hy0 * g.dx0 * dzk;
+        if (q.j == g.NY) d += hy1 * g.dx0 * dzk;
+        if (q.k == 1) d -= hz0 * g.dx0 * g.dy0;
+        if (q.k == g.NZ) d += hz1 * g.dx0 * g.dy0;
+        v += d * mu_next[c];
+    }
+    rhs[c] = v;
+}
+
+// acc_l += wl·gl, acc_h += wh·gh over the whole array
+__global__ void __launch_bounds__(256) k_accumulate(double *acc_l, double *acc_h, const double *gl, const double *gh,
+                                                    double wl, double wh, size_t n)
+{
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        acc_l[c] += wl * gl[c];
+        acc_h[c] += wh * gh[c];
+    }
+}
+
+// dst = f·src over the whole array
+__global__ void __launch_bounds__(256) k_scaled_copy(double *dst, const double *src, double f, size_t n)
+{
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x)
+        dst[c] = f * src[c];
+}
+
 static dim3 cell_grid(const Geo<double> &g)
 {
     return dim3((g.NX + BX - 1) / BX, (g.NY + BY - 1) / BY, g.NZ);
@@ -173,6 +218,33 @@ hipError_t launch_gradients(const double *th, const double *mu, const Geo<double
                             double *ghs, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gradients, cell_grid(g), dim3(BX, BY), 0, s, th, mu, g, iso, glam, ghs);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_rhs(double *rhs, const double *seed, const double *mu_next, const Geo<double> &g, double w,
+                              const double hconv[6], hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adjoint_rhs, cell_grid(g), dim3(BX, BY), 0, s, rhs, seed, mu_next, g, w, hconv[0], hconv[1],
+                       hconv[2], hconv[3], hconv[4], hconv[5]);
+    return hipGetLastError();
+}
+
+static int flat_blocks(size_t n)
+{
+    const size_t b = (n + 255) / 256;
+    return (int)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
+hipError_t launch_accumulate(double *acc_l, double *acc_h, const double *gl, const double *gh, double wl, double wh,
+                             size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_accumulate, dim3(flat_blocks(n)), dim3(256), 0, s, acc_l, acc_h, gl, gh, wl, wh, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scaled_copy(double *dst, const double *src, double f, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scaled_copy, dim3(flat_blocks(n)), dim3(256), 0, s, dst, src, f, n);
     return hipGetLastError();
 }
 

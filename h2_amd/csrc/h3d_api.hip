@@ -94,7 +94,10 @@ struct h3d_ctx {
     uint8_t *d_region = nullptr;
     double *d_seed = nullptr, *d_mu = nullptr, *d_save[2] = {nullptr, nullptr};
     double *d_bmax = nullptr, *d_scalar = nullptr;
-    bool seed_valid = false, mu_valid = false;
+    bool seed_valid = false, mu_valid = false;  // seed of the current θ; μ of that seed
+    bool mu_have = false;                        // d_mu holds a completed adjoint solve
+    double *d_gacc[2] = {nullptr, nullptr};  // accumulated dJ/dλ, dJ/dhsrc
+    std::vector<double *> slots;             // device copies of fields (h3d_store_field)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evb[2] = {nullptr, nullptr};
     double last_ms = 0.0;
     int verbose = 0;             // H3D_VERBOSE=1: refinement diagnostics on stderr
@@ -916,6 +919,10 @@ int h3d_destroy(h3d_ctx *c)
                    c->d_seed,   c->d_mu,    c->d_save[0],   c->d_save[1],   c->d_bmax,     c->d_scalar};
     for (void *p : rbs)
         if (p) (void)hipFree(p);
+    for (double *p : c->slots)
+        if (p) (void)hipFree(p);
+    for (int a = 0; a < 2; ++a)
+        if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_tab64, c->d_tab32, c->rb.partials,
                   c->d_corr, c->d_best};
@@ -1327,11 +1334,16 @@ int h3d_objective(h3d_ctx *c, const uint8_t *region, size_t n, double beta, doub
     return H3D_OK;
 }
 
-int h3d_adjoint_solve(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
-                      int *isconverged, int *itr, double *res0)
+// Aᵀ μ = weight·seed (+ D∘μ_next when chain), θ and b restored afterwards.
+// D is ∂b/∂θ_old of calRHS! (k_adjoint_rhs), μ_next the adjoint in d_mu.
+int adjoint_core(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
+                 int precision, double weight, int chain, int *isconverged, int *itr, double *res0)
 {
     if (!c) return H3D_ERR_ARG;
     if (!c->seed_valid) return fail(c, H3D_ERR_STATE, "h3d_objective has not been called");
+    if (chain && is_steady) return fail(c, H3D_ERR_ARG, "chaining needs the transient operator (is_steady = 0)");
+    if (chain && !c->mu_have) return fail(c, H3D_ERR_STATE, "no adjoint of the following step to chain");
+    if (!std::isfinite(weight)) return fail(c, H3D_ERR_ARG, "weight must be finite");
     hipStream_t s = c->stream;
     const size_t bytes = c->H * sizeof(double);
     for (int a = 0; a < 2; ++a)
@@ -1340,16 +1352,138 @@ int h3d_adjoint_solve(h3d_ctx *c, int solver, double dt, double tol, int maxItr,
     double *th = c->f[H3D_F_THETA], *b = c->f[H3D_F_B];
     HIPCHK(c, hipMemcpyAsync(c->d_save[0], th, bytes, hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipMemcpyAsync(c->d_save[1], b, bytes, hipMemcpyDeviceToDevice, s));
-    // μ starts at 0 with a zero halo; the right-hand side is the seed
+    // μ starts at 0 with a zero halo; the right-hand side has a zero halo
+    if (weight == 1.0 && !chain) {
+        HIPCHK(c, hipMemcpyAsync(b, c->d_seed, bytes, hipMemcpyDeviceToDevice, s));
+    } else {
+        Geo<double> g;
+        const int rg = make_geo64(c, dt, chain ? 0 : 1, &g);
+        if (rg != H3D_OK) return rg;
+        double hconv[6];
+        for (int f = 0; f < 6; ++f) hconv[f] = c->bc_type[f] == H3D_CONVECTION ? c->bc_h[f] : 0.0;
+        HIPCHK(c, launch_fill<double>(b, 0.0, c->H, s));
+        HIPCHK(c, launch_adjoint_rhs(b, c->d_seed, chain ? c->d_mu : nullptr, g, weight, hconv, s));
+    }
     HIPCHK(c, launch_fill<double>(th, 0.0, c->H, s));
-    HIPCHK(c, hipMemcpyAsync(b, c->d_seed, bytes, hipMemcpyDeviceToDevice, s));
-    const int rc = solve(c, solver, dt, tol, maxItr, smoother, is_steady, H3D_FP64, 0.0, isconverged, itr, res0);
+    c->mu_valid = c->mu_have = false;
+    const int rc = solve(c, solver, dt, tol, maxItr, smoother, is_steady, precision, 0.0, isconverged, itr, res0);
     HIPCHK(c, hipMemcpyAsync(c->d_mu, th, bytes, hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipMemcpyAsync(th, c->d_save[0], bytes, hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipMemcpyAsync(b, c->d_save[1], bytes, hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipStreamSynchronize(s));
-    c->mu_valid = (rc == H3D_OK);
+    c->mu_valid = c->mu_have = (rc == H3D_OK);
     return rc;
+}
+
+int h3d_adjoint_solve(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int is_steady,
+                      int *isconverged, int *itr, double *res0)
+{
+    return adjoint_core(c, solver, dt, tol, maxItr, smoother, is_steady, H3D_FP64, 1.0, 0, isconverged, itr, res0);
+}
+
+int h3d_adjoint_step(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoother, int precision,
+                     double weight, int chain, int *isconverged, int *itr, double *res0)
+{
+    return adjoint_core(c, solver, dt, tol, maxItr, smoother, 0, precision, weight, chain, isconverged, itr, res0);
+}
+
+int h3d_gradients_accumulate(h3d_ctx *c, double w_lambda, double w_hsrc)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!c->mu_valid) return fail(c, H3D_ERR_STATE, "h3d_adjoint_solve has not completed");
+    hipStream_t s = c->stream;
+    const size_t bytes = c->H * sizeof(double);
+    for (int a = 0; a < 2; ++a)
+        if (!c->d_gacc[a]) {
+            HIPCHK(c, hipMalloc(&c->d_gacc[a], bytes));
+            HIPCHK(c, launch_fill<double>(c->d_gacc[a], 0.0, c->H, s));
+        }
+    Geo<double> g = base_geo<double>(c);
+    g.ZC = c->d_ZC; g.dZ = c->d_dZ; g.lam = c->f[H3D_F_LAMBDA];
+    int iso = 0;
+    for (int f = 0; f < 6; ++f)
+        if (c->bc_type[f] == H3D_ISOTHERMAL) iso |= 1 << f;
+    HIPCHK(c, launch_fill<double>(c->d_save[0], 0.0, c->H, s));
+    HIPCHK(c, launch_fill<double>(c->d_save[1], 0.0, c->H, s));
+    HIPCHK(c, launch_gradients(c->f[H3D_F_THETA], c->d_mu, g, iso, c->d_save[0], c->d_save[1], s));
+    HIPCHK(c, launch_accumulate(c->d_gacc[0], c->d_gacc[1], c->d_save[0], c->d_save[1], w_lambda, w_hsrc, c->H, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return H3D_OK;
+}
+
+int h3d_gradients_accumulated(h3d_ctx *c, double *dJ_dlambda, double *dJ_dhsrc, size_t n, int reset)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (n != c->H) return fail(c, H3D_ERR_ARG, "outputs must have MX*MY*MZ entries");
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipStreamSynchronize(s));
+    double *out[2] = {dJ_dlambda, dJ_dhsrc};
+    for (int a = 0; a < 2; ++a) {
+        if (!out[a]) continue;
+        if (c->d_gacc[a])
+            HIPCHK(c, hipMemcpy(out[a], c->d_gacc[a], n * sizeof(double), hipMemcpyDeviceToHost));
+        else
+            std::fill(out[a], out[a] + n, 0.0);
+    }
+    if (reset)
+        for (int a = 0; a < 2; ++a)
+            if (c->d_gacc[a]) HIPCHK(c, launch_fill<double>(c->d_gacc[a], 0.0, c->H, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return H3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// field slots (checkpoints) and memory use
+// ---------------------------------------------------------------------------
+int h3d_store_field(h3d_ctx *c, int field, int slot)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(field) || field == H3D_F_MASK) return fail(c, H3D_ERR_ARG, "bad field id");
+    if (slot < 0 || slot >= H3D_MAX_SLOTS) return fail(c, H3D_ERR_ARG, "slot out of range");
+    if ((size_t)slot >= c->slots.size()) c->slots.resize((size_t)slot + 1, nullptr);
+    const size_t bytes = c->H * sizeof(double);
+    if (!c->slots[slot]) HIPCHK(c, hipMalloc(&c->slots[slot], bytes));
+    HIPCHK(c, hipMemcpyAsync(c->slots[slot], c->f[field], bytes, hipMemcpyDeviceToDevice, c->stream));
+    return H3D_OK;
+}
+
+int h3d_load_field(h3d_ctx *c, int field, int slot, double scale)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(field) || field == H3D_F_MASK) return fail(c, H3D_ERR_ARG, "bad field id");
+    if (slot < 0 || (size_t)slot >= c->slots.size() || !c->slots[slot])
+        return fail(c, H3D_ERR_STATE, "slot holds no field");
+    if (!std::isfinite(scale)) return fail(c, H3D_ERR_ARG, "scale must be finite");
+    if (scale == 1.0)
+        HIPCHK(c, hipMemcpyAsync(c->f[field], c->slots[slot], c->H * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    else
+        HIPCHK(c, launch_scaled_copy(c->f[field], c->slots[slot], scale, c->H, c->stream));
+    if (field == H3D_F_LAMBDA) c->lam_version++;
+    return H3D_OK;
+}
+
+int h3d_release_slots(h3d_ctx *c)
+{
+    if (!c) return H3D_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (double *&p : c->slots) {
+        if (p) HIPCHK(c, hipFree(p));
+        p = nullptr;
+    }
+    c->slots.clear();
+    return H3D_OK;
+}
+
+int h3d_mem_info(h3d_ctx *c, size_t *used, size_t *total)
+{
+    if (!c) return H3D_ERR_ARG;
+    size_t fr = 0, tot = 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemGetInfo(&fr, &tot));
+    if (used) *used = tot - fr;
+    if (total) *total = tot;
+    return H3D_OK;
 }
 
 int h3d_gradients(h3d_ctx *c, double *dJ_dlambda, double *dJ_dhsrc, size_t n)

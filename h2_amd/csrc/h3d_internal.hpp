@@ -227,6 +227,13 @@ hipError_t launch_scale_interior(double *w, const Geo<double> &g, double sc, hip
 hipError_t launch_gradients(const double *th, const double *mu, const Geo<double> &g, int iso, double *glam,
                             double *ghs, hipStream_t s);
 
+// rhs = w·seed + D∘μ_next (μ_next may be nullptr): one step of the transient adjoint
+hipError_t launch_adjoint_rhs(double *rhs, const double *seed, const double *mu_next, const Geo<double> &g, double w,
+                              const double hconv[6], hipStream_t s);
+hipError_t launch_accumulate(double *acc_l, double *acc_h, const double *gl, const double *gh, double wl, double wh,
+                             size_t n, hipStream_t s);
+hipError_t launch_scaled_copy(double *dst, const double *src, double f, size_t n, hipStream_t s);
+
 // ---- RB-layout launchers (h3d_rb.hip) ---------------------------------------
 // dst (RB) = src (standard layout), every cell including the halo
 template <typename D> hipError_t launch_to_rb(D *dst, const D *src, int MX, int MY, int MZ, hipStream_t s);

@@ -298,6 +298,45 @@ class WorkBuffers:
         _check(self.ctx, self.lib.h3d_get_adjoint(self.ctx, L.ptr(mu), L.ptr(seed), self.n))
         return mu, seed
 
+    def adjoint_step(self, solver, dt, tol, weight, chain, max_itr=ITR_MAX, smoother=L.SMOOTHER_GS,
+                     precision=L.FP64):
+        """Aᵀμ_n = weight·seed + D μ_{n+1} (chain) on the transient operator."""
+        conv, itr, res0 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_adjoint_step(self.ctx, int(solver), float(dt), float(tol), int(max_itr),
+                                                   int(smoother), int(precision), float(weight), int(bool(chain)),
+                                                   ctypes.byref(conv), ctypes.byref(itr), ctypes.byref(res0)))
+        return bool(conv.value), itr.value, res0.value
+
+    def accumulate_gradients(self, w_lambda, w_hsrc):
+        _check(self.ctx, self.lib.h3d_gradients_accumulate(self.ctx, float(w_lambda), float(w_hsrc)))
+
+    def accumulated_gradients(self, reset=False):
+        """(Σ w·dJ/dλ, Σ w·dJ/dhsrc) accumulated on the device."""
+        gl = np.zeros(self.shape, order="F")
+        gh = np.zeros(self.shape, order="F")
+        _check(self.ctx, self.lib.h3d_gradients_accumulated(self.ctx, L.ptr(gl), L.ptr(gh), self.n,
+                                                            int(bool(reset))))
+        return gl, gh
+
+    def reset_gradients(self):
+        _check(self.ctx, self.lib.h3d_gradients_accumulated(self.ctx, None, None, self.n, 1))
+
+    # device-resident field copies (checkpoints) ------------------------------
+    def store(self, name, slot):
+        _check(self.ctx, self.lib.h3d_store_field(self.ctx, self._fid(name), int(slot)))
+
+    def load(self, name, slot, scale=1.0):
+        _check(self.ctx, self.lib.h3d_load_field(self.ctx, self._fid(name), int(slot), float(scale)))
+
+    def release_slots(self):
+        _check(self.ctx, self.lib.h3d_release_slots(self.ctx))
+
+    def mem_info(self):
+        """(bytes in use on the device, device total)."""
+        used, total = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(self.ctx, self.lib.h3d_mem_info(self.ctx, ctypes.byref(used), ctypes.byref(total)))
+        return used.value, total.value
+
 
 # ---------------------------------------------------------------------------
 # host model builder (libh3d's h3d_model_* functions)

@@ -72,6 +72,90 @@ def delta_J(theta_p, theta_m, region, beta=BETA):
     return np.log1p(np.sum(wm * np.expm1(beta * (tp - tm))) / np.sum(wm)) / beta
 
 
+def halo_lambda(m, lam):
+    """λ with the halo apply_boundary_conditions! leaves: the interior
+    neighbour's λ on isothermal faces, 0 on flux and convective faces
+    (src/boundary_conditions.jl:94-313; faces x-,x+,y-,y+,z-,z+ in order)."""
+    lam = np.array(lam, order="F", copy=True)
+    sl = [(0, 1), (-1, -2)]
+    for f, bc in enumerate(m.bc_set.faces()):
+        ax, side = f // 2, f % 2
+        dst, src = [slice(None)] * 3, [slice(None)] * 3
+        dst[ax], src[ax] = sl[side][0], sl[side][1]
+        lam[tuple(dst)] = lam[tuple(src)] if bc.type == L.ISOTHERMAL else 0.0
+    return lam
+
+
+def conductance_change(m, lam_p, lam_0):
+    """Per face direction, G·(λf(λp) - λf(λ0)) over the interior cells:
+    the change of the flux-form face conductances (G the face's area over
+    distance, λf the harmonic mean with factor 2 at the halo)."""
+    NX, NY, NZ = m.NX, m.NY, m.NZ
+    I = (slice(1, NX + 1), slice(1, NY + 1), slice(1, NZ + 1))
+    lp, l0 = halo_lambda(m, lam_p), halo_lambda(m, lam_0)
+    dz = m.dZ[None, None, 1:NZ + 1]
+    zc = m.ZC
+    out = []
+    for ax in range(3):
+        for d in (-1, 1):
+            nb = list(I)
+            nb[ax] = slice(I[ax].start + d, I[ax].stop + d)
+            nb = tuple(nb)
+            n_ax = (NX, NY, NZ)[ax]
+            inner = np.ones((NX, NY, NZ), dtype=bool)
+            edge = [slice(None)] * 3
+            edge[ax] = 0 if d < 0 else n_ax - 1
+            inner[tuple(edge)] = False
+            fac = np.where(inner, 1.0, 2.0)
+            if ax == 0:
+                G = m.dy * dz / m.dx
+            elif ax == 1:
+                G = m.dx * dz / m.dy
+            else:
+                k = np.arange(1, NZ + 1)
+                G = (m.dx * m.dy / np.abs(zc[k + d] - zc[k]))[None, None, :]
+
+            def lamf(a, b):
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    v = 2.0 * a * b / (a + b) * fac
+                return np.where(a + b > 0, v, 0.0)
+
+            dG = G * (lamf(lp[nb], lp[I]) - lamf(l0[nb], l0[I]))
+            sel = np.nonzero(dG)
+            out.append((nb, sel, dG[sel]))
+    return out
+
+
+def apply_conductance_change(m, change, theta):
+    """(A(λp) - A(λ0))·θ over the interior for a conductance_change."""
+    NX, NY, NZ = m.NX, m.NY, m.NZ
+    I = (slice(1, NX + 1), slice(1, NY + 1), slice(1, NZ + 1))
+    out = np.zeros(theta.shape, order="F")
+    oi = out[I]
+    ti = theta[I]
+    for nb, sel, dG in change:
+        oi[sel] += dG * (theta[nb][sel] - ti[sel])
+    return out
+
+
+def operator_change(m, lam_p, lam_0, theta):
+    """(A(λp) - A(λ0))·θ over the interior in the flux form: per face,
+    G·(λf(λp) - λf(λ0))·(θ_n - θ_c).  Convective and transient terms do
+    not depend on λ."""
+    return apply_conductance_change(m, conductance_change(m, lam_p, lam_0), theta)
+
+
+def delta_J_split(theta0, d_p, d_m, region, beta=BETA):
+    """J(θ0 + d_p) - J(θ0 + d_m) from the changes themselves: the
+    exponents' differences are β(d_p - d_m), never a difference of two
+    temperatures near 300 K."""
+    sel = np.asarray(region, dtype=bool)
+    t0, dp, dm = theta0[sel], d_p[sel], d_m[sel]
+    tm = t0 + dm
+    wm = np.exp(beta * (tm - tm.max()))
+    return np.log1p(np.sum(wm * np.expm1(beta * (dp - dm))) / np.sum(wm)) / beta
+
+
 class SteadySensitivity:
     """Forward solve, objective and adjoint gradients on one device context."""
 
@@ -131,56 +215,10 @@ class SteadySensitivity:
                 "adjoint": {"converged": conv, "itr": itr, "res0": res0, "solve_ms": adj_ms}}
 
     def halo_lambda(self, lam):
-        """λ with the halo apply_boundary_conditions! leaves: the interior
-        neighbour's λ on isothermal faces, 0 on flux and convective faces
-        (src/boundary_conditions.jl:94-313; faces x-,x+,y-,y+,z-,z+ in order)."""
-        lam = np.array(lam, order="F", copy=True)
-        sl = [(0, 1), (-1, -2)]
-        for f, bc in enumerate(self.m.bc_set.faces()):
-            ax, side = f // 2, f % 2
-            dst, src = [slice(None)] * 3, [slice(None)] * 3
-            dst[ax], src[ax] = sl[side][0], sl[side][1]
-            lam[tuple(dst)] = lam[tuple(src)] if bc.type == L.ISOTHERMAL else 0.0
-        return lam
+        return halo_lambda(self.m, lam)
 
     def operator_change(self, lam_p, lam_0, theta):
-        """(A(λp) - A(λ0))·θ over the interior in the flux form: per face,
-        G·(λf(λp) - λf(λ0))·(θ_n - θ_c).  Convective and transient terms do
-        not depend on λ."""
-        m = self.m
-        NX, NY, NZ = m.NX, m.NY, m.NZ
-        I = (slice(1, NX + 1), slice(1, NY + 1), slice(1, NZ + 1))
-        lp, l0 = self.halo_lambda(lam_p), self.halo_lambda(lam_0)
-        dz = m.dZ[None, None, 1:NZ + 1]
-        zc = m.ZC
-        out = np.zeros(lam_p.shape, order="F")
-        for ax in range(3):
-            for d in (-1, 1):
-                nb = list(I)
-                nb[ax] = slice(I[ax].start + d, I[ax].stop + d)
-                nb = tuple(nb)
-                n_ax = (NX, NY, NZ)[ax]
-                inner = np.ones((NX, NY, NZ), dtype=bool)
-                edge = [slice(None)] * 3
-                edge[ax] = 0 if d < 0 else n_ax - 1
-                inner[tuple(edge)] = False
-                fac = np.where(inner, 1.0, 2.0)
-                if ax == 0:
-                    G = m.dy * dz / m.dx
-                elif ax == 1:
-                    G = m.dx * dz / m.dy
-                else:
-                    k = np.arange(1, NZ + 1)
-                    G = (m.dx * m.dy / np.abs(zc[k + d] - zc[k]))[None, None, :]
-
-                def lamf(a, b):
-                    with np.errstate(invalid="ignore", divide="ignore"):
-                        v = 2.0 * a * b / (a + b) * fac
-                    return np.where(a + b > 0, v, 0.0)
-
-                dl = lamf(lp[nb], lp[I]) - lamf(l0[nb], l0[I])
-                out[I] += G * dl * (theta[nb] - theta[I])
-        return out
+        return operator_change(self.m, lam_p, lam_0, theta)
 
     def _solve_change(self, rhs, lam, solver=None):
         """δ of A(λ) δ = rhs with a zero halo (the solve's tolerance is

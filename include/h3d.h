@@ -193,6 +193,41 @@ int h3d_gradients(h3d_ctx *ctx, double *dJ_dlambda, double *dJ_dhsrc, size_t n);
 /* The adjoint μ and the seed ∂J/∂θ (either may be NULL). */
 int h3d_get_adjoint(h3d_ctx *ctx, double *mu, double *seed, size_t n);
 
+/* One step of the adjoint of an implicit-Euler run (no reference
+ * counterpart; the forward step is calRHS! then PBiCGSTAB!/CG! with
+ * is_steady = false, src/heat3ds.jl:124-139).  For
+ * J_total = Σ_n w_n J(θⁿ) and θⁿ solving A θⁿ = b(θⁿ⁻¹), the adjoint runs
+ * backwards from μ_{N+1} = 0:
+ *     Aᵀ μ_n = w_n ∂J/∂θ(θⁿ) + D μ_{n+1},
+ *     D_c = ∂b_c/∂θ_c^old = -ap0_c + Σ_{convective faces f of c} ±h_f·area_f
+ * (+ on x+/y+/z+ faces, - on x-/y-/z- faces: calRHS! subtracts the minus
+ * faces' h·area·(θ∞ - θ) and adds the plus faces', src/RHS.jl).
+ * Call h3d_objective on θⁿ first (it sets the seed); chain = 0 for the last
+ * step n = N, 1 for the others (μ_{n+1} is the adjoint the previous call
+ * left).  precision as h3d_pbicgstab.  θ and b are left as they were. */
+int h3d_adjoint_step(h3d_ctx *ctx, int solver, double dt, double tol, int maxItr, int smoother, int precision,
+                     double weight, int chain, int *isconverged, int *itr, double *res0);
+/* Adds w_lambda·dJ/dλ and w_hsrc·dJ/dhsrc of the current θ and μ (the
+ * per-cell gradients of h3d_gradients) to device accumulators; with
+ * hsrc_n = f_n·hsrc_base, w_hsrc = f_n gives dJ_total/dhsrc_base. */
+int h3d_gradients_accumulate(h3d_ctx *ctx, double w_lambda, double w_hsrc);
+/* Copies the accumulators out (either pointer may be NULL; zeros before
+ * the first accumulation) and, when reset != 0, clears them. */
+int h3d_gradients_accumulated(h3d_ctx *ctx, double *dJ_dlambda, double *dJ_dhsrc, size_t n, int reset);
+
+/* Device-resident field copies (checkpoints; no reference counterpart).
+ * h3d_store_field: slot := field (allocated on first use, 0 <= slot <
+ * H3D_MAX_SLOTS).  h3d_load_field: field := scale·slot (scale 1 copies
+ * bit for bit; the time-varying heat source is a scaled base field).
+ * h3d_release_slots frees them all.  The mask field has no storage. */
+#define H3D_MAX_SLOTS 4096
+int h3d_store_field(h3d_ctx *ctx, int field, int slot);
+int h3d_load_field(h3d_ctx *ctx, int field, int slot, double scale);
+int h3d_release_slots(h3d_ctx *ctx);
+/* Device memory in use (all processes) and the device's total, in bytes
+ * (hipMemGetInfo on the context's device). */
+int h3d_mem_info(h3d_ctx *ctx, size_t *used, size_t *total);
+
 /* Timing of one kernel family on the context stream (HIP events around
  * reps back-to-back launches after one warm-up launch; *ms = mean per
  * launch).  which: 0 = one full red-black GS sweep (both colours),

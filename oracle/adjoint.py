@@ -151,3 +151,119 @@ class SteadyProblem:
         V = np.zeros(th.shape, order="F")
         V[1:-1, 1:-1, 1:-1] = c.dx * c.dy * self.base.dZ[None, None, 1:-1]
         return {"J": J, "theta": th, "mu": mu, "seed": g, "dJ_dscale": dscale, "dJ_dhsrc": -mu * V}
+
+
+def probe_rhs_derivative(c):
+    """∂b/∂θ_old of the oracle's calRHS! (orc_cal_rhs, src/NonUniform.jl:85-90
+    and src/RHS.jl) as a sparse interior matrix.  b is affine in the θ it
+    is formed at, so 27 lattice probes of b(θ = 2²⁰·e) - b(θ = 0) with hsrc
+    = 0 recover it (the large probe keeps θ∞ - θ free of rounding)."""
+    MX, MY, MZ = c.shape
+    NX, NY, NZ = MX - 2, MY - 2, MZ - 2
+    n = NX * NY * NZ
+    idx = np.arange(n).reshape((NX, NY, NZ), order="F")
+    ii, jj, kk = np.meshgrid(np.arange(NX), np.arange(NY), np.arange(NZ), indexing="ij")
+    saved = {f: c.arr[f].copy(order="F") for f in ("theta", "b", "hsrc")}
+    big = 2.0 ** 20
+
+    def rhs_at(x):
+        c.arr["theta"][:] = 0.0
+        c.arr["theta"][1:-1, 1:-1, 1:-1] = x
+        c.arr["hsrc"][:] = 0.0
+        c.arr["b"][:] = 0.0
+        c.cal_rhs()
+        return c.arr["b"][1:-1, 1:-1, 1:-1].copy()
+
+    b0 = rhs_at(np.zeros((NX, NY, NZ)))
+    rows, cols, vals = [], [], []
+    for a in range(3):
+        for b in range(3):
+            for d in range(3):
+                sel = (ii % 3 == a) & (jj % 3 == b) & (kk % 3 == d)
+                y = (rhs_at(np.where(sel, big, 0.0)) - b0) / big
+                for di, dj, dk in ((0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                    ci, cj, ck = ii + di, jj + dj, kk + dk
+                    ok = (ci >= 0) & (ci < NX) & (cj >= 0) & (cj < NY) & (ck >= 0) & (ck < NZ)
+                    ok &= (ci % 3 == a) & (cj % 3 == b) & (ck % 3 == d)
+                    rows.append(idx[ok])
+                    cols.append(idx[ci[ok], cj[ok], ck[ok]])
+                    vals.append(y[ok])
+    for f, v in saved.items():
+        c.arr[f][:] = v
+    D = coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n)).tocsr()
+    D.eliminate_zeros()
+    return D
+
+
+class TransientProblem(SteadyProblem):
+    """N implicit-Euler steps of Δt from 300 K on a modelA case with
+    hsrc_n = f_n·hsrc, f_n = 1 + amp·sin(2π nΔt/period), b cleared before
+    every calRHS!, J_total = Σ_n Δt·J(θⁿ).  Direct solves; the adjoint
+    recursion Aᵀμ_n = Δt g_n + Dᵀ μ_{n+1} uses the probed A and D."""
+
+    def __init__(self, NX, NY, NZ, groups, dt, nsteps, amp=0.5, period=20e-3, hsrc=None):
+        super().__init__(NX, NY, NZ, groups, hsrc)
+        self.dt, self.nsteps, self.amp, self.period = dt, nsteps, amp, period
+
+    def factor(self, n):
+        return 1.0 + self.amp * np.sin(2.0 * np.pi * (n * self.dt) / self.period)
+
+    def case(self, scales, hsrc=None):
+        NX, NY, NZ = [s - 2 for s in self.base.shape]
+        c = O.modelA_case(NX, NY, NZ, steady=False, dt=self.dt)
+        lam = self.lam0.copy(order="F")
+        for k, s in enumerate(scales):
+            sel = self.groups == k
+            lam[sel] = self.lam0[sel] * s
+        c.arr["lam"][:] = lam
+        c.arr["theta"][:] = 300.0
+        c.apply_bc()
+        return c
+
+    def run(self, scales, beta, hsrc=None):
+        """J_total, the states θ⁰..θᴺ, the seeds, A, D and the case."""
+        base = self.hsrc if hsrc is None else hsrc
+        c = self.case(scales)
+        A = probe_matrix(c)
+        D = probe_rhs_derivative(c)
+        thetas, seeds, J = [c.arr["theta"].copy(order="F")], [None], 0.0
+        for n in range(1, self.nsteps + 1):
+            c.arr["hsrc"][:] = self.factor(n) * base
+            c.arr["b"][:] = 0.0
+            c.cal_rhs()
+            c.calc_rk("r", "theta", "b")
+            d = spsolve(A, c.arr["r"][1:-1, 1:-1, 1:-1].ravel(order="F"))
+            c.arr["theta"][1:-1, 1:-1, 1:-1] += d.reshape(c.arr["theta"][1:-1, 1:-1, 1:-1].shape, order="F")
+            Jn, g = smooth_max(c.arr["theta"], self.region, beta)
+            J += self.dt * Jn
+            thetas.append(c.arr["theta"].copy(order="F"))
+            seeds.append(g)
+        return J, thetas, seeds, A, D, c
+
+    def adjoint(self, scales, beta, h=1e-3, groups=None):
+        """J_total and its gradients by the adjoint recursion; the operator
+        derivative per step as in SteadyProblem.adjoint.  groups limits
+        the conductivity groups (all by default)."""
+        J, thetas, seeds, A, D, c = self.run(scales, beta)
+        AT, DT = A.T.tocsr(), D.T.tocsr()
+        shape = thetas[0].shape
+        groups = range(len(scales)) if groups is None else groups
+        V = np.zeros(shape, order="F")
+        V[1:-1, 1:-1, 1:-1] = c.dx * c.dy * self.base.dZ[None, None, 1:-1]
+        dscale = np.zeros(len(scales))
+        ghs = np.zeros(shape, order="F")
+        mus = [None] * (self.nsteps + 2)
+        nxt = np.zeros(AT.shape[0])
+        for n in range(self.nsteps, 0, -1):
+            rhs = self.dt * seeds[n][1:-1, 1:-1, 1:-1].ravel(order="F") + DT @ nxt
+            nxt = spsolve(AT, rhs)
+            mu = np.zeros(shape, order="F")
+            mu[1:-1, 1:-1, 1:-1] = nxt.reshape(mu[1:-1, 1:-1, 1:-1].shape, order="F")
+            mus[n] = mu
+            for k in groups:
+                dscale[k] -= np.sum(mu * self.operator_derivative(thetas[n], scales, k, h))
+            ghs -= self.factor(n) * mu * V
+        return {"J": J, "thetas": thetas, "mus": mus, "dJ_dscale": dscale, "dJ_dhsrc": ghs, "D": D, "A": A}
+
+    def objective(self, scales, beta, hsrc=None):
+        return self.run(scales, beta, hsrc)[0]
