@@ -52,7 +52,8 @@ struct h3d_ctx {
     unsigned long lam_version = 1, cls_version = 0;
     int K = 0;                   // 0: table form unavailable for the current λ
     uint8_t *d_cls = nullptr;
-    uint64_t *d_vals = nullptr;
+    uint64_t *d_vals = nullptr;          // class keys (λ bits, ρ·cp bits), sorted
+    double *d_capc = nullptr;            // ρ·cp per class
     double *d_tab64 = nullptr;
     float *d_tab32 = nullptr;
     // FP32 shadows for the refinement inner solve (allocated on first use)
@@ -73,8 +74,7 @@ struct h3d_ctx {
     uint8_t *d_cls_rb = nullptr;
     double *d_ZF64 = nullptr;
     float *d_ZF32 = nullptr;
-    double *d_ap0_rb = nullptr;
-    float *d_ap0_rb32 = nullptr;
+    double ap0_ddt = 0.0;                 // 1/Δt of the last transient ap0
     double *rb64[8] = {};                 // p, p_, r, r0, q, s, s_, t in RB layout
     float *rb32[8] = {};
     // instantiated iteration graphs, reused while nothing they captured changed
@@ -123,6 +123,9 @@ int fail(h3d_ctx *c, int code, const std::string &msg)
 
 bool field_ok(int f) { return f >= 0 && f < H3D_NFIELDS; }
 
+// fields the coefficient classes are formed from
+bool coef_field(int f) { return f == H3D_F_LAMBDA || f == H3D_F_RHO || f == H3D_F_CP; }
+
 // reduction scratch for up to nb workgroups, with the pre-reduction area after it
 hipError_t alloc_partials(RedBuf *rb, long nb)
 {
@@ -154,25 +157,32 @@ T lamf_host(T a, T b, int m)
     return T(2) * a * b / (a + b) * f;
 }
 
-// Classes of the conductivity field for the table form: the distinct λ bit
-// patterns (at most kMaxClasses), every cell's index among them, and the
-// tables tab[m][a][b] = λf(v_a, v_b, m, 1) in FP64 and FP32.  Rebuilt when
-// λ changed since the last build; K = 0 when there are too many values.
+// Classes of the coefficient fields for the table form: the distinct
+// (λ, ρ·cp) bit-pattern pairs (at most kMaxClasses), every cell's index
+// among them, the tables tab[m][a][b] = λf(λ_a, λ_b, m, 1) in FP64 and
+// FP32, and the heat capacities ρ·cp per class (the colour-split kernels
+// form ap0 from them per plane).  Rebuilt when λ, ρ or cp changed since the
+// last build; K = 0 when there are too many pairs.
 int ensure_classes(h3d_ctx *c)
 {
     if (c->cls_version == c->lam_version) return H3D_OK;
     c->K = 0;
     c->cls_version = c->lam_version;
     if (c->coef_form == H3D_COEF_DIRECT) return H3D_OK;
-    std::vector<double> lam(c->H);
+    std::vector<double> lam(c->H), rho(c->H), cp(c->H);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(lam.data(), c->f[H3D_F_LAMBDA], c->H * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<uint64_t> vals;
-    uint64_t last = 0;
+    HIPCHK(c, hipMemcpy(rho.data(), c->f[H3D_F_RHO], c->H * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cp.data(), c->f[H3D_F_CP], c->H * sizeof(double), hipMemcpyDeviceToHost));
+    using Key = std::pair<uint64_t, uint64_t>;
+    std::vector<Key> vals;
+    Key last{0, 0};
     bool have_last = false;
     for (size_t a = 0; a < c->H; ++a) {
-        uint64_t v;
-        std::memcpy(&v, &lam[a], sizeof v);
+        const double pc = rho[a] * cp[a];
+        Key v;
+        std::memcpy(&v.first, &lam[a], 8);
+        std::memcpy(&v.second, &pc, 8);
         if (have_last && v == last) continue;
         last = v;
         have_last = true;
@@ -183,27 +193,36 @@ int ensure_classes(h3d_ctx *c)
     }
     std::sort(vals.begin(), vals.end());
     const int K = (int)vals.size();
-    std::vector<double> t64((size_t)2 * K * K);
+    std::vector<double> t64((size_t)2 * K * K), pcs(K);
     std::vector<float> t32((size_t)2 * K * K);
+    std::vector<uint64_t> flat(2 * (size_t)K);
+    for (int a = 0; a < K; ++a) {
+        std::memcpy(&pcs[a], &vals[a].second, 8);
+        flat[2 * a] = vals[a].first;
+        flat[2 * a + 1] = vals[a].second;
+    }
     for (int m = 0; m < 2; ++m)
         for (int a = 0; a < K; ++a)
             for (int b = 0; b < K; ++b) {
                 double va, vb;
-                std::memcpy(&va, &vals[a], 8);
-                std::memcpy(&vb, &vals[b], 8);
+                std::memcpy(&va, &vals[a].first, 8);
+                std::memcpy(&vb, &vals[b].first, 8);
                 t64[((size_t)m * K + a) * K + b] = lamf_host<double>(va, vb, m);
                 t32[((size_t)m * K + a) * K + b] = lamf_host<float>((float)va, (float)vb, m);
             }
     if (!c->d_cls) HIPCHK(c, hipMalloc(&c->d_cls, c->H));
-    if (!c->d_vals) HIPCHK(c, hipMalloc(&c->d_vals, kMaxClasses * sizeof(uint64_t)));
+    if (!c->d_vals) HIPCHK(c, hipMalloc(&c->d_vals, 2 * kMaxClasses * sizeof(uint64_t)));
+    if (!c->d_capc) HIPCHK(c, hipMalloc(&c->d_capc, kMaxClasses * sizeof(double)));
     if (!c->d_tab64) HIPCHK(c, hipMalloc(&c->d_tab64, 2 * kMaxClasses * kMaxClasses * sizeof(double)));
     if (!c->d_tab32) HIPCHK(c, hipMalloc(&c->d_tab32, 2 * kMaxClasses * kMaxClasses * sizeof(float)));
-    HIPCHK(c, hipMemcpy(c->d_vals, vals.data(), K * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_vals, flat.data(), flat.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_capc, pcs.data(), K * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
     c->h_tab64 = t64;
     c->h_tab32 = t32;
-    HIPCHK(c, launch_classify(c->d_cls, c->f[H3D_F_LAMBDA], c->d_vals, K, c->H, c->stream));
+    HIPCHK(c, launch_classify(c->d_cls, c->f[H3D_F_LAMBDA], c->f[H3D_F_RHO], c->f[H3D_F_CP], c->d_vals, K, c->H,
+                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->K = K;
     return H3D_OK;
@@ -231,6 +250,7 @@ int make_geo64(h3d_ctx *c, double dt, int is_steady, Geo<double> *out)
         HIPCHK(c, launch_fill<double>(c->d_ap0, 0.0, c->H, c->stream));
         const double ddt = 1.0 / dt;  // ddt = inv(Δt)
         HIPCHK(c, launch_ap0(c->d_ap0, c->f[H3D_F_RHO], c->f[H3D_F_CP], g, ddt, c->stream));
+        c->ap0_ddt = ddt;
         g.ap0 = c->d_ap0;
     }
     *out = g;
@@ -571,16 +591,13 @@ int make_rbgeo(h3d_ctx *c, const Geo<T> &g, RbGeo<T> *out)
     r.flux = g.flux;
     const bool dbl = sizeof(T) == sizeof(double);
     r.ZF = dbl ? reinterpret_cast<const T *>(c->d_ZF64) : reinterpret_cast<const T *>(c->d_ZF32);
-    r.ap0 = nullptr;
-    if (g.ap0) {
-        T **buf = dbl ? reinterpret_cast<T **>(&c->d_ap0_rb) : reinterpret_cast<T **>(&c->d_ap0_rb32);
-        if (!*buf) {
-            HIPCHK(c, hipMalloc(buf, 2 * c->HS * sizeof(T)));
-            HIPCHK(c, launch_fill<T>(*buf, T(0), 2 * c->HS, c->stream));
-        }
-        HIPCHK(c, launch_to_rb<T>(*buf, g.ap0, c->MX, c->MY, c->MZ, c->stream));
-        r.ap0 = *buf;
-    }
+    // ap0 = ρ·cp·dx·dy·ΔZ[k]·(1/Δt) per class and plane, staged in LDS
+    r.ap = g.ap0 != nullptr;
+    r.capc = c->d_capc;
+    r.dZd = c->d_dZ;
+    r.dx0d = c->dx;
+    r.dy0d = c->dy;
+    r.ddt = r.ap ? c->ap0_ddt : 0.0;
     rb_shape(r);
     // CalcAX! launches two workgroups per (m-block, row block, plane)
     if (2 * r.nb1 > (unsigned)c->rb.nblocks_max) {
@@ -915,7 +932,7 @@ int h3d_destroy(h3d_ctx *c)
         if (c->rb64[a]) (void)hipFree(c->rb64[a]);
         if (c->rb32[a]) (void)hipFree(c->rb32[a]);
     }
-    void *rbs[] = {c->d_cls_rb, c->d_ZF64,  c->d_ZF32,      c->d_ap0_rb,    c->d_ap0_rb32, c->d_region,
+    void *rbs[] = {c->d_cls_rb, c->d_ZF64,  c->d_ZF32,      c->d_region,
                    c->d_seed,   c->d_mu,    c->d_save[0],   c->d_save[1],   c->d_bmax,     c->d_scalar};
     for (void *p : rbs)
         if (p) (void)hipFree(p);
@@ -924,7 +941,7 @@ int h3d_destroy(h3d_ctx *c)
     for (int a = 0; a < 2; ++a)
         if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_state,
-                  c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_tab64, c->d_tab32, c->rb.partials,
+                  c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
                   c->d_corr, c->d_best};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -1051,7 +1068,7 @@ int h3d_upload(h3d_ctx *c, int field, const double *host, size_t n)
     }
     HIPCHK(c, hipMemcpyAsync(c->f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (field == H3D_F_LAMBDA) c->lam_version++;
+    if (coef_field(field)) c->lam_version++;
     return H3D_OK;
 }
 
@@ -1078,7 +1095,7 @@ int h3d_fill(h3d_ctx *c, int field, double value)
     if (field == H3D_F_MASK) return fail(c, H3D_ERR_UNSUPPORTED, "the mask is implicit");
     HIPCHK(c, launch_fill<double>(c->f[field], value, c->H, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (field == H3D_F_LAMBDA) c->lam_version++;
+    if (coef_field(field)) c->lam_version++;
     return H3D_OK;
 }
 
@@ -1089,7 +1106,7 @@ int h3d_copy(h3d_ctx *c, int dst, int src)
         return fail(c, H3D_ERR_ARG, "bad field id");
     HIPCHK(c, hipMemcpyAsync(c->f[dst], c->f[src], c->H * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (dst == H3D_F_LAMBDA) c->lam_version++;
+    if (coef_field(dst)) c->lam_version++;
     return H3D_OK;
 }
 
@@ -1459,7 +1476,7 @@ int h3d_load_field(h3d_ctx *c, int field, int slot, double scale)
                                  c->stream));
     else
         HIPCHK(c, launch_scaled_copy(c->f[field], c->slots[slot], scale, c->H, c->stream));
-    if (field == H3D_F_LAMBDA) c->lam_version++;
+    if (coef_field(field)) c->lam_version++;
     return H3D_OK;
 }
 

@@ -122,7 +122,12 @@ struct RbGeo {
     T HC[6];
     const T *dZ;            // device, MZ entries
     const T *ZF;            // device, (NZ+1)*K*K face conductances
-    const T *ap0;           // device, RB layout; nullptr when steady
+    // transient term ap0 = ((((ρ·cp)·dx)·dy)·ΔZ[k])·ddt formed per class and
+    // plane from the FP64 values (the products k_ap0 forms per cell)
+    int ap;                 // 1 for a transient operator
+    const double *capc;     // device, ρ·cp per class
+    const double *dZd;      // device, FP64 ΔZ
+    double dx0d, dy0d, ddt;
     const uint8_t *cls;     // device, RB layout class field
     const T *tab;           // device [2][K][K] λf table
     int K;
@@ -209,7 +214,8 @@ hipError_t launch_rhs_final(double *b, const double *th, const double *hsrc, con
 hipError_t launch_ap0(double *ap0, const double *rho, const double *cp, const Geo<double> &g, double ddt,
                       hipStream_t s);
 // class of every cell: index of λ's bit pattern in the sorted list vals[0..K-1]
-hipError_t launch_classify(uint8_t *cls, const double *lam, const uint64_t *vals, int K, size_t n, hipStream_t s);
+hipError_t launch_classify(uint8_t *cls, const double *lam, const double *rho, const double *cp, const uint64_t *vals,
+                           int K, size_t n, hipStream_t s);
 template <typename D, typename S> hipError_t launch_convert(D *dst, const S *src, size_t n, hipStream_t s);
 // θ += d on the interior: the refinement update
 hipError_t launch_add_correction(double *th, const double *d, const Geo<double> &g, hipStream_t s);

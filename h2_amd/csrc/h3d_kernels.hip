@@ -967,25 +967,30 @@ hipError_t launch_ap0(double *ap0, const double *rho, const double *cp, const Ge
 }
 
 // class of a cell = index of its λ bit pattern in the sorted distinct list
-__global__ void __launch_bounds__(256) k_classify(uint8_t *cls, const double *lam, const uint64_t *vals, int K,
-                                                  size_t n)
+// Class index of every cell: the position of its (λ bits, ρ·cp bits) pair
+// among the K sorted pairs vals[2a], vals[2a+1] (every pair is present).
+__global__ void __launch_bounds__(256) k_classify(uint8_t *cls, const double *lam, const double *rho, const double *cp,
+                                                  const uint64_t *vals, int K, size_t n)
 {
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
         const uint64_t v = (uint64_t)__double_as_longlong(lam[c]);
+        const uint64_t w = (uint64_t)__double_as_longlong(rho[c] * cp[c]);
         int lo = 0, hi = K - 1;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (vals[mid] < v) lo = mid + 1;
+            const uint64_t a = vals[2 * mid], b = vals[2 * mid + 1];
+            if (a < v || (a == v && b < w)) lo = mid + 1;
             else hi = mid;
         }
         cls[c] = (uint8_t)lo;
     }
 }
 
-hipError_t launch_classify(uint8_t *cls, const double *lam, const uint64_t *vals, int K, size_t n, hipStream_t s)
+hipError_t launch_classify(uint8_t *cls, const double *lam, const double *rho, const double *cp, const uint64_t *vals,
+                           int K, size_t n, hipStream_t s)
 {
     const int nb = (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
-    hipLaunchKernelGGL(k_classify, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, cls, lam, vals, K, n);
+    hipLaunchKernelGGL(k_classify, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, cls, lam, rho, cp, vals, K, n);
     return hipGetLastError();
 }
 

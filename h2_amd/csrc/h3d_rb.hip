@@ -75,7 +75,12 @@ __device__ __forceinline__ RbThread rb_thread(const RbGeo<T> &g)
 // 2K² entries indexed [(m ? K : 0) + neighbour class][centre class]:
 //   XT = ((λf * dy0) * ΔZ[k]) * ddx,   YT = ((λf * dx0) * ΔZ[k]) * ddy,
 // the products CalcAX! forms per face, in its order; then the z-face
-// conductances of faces k-1 and k (K² each).  All threads must call.
+// conductances of faces k-1 and k (K² each); for a transient operator the
+// K values ap0 = ((((ρ·cp)·dx)·dy)·ΔZ[k])·ddt of the plane, formed in FP64
+// as k_ap0 forms them per cell and rounded to T.  All threads must call.
+// Kernels issue their loads of x, b and the class bytes first and stage
+// afterwards, so that the table loads (L2 hits) and the barrier overlap
+// the HBM latency instead of preceding it.
 template <typename T>
 __device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g, int k)
 {
@@ -91,14 +96,17 @@ __device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g, int k)
         t[n + a] = f * g.dx0 * dz_k * g.ddy;
         t[2 * n + a] = zf[a];
     }
-    __syncthreads();
+    if (g.ap && tid < g.K) t[3 * n + tid] = (T)(g.capc[tid] * g.dx0d * g.dy0d * g.dZd[k] * g.ddt);
+    // LDS-only: the kernels' HBM loads, issued before the staging, stay
+    // in flight across the barrier
+    lds_barrier();
     return t;
 }
 
 template <typename T>
 static size_t rb_lds(const RbGeo<T> &g)
 {
-    return (size_t)6 * g.K * g.K * sizeof(T);
+    return ((size_t)6 * g.K * g.K + (size_t)g.K) * sizeof(T);
 }
 
 // Byte offsets of a whole RB array (both halves) fit in 32 bits (the host
@@ -147,26 +155,43 @@ __device__ __forceinline__ Nb<T> rb_nb(const T *__restrict__ xt, unsigned no, in
     return n;
 }
 
-// The conductances of cell (i,j,k) from the class field and tables; the
-// same values conductances() forms in h3d_kernels.hip.  clo/clt: the class
-// arrays of the cell's half and of the other half; ap: ap0 of the cell's
-// half or nullptr.
+// The class bytes of a cell and of its six neighbours (all in the other
+// half): clo/clt = the class arrays of the cell's half and of the other
+// half, co/no = the cell's offset in each.
+struct Cls {
+    int c, xm, xp, ym, yp, zm, zp;
+};
+
+__device__ __forceinline__ Cls rb_cls(const uint8_t *__restrict__ clo, const uint8_t *__restrict__ clt, unsigned co,
+                                      unsigned no, int s, unsigned HX, unsigned HXY)
+{
+    Cls q;
+    q.c = clo[co];
+    q.xm = clt[no - 1 + s];
+    q.xp = clt[no + s];
+    q.ym = clt[no - HX];
+    q.yp = clt[no + HX];
+    q.zm = clt[no - HXY];
+    q.zp = clt[no + HXY];
+    return q;
+}
+
+// The conductances of cell (i,j,k) from its classes and the staged tables;
+// the same values conductances() forms in h3d_kernels.hip.
 template <typename T>
-__device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, const uint8_t *__restrict__ clo,
-                                           const uint8_t *__restrict__ clt, const T *__restrict__ ap, int i, int j,
-                                           int k, unsigned co, unsigned no, int s, unsigned HX, unsigned HXY)
+__device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, const Cls &q, int i, int j, int k)
 {
     const int K = g.K, n2 = 2 * K * K;
-    const int b = clo[co];
+    const int b = q.c;
     const bool ixm = i > 1, ixp = i < g.NX, iym = j > 1, iyp = j < g.NY;
     // stab: XT, YT (plane k) and the two z faces, staged by rb_stage
     Cond<T> o;
-    o.xm = stab[((ixm ? K : 0) + clt[no - 1 + s]) * K + b];
-    o.xp = stab[((ixp ? K : 0) + clt[no + s]) * K + b];
-    o.ym = stab[n2 + ((iym ? K : 0) + clt[no - HX]) * K + b];
-    o.yp = stab[n2 + ((iyp ? K : 0) + clt[no + HX]) * K + b];
-    o.zm = stab[2 * n2 + clt[no - HXY] * K + b];
-    o.zp = stab[2 * n2 + K * K + clt[no + HXY] * K + b];
+    o.xm = stab[((ixm ? K : 0) + q.xm) * K + b];
+    o.xp = stab[((ixp ? K : 0) + q.xp) * K + b];
+    o.ym = stab[n2 + ((iym ? K : 0) + q.ym) * K + b];
+    o.yp = stab[n2 + ((iyp ? K : 0) + q.yp) * K + b];
+    o.zm = stab[2 * n2 + q.zm * K + b];
+    o.zp = stab[2 * n2 + K * K + q.zp * K + b];
     // convective terms: exact zeros away from the boundary (skipped there)
     o.cxm = o.cxp = o.cym = o.cyp = o.czm = o.czp = T(0);
     if (!(ixm && ixp && iym && iyp && k > 1 && k < g.NZ)) {
@@ -178,7 +203,7 @@ __device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, con
         o.czm = k > 1 ? T(0) : g.HC[4] * g.dx0 * g.dy0 * T(1);
         o.czp = k < g.NZ ? T(0) : g.HC[5] * g.dx0 * g.dy0 * T(1);
     }
-    o.ap0 = ap ? ldb(ap, co * (unsigned)sizeof(T)) : T(0);
+    o.ap0 = g.ap ? stab[3 * n2 + b] : T(0);
     return o;
 }
 
@@ -275,49 +300,101 @@ static Halves<T> halves(T *a, size_t HS)
 // ---------------------------------------------------------------------------
 // red-black Gauss-Seidel
 // ---------------------------------------------------------------------------
+// Workgroup decode for kernels whose threads handle R rows each (rows j0,
+// j0 + BY, ..., j0 + (R-1)·BY of one plane): gx x ceil(NY / (BY·R)) x NZ
+// workgroups rounded up to a multiple of 8, renumbered per XCD as in
+// rb_thread.  live = the workgroup and m are real; rows are checked per row.
+struct RbRows {
+    int m, j0, k;
+    bool live;
+};
+
+template <int R, typename T>
+__device__ __forceinline__ RbRows rb_rows(const RbGeo<T> &g)
+{
+    const unsigned gy = ((unsigned)g.NY + BY * R - 1) / (BY * R);
+    const unsigned gz = (unsigned)g.NZ;
+    const unsigned nreal = (unsigned)g.gx * gy * gz;
+    const unsigned nb = (nreal + 7u) & ~7u;
+    const unsigned p = blockIdx.x;
+    const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
+    const unsigned bx = L % (unsigned)g.gx, t = L / (unsigned)g.gx;
+    const unsigned bz = t % gz, by = t / gz;
+    const int m = (int)bx * BX + threadIdx.x;
+    RbRows r;
+    r.live = L < nreal && m < g.HX;
+    r.m = min(m, g.HX - 1);
+    r.j0 = (int)by * BY * R + threadIdx.y + 1;
+    r.k = (int)bz + 1;
+    return r;
+}
+
+template <int R, typename T>
+static unsigned rb_rows_nb(const RbGeo<T> &g)
+{
+    const unsigned gy = ((unsigned)g.NY + BY * R - 1) / (BY * R);
+    return ((unsigned)g.gx * gy * (unsigned)g.NZ + 7u) & ~7u;
+}
+
 // One colour of rbsor_core! (src/NonUniform.jl:848-915), ω = 1, in place:
 // colour c updates half h = 1-c.  The kernel gets the two halves of x as
 // separate (non-aliasing) pointers: xo = the half it updates, xt = the half
 // it reads the neighbours from; bo = b's half h; clo/clt = the class
-// field's halves; ap = ap0's half h (or nullptr).  Same arithmetic as
-// gs_update().
-template <typename T>
+// field's halves.  Same arithmetic as gs_update().  Each thread updates one cell in each of R rows; all their
+// loads are issued before the table staging.
+template <typename T, int R>
 __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *__restrict__ xt,
                                                   const T *__restrict__ bo, const uint8_t *__restrict__ clo,
-                                                  const uint8_t *__restrict__ clt, const T *__restrict__ ap, RbGeo<T> g,
-                                                  int h, const KState *st)
+                                                  const uint8_t *__restrict__ clt, RbGeo<T> g, int h, const KState *st)
 {
     if (halted(st)) return;
-    const RbThread t = rb_thread(g);
-    const T *stab = rb_stage(g, t.k);
+    const RbRows t = rb_rows<R>(g);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
-    const int s = (h + t.j + t.k) & 1;
-    const int i = 2 * t.m + s;
-    const bool valid = t.live && i >= 1 && i <= g.NX;
-    const unsigned r = (unsigned)t.m + HX * (unsigned)t.j + HXY * (unsigned)t.k;
-    const unsigned rb = r * (unsigned)sizeof(T);
-    const Nb<T> n = rb_nb(xt, r, s, HX, HXY);
-    const T pp = ldb(xo, rb);
-    const T bc = ldb(bo, rb);
-    const Cond<T> o = rb_cond(g, stab, clo, clt, ap, i, t.j, t.k, r, r, s, HX, HXY);
-    const T dd = diag(o);
-    T xn;
-    if (g.flux) {
-        xn = pp + (rb_flux(o, n, pp) - bc) / dd;
-    } else {
-        const T dp = (((rb_offdiag(o, n) - bc) / dd - pp)) * T(1);
-        xn = pp + T(1) * dp;
+    Nb<T> n[R];
+    Cls q[R];
+    T pp[R], bc[R];
+    int s[R], jj[R];
+    unsigned rr[R];
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
+        jj[a] = min(t.j0 + a * BY, g.NY);
+        s[a] = (h + jj[a] + t.k) & 1;
+        rr[a] = (unsigned)t.m + HX * (unsigned)jj[a] + HXY * (unsigned)t.k;
+        const unsigned rb = rr[a] * (unsigned)sizeof(T);
+        n[a] = rb_nb(xt, rr[a], s[a], HX, HXY);
+        pp[a] = ldb(xo, rb);
+        bc[a] = ldb(bo, rb);
+        q[a] = rb_cls(clo, clt, rr[a], rr[a], s[a], HX, HXY);
     }
-    if (valid) stb(xo, rb, xn);
+    const T *stab = rb_stage(g, t.k);
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
+        const int i = 2 * t.m + s[a];
+        const bool valid = t.live && t.j0 + a * BY <= g.NY && i >= 1 && i <= g.NX;
+        const Cond<T> o = rb_cond(g, stab, q[a], i, jj[a], t.k);
+        const T dd = diag(o);
+        T xn;
+        if (g.flux) {
+            xn = pp[a] + (rb_flux(o, n[a], pp[a]) - bc[a]) / dd;
+        } else {
+            const T dp = (((rb_offdiag(o, n[a]) - bc[a]) / dd - pp[a])) * T(1);
+            xn = pp[a] + T(1) * dp;
+        }
+        if (valid) stb(xo, rr[a] * (unsigned)sizeof(T), xn);
+    }
 }
 
+// Rows per thread of the sweep: two for FP32 (measured faster at
+// 480x480x60 and 960x960x120), one for FP64 (two rows were slower at
+// 240x240x30 and equal at the larger grids).
 template <typename T>
 hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s)
 {
+    constexpr int R = sizeof(T) == 4 ? 2 : 1;
     const int h = 1 - color;
     const size_t own = h ? g.HS : 0, oth = h ? 0 : g.HS;
-    hipLaunchKernelGGL(k_rb_sweep<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, x + own, x + oth, b + own,
-                       g.cls + own, g.cls + oth, g.ap0 ? g.ap0 + own : nullptr, g, h, st);
+    hipLaunchKernelGGL((k_rb_sweep<T, R>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own, x + oth,
+                       b + own, g.cls + own, g.cls + oth, g, h, st);
     return hipGetLastError();
 }
 
@@ -330,12 +407,10 @@ __global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x0, 
                                                   T *__restrict__ b0, T *__restrict__ b1, const T *__restrict__ r0,
                                                   const T *__restrict__ r1, const T *__restrict__ q0,
                                                   const T *__restrict__ q1, const uint8_t *__restrict__ c0,
-                                                  const uint8_t *__restrict__ c1, const T *__restrict__ ap1,
-                                                  RbGeo<T> g, const KState *st)
+                                                  const uint8_t *__restrict__ c1, RbGeo<T> g, const KState *st)
 {
     if (halted(st)) return;
     const RbThread t = rb_thread(g);
-    const T *stab = rb_stage(g, t.k);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
     T beta = T(0), omg = T(0), r_alpha = T(0);
     bool first_itr = false;
@@ -365,7 +440,9 @@ __global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x0, 
     // the half-1 cell (colour 0) of the pair
     const int a1 = h[0] == 1 ? 0 : 1;
     const RbCell &cr = c[a1];
-    const Cond<T> cd = rb_cond(g, stab, c1, c0, ap1, cr.i, t.j, t.k, o, o, cr.s, HX, HXY);
+    const Cls q = rb_cls(c1, c0, o, o, cr.s, HX, HXY);
+    const T *stab = rb_stage(g, t.k);
+    const Cond<T> cd = rb_cond(g, stab, q, cr.i, t.j, t.k);
     const T pp = T(0);
     const T dd = diag(cd);
     T xn;
@@ -393,7 +470,7 @@ hipError_t launch_rb_first(int mode, T *x, T *bb, const T *r, const T *q, const 
     // q is not read in FIRST_COPY mode (nullptr); r neither
     const T *r1 = r ? r + HS : nullptr, *q1 = q ? q + HS : nullptr;
     hipLaunchKernelGGL(k_rb_first<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, mode, x, x + HS, bb, bb + HS, r, r1, q,
-                       q1, g.cls, g.cls + HS, g.ap0 ? g.ap0 + HS : nullptr, g, st);
+                       q1, g.cls, g.cls + HS, g, st);
     return hipGetLastError();
 }
 
@@ -438,22 +515,29 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T 
 {
     if (halted(st)) return;
     const RbThread t = rb_thread(g);
-    const T *stab = rb_stage(g, t.k);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY, HS = (unsigned)g.HS;
     RbCell c[2];
     int h[2];
     rb_pair(g, t, c, h);
     const unsigned sz = (unsigned)sizeof(T);
-    T av[2], dyv[2], dzv[2];
+    Nb<T> n[2];
+    Cls q[2];
+    T xv[2], av[2], dyv[2], dzv[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
         const unsigned co = (h[a] ? HS : 0u) + c[a].co, no = (h[a] ? 0u : HS) + c[a].co;
-        const Nb<T> n = rb_nb(x, no, c[a].s, HX, HXY);
-        const T xv = ldb(x, co * sz);
-        const Cond<T> cd = rb_cond(g, stab, g.cls, g.cls, g.ap0, c[a].i, t.j, t.k, co, no, c[a].s, HX, HXY);
-        av[a] = rb_apply(g, cd, n, xv);
-        dyv[a] = (NS >= 1 && dy) ? ldb(dy, co * sz) : av[a];
+        n[a] = rb_nb(x, no, c[a].s, HX, HXY);
+        xv[a] = ldb(x, co * sz);
+        q[a] = rb_cls(g.cls, g.cls, co, no, c[a].s, HX, HXY);
+        dyv[a] = (NS >= 1 && dy) ? ldb(dy, co * sz) : T(0);
         dzv[a] = NS >= 2 ? ldb(dz, co * sz) : T(0);
+    }
+    const T *stab = rb_stage(g, t.k);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const Cond<T> cd = rb_cond(g, stab, q[a], c[a].i, t.j, t.k);
+        av[a] = rb_apply(g, cd, n[a], xv[a]);
+        if (!(NS >= 1 && dy)) dyv[a] = av[a];
     }
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
@@ -495,7 +579,6 @@ __global__ void __launch_bounds__(256) k_rb_calc_rk(T *__restrict__ r, T *__rest
 {
     if (halted(st)) return;
     const RbThread t = rb_thread(g);
-    const T *stab = rb_stage(g, t.k);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
     const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
     RbCell c[2];
@@ -504,25 +587,30 @@ __global__ void __launch_bounds__(256) k_rb_calc_rk(T *__restrict__ r, T *__rest
     const unsigned o = c[0].co;
     const uint8_t *cl = g.cls;
     const size_t HS = g.HS;
-    T rs[2];
+    Nb<T> n[2];
+    Cls q[2];
+    T x0[2], bv[2], rs[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
         // standard-layout offset of the cell (i is at most MX-1 for a loadable m)
         const size_t cs = (size_t)min(c[a].i, g.MX - 2 + a) + sy * (size_t)t.j + sz * (size_t)t.k;
-        Nb<T> n;
-        n.xm = th[cs - 1];
-        n.xp = th[cs + 1];
-        n.ym = th[cs - sy];
-        n.yp = th[cs + sy];
-        n.zm = th[cs - sz];
-        n.zp = th[cs + sz];
-        const T x0 = th[cs];
-        const T bv = b[cs];
+        n[a].xm = th[cs - 1];
+        n[a].xp = th[cs + 1];
+        n[a].ym = th[cs - sy];
+        n[a].yp = th[cs + sy];
+        n[a].zm = th[cs - sz];
+        n[a].zp = th[cs + sz];
+        x0[a] = th[cs];
+        bv[a] = b[cs];
         const int hh = h[a];
-        const Cond<T> cd = rb_cond(g, stab, cl + (hh ? HS : 0), cl + (hh ? 0 : HS), g.ap0 ? g.ap0 + (hh ? HS : 0) : nullptr,
-                                   c[a].i, t.j, t.k, o, o, c[a].s, HX, HXY);
-        if (g.flux) rs[a] = bv - rb_flux(cd, n, x0);
-        else rs[a] = (bv - (rb_offdiag(cd, n) - diag(cd) * x0)) * T(1);
+        q[a] = rb_cls(cl + (hh ? HS : 0), cl + (hh ? 0 : HS), o, o, c[a].s, HX, HXY);
+    }
+    const T *stab = rb_stage(g, t.k);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const Cond<T> cd = rb_cond(g, stab, q[a], c[a].i, t.j, t.k);
+        if (g.flux) rs[a] = bv[a] - rb_flux(cd, n[a], x0[a]);
+        else rs[a] = (bv[a] - (rb_offdiag(cd, n[a]) - diag(cd) * x0[a])) * T(1);
     }
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll

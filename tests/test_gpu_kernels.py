@@ -34,22 +34,26 @@ def mixed_bc():
 
 def make_pair(oracle, NX, NY, NZ, seed, bc_set, random_lam=True, steady=True, dt=0.25, form=L.COEF_AUTO):
     """Identical random inputs on the device and in the oracle.  random_lam:
-    True = a different λ in every cell (the direct form), "classes" = λ
-    drawn from 9 random values (the table form), False = modelA's λ."""
+    True = a different λ, ρ and cp in every cell (the direct form),
+    "classes" = (λ, ρ, cp) drawn from 9 random materials (the table form),
+    False = modelA's λ, ρ and cp (the table form)."""
     rng = np.random.default_rng(seed)
     MX, MY, MZ = NX + 2, NY + 2, NZ + 2
     shape = (MX, MY, MZ)
     dx, dy = h2_amd.pitch(NX), h2_amd.pitch(NY)
     Z, ZC, dZ = h2_amd.genZ(NZ)
     if random_lam == "classes":
-        lam = rng.choice(rng.uniform(0.3, 400.0, 9), size=shape)
+        mat = rng.integers(0, 9, size=shape)
+        lam = rng.uniform(0.3, 400.0, 9)[mat]
+        rho = rng.uniform(1000.0, 9000.0, 9)[mat]
+        cp = rng.uniform(300.0, 1100.0, 9)[mat]
     elif random_lam:
         lam = rng.uniform(0.3, 400.0, shape)
+        rho = rng.uniform(1000.0, 9000.0, shape)
+        cp = rng.uniform(300.0, 1100.0, shape)
     else:
         m = h2_amd.build_model(NX, NY, NZ)
-        lam = m.lam
-    rho = rng.uniform(1000.0, 9000.0, shape)
-    cp = rng.uniform(300.0, 1100.0, shape)
+        lam, rho, cp = m.lam, m.rho, m.cp
     th = rng.uniform(290.0, 350.0, shape)
     wk = WorkBuffers(*shape)
     wk.set_coef_form(form)
@@ -118,7 +122,7 @@ def test_calc_ax_bitwise(oracle, size, random_lam, steady, form):
     wk.calc_ax("pcg_q", "pcg_p_", c.grid.dt, steady)
     c.calc_ax("q", "p_")
     assert_bitwise(wk.download("pcg_q"), c.arr["q"], "ax")
-    # the table form is in use exactly when λ has few distinct values
+    # the table form is in use exactly when (λ, ρ·cp) has few distinct pairs
     expect_table = form == L.COEF_AUTO and random_lam is not True
     assert (wk.num_classes() > 0) == expect_table
     wk.close()
