@@ -19,6 +19,9 @@ Printed JSON line (one, on stdout):
              algorithm) on a bounded sample of the same workload
   fp32_refined  the same case solved to a true relative residual of 1e-8
              with FP64 and with FP32-inner/FP64-refinement, and the speed-up
+  c3         480 x 480 x 60 with the seeded power map: the same two solves
+             to 1e-8 and their maximum temperature difference (K) from an
+             FP64 solve to 1e-12
 """
 import argparse
 import json
@@ -45,6 +48,7 @@ def parse():
     ap.add_argument("--eps", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-refined", action="store_true", help="skip the FP32-refinement comparison")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 480x480x60 precision comparison")
     ap.add_argument("--cpu-iters", type=int, default=200)
     return ap.parse_args()
 
@@ -80,6 +84,35 @@ def cpu_baseline(nx, nz, iters):
             "sample": "oracle (OpenMP C restatement of the reference algorithm) PBiCGSTAB+GS FP64 at %dx%dx%d, "
                       "first %d iterations, %.1f s solve (%.1f s wall incl. geometry)" % (nx, nx, nz, itr, solve_s, wall),
             "s_per_iteration": round(solve_s / itr, 4)}
+
+
+def precision_c3(np, h2_amd, L, WorkBuffers, load_model):
+    """480x480x60, seeded power map: FP64 and FP32-refined solves to 1e-8
+    against an FP64 solve to 1e-12."""
+    log("building model 480x480x60 (power map)")
+    m = h2_amd.build_model(480, 480, 60, power="map")
+    wk = WorkBuffers(*m.shape)
+    load_model(wk, m)
+    th0 = wk.download("theta")
+
+    def run(prec, tol):
+        wk.upload("theta", th0)
+        wk.fill("b", 0.0)
+        t0 = time.perf_counter()
+        wk.calc_rhs(10000.0, True)
+        conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, tol, 20000, L.SMOOTHER_GS, True, precision=prec)
+        return time.perf_counter() - t0, conv, itr, wk.download("theta")
+
+    _, _, itr_ref, ref = run(L.FP64, 1e-12)
+    out = {"tol": 1e-8, "reference": "FP64 to 1e-12 (%d iterations)" % itr_ref}
+    for name, prec in (("fp64", L.FP64), ("fp32_refined", L.FP32_REFINED)):
+        t, conv, itr, th = run(prec, 1e-8)
+        out[name] = {"ms": round(t * 1e3, 2), "itr": itr, "converged": conv,
+                     "max_abs_err_K": float("%.3e" % np.max(np.abs(th - ref)))}
+        log("c3 %s: %.1f ms, itr %d, converged %s" % (name, t * 1e3, itr, conv))
+    out["speedup_fp32_refined"] = round(out["fp64"]["ms"] / out["fp32_refined"]["ms"], 3)
+    wk.close()
+    return out
 
 
 def main():
@@ -155,6 +188,10 @@ def main():
             refined["fp32_refined_error"] = str(e)
     wk.close()
 
+    c3 = None
+    if not args.no_c3:
+        c3 = precision_c3(np, h2_amd, L, WorkBuffers, load_model)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -169,7 +206,7 @@ def main():
             "config": {"workload": "modelA %dx%dx%d steady PBiCGSTAB+GS(5) eps=%g, calRHS!+solve"
                                    % (nx, nx, nz, args.eps), "cells": cells, "iterations": itr,
                        "device_solve_ms": round(sum(dev_ms) / len(dev_ms), 3)},
-            "roofline": roofline, "cpu_baseline": cpu, "fp32_refined": refined}
+            "roofline": roofline, "cpu_baseline": cpu, "fp32_refined": refined, "c3": c3}
     print(json.dumps(line), flush=True)
 
 
