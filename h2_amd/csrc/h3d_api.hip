@@ -34,6 +34,8 @@ struct h3d_ctx {
     double *f[H3D_NFIELDS] = {};  // f[H3D_F_MASK] stays null: the mask is implicit
     double *d_ZC = nullptr, *d_dZ = nullptr, *d_ap0 = nullptr;
     float *d_ZC32 = nullptr, *d_dZ32 = nullptr, *d_ap032 = nullptr;
+    float *d_DZC32 = nullptr;             // FP32 z-face distances (Geo::DZC)
+    unsigned long dzc32_grid = 0;         // grid version d_DZC32 was formed for
     double dx = 0, dy = 0;
     bool grid_set = false;
     int bc_type[6] = {H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX, H3D_HEAT_FLUX};
@@ -547,7 +549,7 @@ int ensure_rb(h3d_ctx *c)
     for (int kf = 0; kf <= NZ; ++kf) {
         const int mb = (kf == 0 || kf == NZ) ? 0 : K;
         const double dzc = c->hZC[(size_t)kf + 1] - c->hZC[(size_t)kf];
-        const float dzc32 = (float)c->hZC[(size_t)kf + 1] - (float)c->hZC[(size_t)kf];
+        const float dzc32 = (float)dzc;
         for (int a = 0; a < K; ++a)
             for (int b = 0; b < K; ++b) {
                 const size_t o = ((size_t)kf * K + a) * K + b;
@@ -734,7 +736,15 @@ int make_geo32(h3d_ctx *c, int is_steady, Geo<float> *out)
     HIPCHK(c, launch_convert<float, double>(c->d_ZC32, c->d_ZC, (size_t)c->MZ, s));
     HIPCHK(c, launch_convert<float, double>(c->d_dZ32, c->d_dZ, (size_t)c->MZ, s));
     Geo<float> g32 = base_geo<float>(c);
-    g32.ZC = c->d_ZC32; g32.dZ = c->d_dZ32; g32.ap0 = nullptr;
+    if (!c->d_DZC32) HIPCHK(c, hipMalloc(&c->d_DZC32, (size_t)c->MZ * sizeof(float)));
+    if (c->dzc32_grid != c->grid_version) {
+        std::vector<float> dzc((size_t)c->MZ, 0.f);
+        for (int k = 0; k + 1 < c->MZ; ++k) dzc[(size_t)k] = (float)(c->hZC[(size_t)k + 1] - c->hZC[(size_t)k]);
+        HIPCHK(c, hipStreamSynchronize(s));
+        HIPCHK(c, hipMemcpy(c->d_DZC32, dzc.data(), dzc.size() * sizeof(float), hipMemcpyHostToDevice));
+        c->dzc32_grid = c->grid_version;
+    }
+    g32.ZC = c->d_ZC32; g32.DZC = c->d_DZC32; g32.dZ = c->d_dZ32; g32.ap0 = nullptr;
     g32.lam = c->g32[F32_LAM];
     if (c->K > 0) {
         g32.cls = c->d_cls;
@@ -940,7 +950,7 @@ int h3d_destroy(h3d_ctx *c)
         if (p) (void)hipFree(p);
     for (int a = 0; a < 2; ++a)
         if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
-    void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_state,
+    void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_DZC32, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
                   c->d_corr, c->d_best};
     for (void *p : ps)

@@ -88,6 +88,11 @@ struct Geo {
     T dx0, dy0, ddx, ddy;
     T HC[6];
     const T *ZC;            // device, MZ entries
+    // device, MZ-1 z-face distances ZC[k+1] - ZC[k] formed in FP64 and
+    // rounded to T (the FP32 operator: the difference of two rounded ZC
+    // would carry their absolute rounding error); nullptr for FP64, which
+    // forms the differences of ZC as the reference does
+    const T *DZC;
     const T *dZ;            // device, MZ entries
     const T *ap0;           // device, per-cell ρ·cp·dx·dy·dz/Δt (transient); nullptr when steady
     const T *lam;           // device λ field (direct form)
@@ -112,7 +117,8 @@ constexpr int kMaxClasses = 32;
 //   ZF[kf][a][b] = ((λf(v_a, v_b, m) * dx0) * dy0) / (ZC[kf+1] - ZC[kf]),
 // for the face between planes kf and kf+1 (m = 0 on the two boundary faces,
 // 1 inside), which is the value CalcAX! forms for the zm term of plane
-// kf+1 and the zp term of plane kf (λf is symmetric in its two values).
+// kf+1 and the zp term of plane kf (λf is symmetric in its two values);
+// in FP32 the distance is the FP64 difference rounded (Geo::DZC).
 template <typename T>
 struct RbGeo {
     int MX, MY, MZ, NX, NY, NZ;

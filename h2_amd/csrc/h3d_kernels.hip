@@ -61,8 +61,10 @@ __device__ __forceinline__ Cond<T> conductances(const Geo<T> &g, const T *stab, 
     o.xp = fxp * g.dy0 * dz_k * g.ddx;
     o.ym = fym * g.dx0 * dz_k * g.ddy;
     o.yp = fyp * g.dx0 * dz_k * g.ddy;
-    o.zm = fzm * g.dx0 * g.dy0 / (g.ZC[k] - g.ZC[k - 1]);
-    o.zp = fzp * g.dx0 * g.dy0 / (g.ZC[k + 1] - g.ZC[k]);
+    const T dzm = g.DZC ? g.DZC[k - 1] : g.ZC[k] - g.ZC[k - 1];
+    const T dzp = g.DZC ? g.DZC[k] : g.ZC[k + 1] - g.ZC[k];
+    o.zm = fzm * g.dx0 * g.dy0 / dzm;
+    o.zp = fzp * g.dx0 * g.dy0 / dzp;
     o.cxm = g.HC[0] * g.dy0 * dz_k * (ixm ? T(0) : T(1));
     o.cxp = g.HC[1] * g.dy0 * dz_k * (ixp ? T(0) : T(1));
     o.cym = g.HC[2] * g.dx0 * dz_k * (iym ? T(0) : T(1));
