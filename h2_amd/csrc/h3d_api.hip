@@ -52,6 +52,8 @@ struct h3d_ctx {
     // conductivity classes (table form of the operator)
     int coef_form = H3D_COEF_AUTO;
     unsigned long lam_version = 1, cls_version = 0;
+    bool cls_cap = false;                 // classes keyed by ρ·cp too (transient operators)
+    unsigned long cls_build = 0;          // bumped by every (re)classification
     int K = 0;                   // 0: table form unavailable for the current λ
     uint8_t *d_cls = nullptr;
     uint64_t *d_vals = nullptr;          // class keys (λ bits, ρ·cp bits), sorted
@@ -67,7 +69,7 @@ struct h3d_ctx {
     int layout = H3D_LAYOUT_AUTO;
     int op_form = H3D_FORM_REFERENCE;     // H3D_OPT_OPERATOR_FORM of the FP64 operator
     unsigned long grid_version = 1;       // bumped by h3d_set_grid / h3d_set_bc
-    unsigned long rb_lam = 0, rb_grid = 0; // versions the RB tables were built for
+    unsigned long rb_lam = 0, rb_grid = 0; // class build and grid version the RB tables were built for
     std::vector<double> hZC, hdZ;         // host copies of the Z metrics
     std::vector<double> h_tab64;          // host copies of the λf tables
     std::vector<float> h_tab32;
@@ -163,25 +165,31 @@ T lamf_host(T a, T b, int m)
 // (λ, ρ·cp) bit-pattern pairs (at most kMaxClasses), every cell's index
 // among them, the tables tab[m][a][b] = λf(λ_a, λ_b, m, 1) in FP64 and
 // FP32, and the heat capacities ρ·cp per class (the colour-split kernels
-// form ap0 from them per plane).  Rebuilt when λ, ρ or cp changed since the
-// last build; K = 0 when there are too many pairs.
-int ensure_classes(h3d_ctx *c)
+// form ap0 from them per plane).  A steady operator needs no ρ·cp (cap
+// false): then the classes are the distinct λ alone, fewer and smaller
+// tables.  Rebuilt when λ, ρ or cp changed since the last build or cap
+// differs; K = 0 when there are too many classes.
+int ensure_classes(h3d_ctx *c, bool cap)
 {
-    if (c->cls_version == c->lam_version) return H3D_OK;
+    if (c->cls_version == c->lam_version && c->cls_cap == cap) return H3D_OK;
     c->K = 0;
     c->cls_version = c->lam_version;
+    c->cls_cap = cap;
+    c->cls_build++;
     if (c->coef_form == H3D_COEF_DIRECT) return H3D_OK;
-    std::vector<double> lam(c->H), rho(c->H), cp(c->H);
+    std::vector<double> lam(c->H), rho(cap ? c->H : 0), cp(cap ? c->H : 0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(lam.data(), c->f[H3D_F_LAMBDA], c->H * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(rho.data(), c->f[H3D_F_RHO], c->H * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(cp.data(), c->f[H3D_F_CP], c->H * sizeof(double), hipMemcpyDeviceToHost));
+    if (cap) {
+        HIPCHK(c, hipMemcpy(rho.data(), c->f[H3D_F_RHO], c->H * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(cp.data(), c->f[H3D_F_CP], c->H * sizeof(double), hipMemcpyDeviceToHost));
+    }
     using Key = std::pair<uint64_t, uint64_t>;
     std::vector<Key> vals;
     Key last{0, 0};
     bool have_last = false;
     for (size_t a = 0; a < c->H; ++a) {
-        const double pc = rho[a] * cp[a];
+        const double pc = cap ? rho[a] * cp[a] : 0.0;
         Key v;
         std::memcpy(&v.first, &lam[a], 8);
         std::memcpy(&v.second, &pc, 8);
@@ -223,8 +231,8 @@ int ensure_classes(h3d_ctx *c)
     HIPCHK(c, hipMemcpy(c->d_tab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
     c->h_tab64 = t64;
     c->h_tab32 = t32;
-    HIPCHK(c, launch_classify(c->d_cls, c->f[H3D_F_LAMBDA], c->f[H3D_F_RHO], c->f[H3D_F_CP], c->d_vals, K, c->H,
-                              c->stream));
+    HIPCHK(c, launch_classify(c->d_cls, c->f[H3D_F_LAMBDA], cap ? c->f[H3D_F_RHO] : nullptr,
+                              cap ? c->f[H3D_F_CP] : nullptr, c->d_vals, K, c->H, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->K = K;
     return H3D_OK;
@@ -234,7 +242,7 @@ int ensure_classes(h3d_ctx *c)
 int make_geo64(h3d_ctx *c, double dt, int is_steady, Geo<double> *out)
 {
     if (!c->grid_set) return fail(c, H3D_ERR_STATE, "h3d_set_grid has not been called");
-    int rc = ensure_classes(c);
+    int rc = ensure_classes(c, !is_steady);
     if (rc != H3D_OK) return rc;
     Geo<double> g = base_geo<double>(c);
     g.ZC = c->d_ZC; g.dZ = c->d_dZ;
@@ -443,7 +451,7 @@ int iteration_graph(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P,
                     hipGraphExec_t *out)
 {
     h3d_ctx::GraphKey key{solver, smoother, (int)sizeof(T), P.g.ap0 != nullptr, P.rb ? 1 : 0,
-                          c->lam_version,   c->grid_version, c->d_hist, c->rb.partials, v.x};
+                          c->cls_build,     c->grid_version, c->d_hist, c->rb.partials, v.x};
     for (auto &e : c->graphs)
         if (e.first == key) {
             *out = e.second;
@@ -531,7 +539,7 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
 int ensure_rb(h3d_ctx *c)
 {
     if (c->K <= 0) return H3D_ERR_UNSUPPORTED;
-    if (c->rb_lam == c->lam_version && c->rb_grid == c->grid_version && c->d_cls_rb) return H3D_OK;
+    if (c->rb_lam == c->cls_build && c->rb_grid == c->grid_version && c->d_cls_rb) return H3D_OK;
     hipStream_t s = c->stream;
     c->HX = (c->MX + 1) / 2;
     c->HS = (size_t)c->HX * c->MY * c->MZ;
@@ -563,7 +571,7 @@ int ensure_rb(h3d_ctx *c)
     HIPCHK(c, hipStreamSynchronize(s));
     HIPCHK(c, hipMemcpy(c->d_ZF64, z64.data(), nzf * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_ZF32, z32.data(), nzf * sizeof(float), hipMemcpyHostToDevice));
-    c->rb_lam = c->lam_version;
+    c->rb_lam = c->cls_build;
     c->rb_grid = c->grid_version;
     return H3D_OK;
 }

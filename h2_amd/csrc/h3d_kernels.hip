@@ -970,13 +970,14 @@ hipError_t launch_ap0(double *ap0, const double *rho, const double *cp, const Ge
 
 // class of a cell = index of its λ bit pattern in the sorted distinct list
 // Class index of every cell: the position of its (λ bits, ρ·cp bits) pair
-// among the K sorted pairs vals[2a], vals[2a+1] (every pair is present).
+// among the K sorted pairs vals[2a], vals[2a+1] (every pair is present);
+// with rho = nullptr the second key is 0 (classes by λ alone).
 __global__ void __launch_bounds__(256) k_classify(uint8_t *cls, const double *lam, const double *rho, const double *cp,
                                                   const uint64_t *vals, int K, size_t n)
 {
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
         const uint64_t v = (uint64_t)__double_as_longlong(lam[c]);
-        const uint64_t w = (uint64_t)__double_as_longlong(rho[c] * cp[c]);
+        const uint64_t w = rho ? (uint64_t)__double_as_longlong(rho[c] * cp[c]) : 0ull;
         int lo = 0, hi = K - 1;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
