@@ -537,12 +537,13 @@ def main(dh, dt, wk, ZC, dZ, model, solver, smoother, bc_set, par="hip", is_stea
 
 
 def q3d(NX, NY, NZ, solver="sor", smoother="", epsilon=1.0e-6, par="hip", is_steady=False, verbose=True,
-        precision=L.FP64, device=0, power="modelA", seed=20251203):
+        precision=L.FP64, device=0, power="modelA", seed=20251203, output_dir=None):
     """q3d() (src/heat3ds.jl:169-281) with par="hip": builds the modelA case
     on the host, runs calRHS! and the Krylov solve on the GPU and returns a
     Q3DResult.  solver "cg" runs CG!, anything else PBiCGSTAB! (as the
     reference's main() does); smoother "gs" selects the red-black
-    Gauss-Seidel preconditioner, anything else none."""
+    Gauss-Seidel preconditioner, anything else none.  With output_dir the
+    reference's text outputs are written there (write_outputs)."""
     _check_par(par)
     if verbose:
         print("Analysis mode: %s" % ("Steady-state" if is_steady else "Transient"))
@@ -553,8 +554,34 @@ def q3d(NX, NY, NZ, solver="sor", smoother="", epsilon=1.0e-6, par="hip", is_ste
         load_model(wk, m)
         if verbose:
             print("(%d, %d, %d)  Itr.ε= %g" % (m.shape + (epsilon,)))
-        return main((m.dx, m.dy, 1.0), dt, wk, m.ZC, m.dZ, m, solver, smoother, m.bc_set, par=par,
-                    is_steady=is_steady, itr_tol=epsilon, verbose=verbose, precision=precision,
-                    log=sys.stdout if verbose else None)
+        res = main((m.dx, m.dy, 1.0), dt, wk, m.ZC, m.dZ, m, solver, smoother, m.bc_set, par=par,
+                   is_steady=is_steady, itr_tol=epsilon, verbose=verbose, precision=precision,
+                   log=sys.stdout if verbose else None)
+        if output_dir is not None:
+            write_outputs(output_dir, res, m, solver, smoother, epsilon, dt)
+        return res
     finally:
         wk.close()
+
+
+def write_outputs(output_dir, res, m, solver, smoother, epsilon, dt):
+    """The text outputs of q3d (src/heat3ds.jl:119-277): log.txt (header,
+    residual lines, θ statistics), the convergence CSV of a Krylov solver
+    (convergence_<solver>_<NX>x<NY>x<NZ>[_<smoother>].csv) and the two
+    Z-line CSVs temp3Z_ctr.csv (x = y = 0.6 mm) and temp3Z_tsv.csv
+    (x = y = 0.4 mm).  The plots are not drawn."""
+    import os
+    from . import output as O
+    os.makedirs(output_dir, exist_ok=True)
+    dh = (m.dx, m.dy, 1.0)
+    log = os.path.join(output_dir, "log.txt")
+    O.write_log(log, O.conditions(m.shape, dh, solver, smoother, epsilon),
+                [(res.history, O.theta_stats_line(1, dt, res.theta))])
+    if solver in ("pbicgstab", "cg"):
+        name = "convergence_%s_%dx%dx%d" % (solver, m.NX, m.NY, m.NZ) + ("_%s" % smoother if smoother else "")
+        conv = O.ConvergenceData(solver, smoother)
+        O.parse_residuals_from_log(conv, log)
+        O.export_convergence_csv(conv, os.path.join(output_dir, name + ".csv"))
+    for xy, tag in ((0.6e-3, "ctr"), (0.4e-3, "tsv")):
+        z, d = O.line_z(res.theta, dh, m.Z, xy, xy)
+        O.export_zline_csv(z, d, os.path.join(output_dir, "temp3Z_%s" % tag))
