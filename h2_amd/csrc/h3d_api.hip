@@ -1333,6 +1333,116 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
 }
 
 // ---------------------------------------------------------------------------
+// stand-alone RB-SOR (resSOR, rbsor!, solveSOR!)
+// ---------------------------------------------------------------------------
+namespace {
+
+int sor_geo(h3d_ctx *c, double dt, int is_steady, double omega, Geo<double> *g)
+{
+    if (!(omega > 0.0) || !std::isfinite(omega)) return fail(c, H3D_ERR_ARG, "omega must be > 0");
+    int rc = make_geo64(c, dt, is_steady, g);
+    if (rc != H3D_OK) return rc;
+    g->flux = 0;  // the reference's form of the update and residual
+    return H3D_OK;
+}
+
+}  // namespace
+
+int h3d_res_sor(h3d_ctx *c, int x, int b, double omega, double dt, int is_steady, double *res)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(x) || !field_ok(b) || x == H3D_F_MASK || b == H3D_F_MASK || !res)
+        return fail(c, H3D_ERR_ARG, "bad field id or output");
+    Geo<double> g;
+    int rc = sor_geo(c, dt, is_steady, omega, &g);
+    if (rc != H3D_OK) return rc;
+    if ((rc = reset_state(c, 0.0, 1)) != H3D_OK) return rc;
+    HIPCHK(c, launch_res_sor(c->f[x], c->f[b], g, omega, FIN_STORE, c->d_state, nullptr, c->rb, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_state, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *res = std::sqrt(c->h_state[0].out[0]);
+    return H3D_OK;
+}
+
+int h3d_rbsor(h3d_ctx *c, int x, int b, double omega, double dt, int is_steady, double *res)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(x) || !field_ok(b) || x == H3D_F_MASK || b == H3D_F_MASK || x == b || !res)
+        return fail(c, H3D_ERR_ARG, "bad field ids or output");
+    Geo<double> g;
+    int rc = sor_geo(c, dt, is_steady, omega, &g);
+    if (rc != H3D_OK) return rc;
+    if ((rc = reset_state(c, 0.0, 1)) != H3D_OK) return rc;
+    double sums[2];
+    for (int color = 0; color < 2; ++color) {
+        HIPCHK(c, launch_sor_color(c->f[x], c->f[b], g, color, omega, FIN_STORE, c->d_state, nullptr, c->rb,
+                                   c->stream));
+        HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_state, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        sums[color] = c->h_state[0].out[0];
+    }
+    double r = 0.0;  // res::T = zero(T); res += each colour's sum
+    r += sums[0];
+    r += sums[1];
+    *res = std::sqrt(r);
+    return H3D_OK;
+}
+
+int h3d_rbsor_solve(h3d_ctx *c, double dt, double omega, double tol, int maxItr, int is_steady, int *isconverged,
+                    int *itr, double *res0)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (maxItr < 1) return fail(c, H3D_ERR_ARG, "maxItr must be >= 1");
+    Geo<double> g;
+    int rc = sor_geo(c, dt, is_steady, omega, &g);
+    if (rc != H3D_OK) return rc;
+    if ((rc = ensure_hist(c, maxItr)) != H3D_OK) return rc;
+    if ((rc = reset_state(c, tol, maxItr)) != H3D_OK) return rc;
+    hipStream_t s = c->stream;
+    double *th = c->f[H3D_F_THETA], *b = c->f[H3D_F_B];
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    HIPCHK(c, launch_res_sor(th, b, g, omega, FIN_SOR_RES0, c->d_state, nullptr, c->rb, s));
+    // one iteration = colour 0 (its sum kept) and colour 1 (the residual
+    // test), captured once and replayed in batches; the host reads the
+    // state once per batch, and replays after the stop are no-ops
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = launch_sor_color(th, b, g, 0, omega, FIN_STORE, c->d_state, c->d_hist, c->rb, s);
+    if (ce == hipSuccess) ce = launch_sor_color(th, b, g, 1, omega, FIN_SOR_RES, c->d_state, c->d_hist, c->rb, s);
+    hipError_t ee = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(c, H3D_ERR_HIP, std::string("capturing the RB-SOR iteration: ") +
+                                        hipGetErrorString(ce != hipSuccess ? ce : ee));
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(c, H3D_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    int launched = 0;
+    KState st{};
+    hipError_t le = hipSuccess;
+    while (le == hipSuccess) {
+        const int nb = std::min(16, maxItr - launched);
+        for (int it = 0; it < nb && le == hipSuccess; ++it) le = hipGraphLaunch(exec, s);
+        launched += nb;
+        if (le != hipSuccess) break;
+        le = hipMemcpyAsync(&c->h_state[0], c->d_state, sizeof(KState), hipMemcpyDeviceToHost, s);
+        if (le == hipSuccess) le = hipStreamSynchronize(s);
+        st = c->h_state[0];
+        if (st.status != ST_RUNNING || launched >= maxItr) break;
+    }
+    (void)hipGraphExecDestroy(exec);
+    if (le != hipSuccess) return fail(c, H3D_ERR_HIP, std::string("RB-SOR replay: ") + hipGetErrorString(le));
+    HIPCHK(c, hipEventRecord(c->ev1, s));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return finish_solve(c, st, isconverged, itr, res0);
+}
+
+// ---------------------------------------------------------------------------
 // sensitivity extension
 // ---------------------------------------------------------------------------
 int h3d_objective(h3d_ctx *c, const uint8_t *region, size_t n, double beta, double *J)

@@ -42,6 +42,8 @@ enum : int {
     FIN_CG_ALPHA,        // itr += 1; breakdown on s0 = (p,q); alpha = rho_old / s0
     FIN_CG_RES,          // res = sqrt(s0)/res0; history; tol
     FIN_CG_BETA,         // breakdown on s0 = rho_new; beta; rho_old = rho_new; maxItr
+    FIN_SOR_RES0,        // res0 = sqrt(s0), 1 when 0 (solveSOR!); itr = 0
+    FIN_SOR_RES,         // itr += 1; res = sqrt(out[0] + s0)/res0; history; tol / maxItr
 };
 
 struct KState {
@@ -238,6 +240,14 @@ hipError_t launch_scale_interior(double *w, const Geo<double> &g, double sc, hip
 // per-cell dJ/dλ and dJ/dhsrc from θ and the adjoint μ; iso: isothermal faces (bit f)
 hipError_t launch_gradients(const double *th, const double *mu, const Geo<double> &g, int iso, double *glam,
                             double *ghs, hipStream_t s);
+
+// Stand-alone RB-SOR (FP64, the reference's operator form): one colour of
+// rbsor_core! with relaxation omega and the sum of its residual squares,
+// and resSOR's sum over every cell; each followed by finalizer fin.
+hipError_t launch_sor_color(double *x, const double *b, const Geo<double> &g, int color, double omega, int fin,
+                            KState *st, double *hist, const RedBuf &rb, hipStream_t s);
+hipError_t launch_res_sor(const double *x, const double *b, const Geo<double> &g, double omega, int fin, KState *st,
+                          double *hist, const RedBuf &rb, hipStream_t s);
 
 // rhs = w·seed + D∘μ_next (μ_next may be nullptr): one step of the transient adjoint
 hipError_t launch_adjoint_rhs(double *rhs, const double *seed, const double *mu_next, const Geo<double> &g, double w,

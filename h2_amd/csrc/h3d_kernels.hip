@@ -235,6 +235,25 @@ __device__ void finalize(int fin, const double *s, KState *st, double *hist)
         else if (res < st->tol) st->status = ST_CONVERGED;
         break;
     }
+    case FIN_SOR_RES0: {  // solveSOR!, NonUniform.jl:1035-1038
+        double r0 = sqrt(s[0]);
+        if (r0 == 0.0) r0 = 1.0;
+        st->res0 = r0;
+        st->itr = 0;
+        if (!isfinite(r0)) st->status = ST_NAN;
+        break;
+    }
+    case FIN_SOR_RES: {  // rbsor!'s sqrt((0 + res_c0) + res_c1) / res0, NonUniform.jl:950-956, 1042-1049
+        st->itr += 1;
+        double res = sqrt(st->out[0] + s[0]);
+        res /= st->res0;
+        st->res = res;
+        if (hist) hist[st->itr - 1] = res;
+        if (!isfinite(res)) st->status = ST_NAN;
+        else if (res < st->tol) st->status = ST_CONVERGED;
+        else if (st->itr >= st->maxItr) st->status = ST_MAXITER;
+        break;
+    }
     case FIN_CG_BETA: {  // NonUniform.jl:205-222
         const double rho_new = s[0];
         if (fabs(rho_new) < kFloatMin) {
@@ -554,6 +573,87 @@ hipError_t launch_rbsor(T *x, const T *b, const Geo<T> &g, int color, const KSta
     else
         hipLaunchKernelGGL((k_rbsor<T, false>), sh.grid, sh.block, 0, s, x, b, g, color, st, sh.KC);
     return hipGetLastError();
+}
+
+// rbsor_core! (src/NonUniform.jl:848-915) with relaxation ω and the residual
+// it sums: per updated cell dp = ((ss - b)/dd - pp)·m0, x = pp + ω·dp,
+// r = (dd + ω(cond_xm + cond_ym + cond_zm))·dp/ω, Σ r² exactly rounded.
+// FP64 in the reference's form (not the flux form).
+template <bool CLS>
+__global__ void __launch_bounds__(256) k_sor_color(double *x, const double *b, Geo<double> g, int color,
+                                                   double omega, KState *st, RedBuf rb, int KC)
+{
+    if (halted(st)) return;
+    const double *stab = stage_table<double, CLS>(g);
+    const int t = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int kb = 1 + (int)blockIdx.z * KC;
+    const int ke = min(kb + KC, g.NZ + 1);
+    const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    if (j <= g.NY) {
+        for (int k = kb; k < ke; ++k) {
+            const int i = (((j + k + color) & 1) ? 2 : 1) + 2 * t;
+            if (i > g.NX) continue;
+            const size_t c = (size_t)i + sy * (size_t)j + sz * (size_t)k;
+            const Cond<double> o = conductances<double, CLS>(g, stab, i, j, k, c);
+            const double pp = x[c];
+            const double dg = diag(o);
+            const double dp = (((offdiag(o, x, c, sy, sz) - b[c]) / dg - pp)) * 1.0;
+            x[c] = pp + omega * dp;
+            const double r = (dg + omega * (o.xm + o.ym + o.zm)) * dp / omega;
+            v[0] = dd_add(v[0], r * r);
+        }
+    }
+    block_partials<1>(v, rb);
+}
+
+hipError_t launch_sor_color(double *x, const double *b, const Geo<double> &g, int color, double omega, int fin,
+                            KState *st, double *hist, const RedBuf &rb, hipStream_t s)
+{
+    Shape sh = interior_shape((g.NX + 1) / 2, g.NY, g.NZ);
+    if (g.cls)
+        hipLaunchKernelGGL((k_sor_color<true>), sh.grid, sh.block, table_bytes(g), s, x, b, g, color, omega, st, rb,
+                           sh.KC);
+    else
+        hipLaunchKernelGGL((k_sor_color<false>), sh.grid, sh.block, 0, s, x, b, g, color, omega, st, rb, sh.KC);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
+}
+
+// resSOR (src/NonUniform.jl:680-744): the residual of rbsor_core! over every
+// interior cell at the current x, without updating it.
+template <bool CLS>
+__global__ void __launch_bounds__(256) k_res_sor(const double *x, const double *b, Geo<double> g, double omega,
+                                                 KState *st, RedBuf rb, int KC)
+{
+    if (halted(st)) return;
+    const double *stab = stage_table<double, CLS>(g);
+    INTERIOR_COLUMN(g, KC)
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    if (active) {
+        for (int k = kb; k < ke; ++k) {
+            const size_t c = c0_ + sz_ * k;
+            const Cond<double> o = conductances<double, CLS>(g, stab, i, j, k, c);
+            const double dg = diag(o);
+            const double dp = (((offdiag(o, x, c, sy_, sz_) - b[c]) / dg - x[c])) * 1.0;
+            const double r = (dg + omega * (o.xm + o.ym + o.zm)) * dp / omega;
+            v[0] = dd_add(v[0], r * r);
+        }
+    }
+    block_partials<1>(v, rb);
+}
+
+hipError_t launch_res_sor(const double *x, const double *b, const Geo<double> &g, double omega, int fin, KState *st,
+                          double *hist, const RedBuf &rb, hipStream_t s)
+{
+    Shape sh = interior_shape(g.NX, g.NY, g.NZ);
+    if (g.cls)
+        hipLaunchKernelGGL((k_res_sor<true>), sh.grid, sh.block, table_bytes(g), s, x, b, g, omega, st, rb, sh.KC);
+    else
+        hipLaunchKernelGGL((k_res_sor<false>), sh.grid, sh.block, 0, s, x, b, g, omega, st, rb, sh.KC);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
 }
 
 // Colour 0 of the first sweep of Preconditioner! (NonUniform.jl:553-558)

@@ -265,6 +265,29 @@ class WorkBuffers:
                                                   ctypes.byref(ms)))
         return ms.value
 
+    # stand-alone RB-SOR (src/NonUniform.jl:680-1050) ---------------------
+    def res_sor(self, x, b, omega, dt=1.0, is_steady=True):
+        """resSOR of field x against field b."""
+        r = ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_res_sor(self.ctx, self._fid(x), self._fid(b), float(omega), float(dt),
+                                              int(bool(is_steady)), ctypes.byref(r)))
+        return r.value
+
+    def rbsor(self, x, b, omega, dt=1.0, is_steady=True):
+        """rbsor!: one red-black SOR sweep of x in place; the residual RMS."""
+        r = ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_rbsor(self.ctx, self._fid(x), self._fid(b), float(omega), float(dt),
+                                            int(bool(is_steady)), ctypes.byref(r)))
+        return r.value
+
+    def solve_sor(self, omega, tol, max_itr=ITR_MAX, dt=1.0, is_steady=True):
+        """solveSOR! on θ and b -> (isconverged, itr, res0)."""
+        conv, itr, res0 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_rbsor_solve(self.ctx, float(dt), float(omega), float(tol), int(max_itr),
+                                                  int(bool(is_steady)), ctypes.byref(conv), ctypes.byref(itr),
+                                                  ctypes.byref(res0)))
+        return bool(conv.value), itr.value, res0.value
+
     # sensitivity extension (no reference counterpart) ----------------------
     def objective(self, region, beta):
         """Smooth maximum J of θ over the interior cells where region != 0;

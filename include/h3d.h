@@ -228,6 +228,23 @@ int h3d_release_slots(h3d_ctx *ctx);
  * (hipMemGetInfo on the context's device). */
 int h3d_mem_info(h3d_ctx *ctx, size_t *used, size_t *total);
 
+/* Stand-alone red-black SOR (FP64, the reference's operator form).
+ * h3d_res_sor: resSOR (src/NonUniform.jl:680-744), sqrt(Σ r²) over every
+ *   interior cell of r = (dd + ω(cond_xm + cond_ym + cond_zm))·dp/ω,
+ *   dp = (ss - b)/dd - x, without updating x.
+ * h3d_rbsor: rbsor! (src/NonUniform.jl:936-957), colour 0 then colour 1
+ *   of rbsor_core! with relaxation ω in place on x; *res = sqrt of the
+ *   summed residual squares of both colours.
+ * h3d_rbsor_solve: solveSOR! (src/NonUniform.jl:1017-1050) on θ and b:
+ *   res0 = resSOR (1 when it is 0), then rbsor! until its residual / res0
+ *   < tol or maxItr iterations; outputs and history as h3d_pbicgstab.
+ * The sums are exactly rounded (the reference's are sequential or
+ * threaded). */
+int h3d_res_sor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, double *res);
+int h3d_rbsor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, double *res);
+int h3d_rbsor_solve(h3d_ctx *ctx, double dt, double omega, double tol, int maxItr, int is_steady,
+                    int *isconverged, int *itr, double *res0);
+
 /* Timing of one kernel family on the context stream (HIP events around
  * reps back-to-back launches after one warm-up launch; *ms = mean per
  * launch).  which: 0 = one full red-black GS sweep (both colours),

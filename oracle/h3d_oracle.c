@@ -803,10 +803,21 @@ static double rbsor_core(double *x, const double *b, const orc_work_t *wk, const
         }
     }
     if (!rr) return 0.0;
+    /* the colour's cells in (k,j,i) order: sequential in mode 0, exactly
+       rounded in mode 1 (the updated cells' entries; the others are 0) */
     double res = 0.0;
-    for (int k = 2; k <= MZ - 1; ++k)
-        for (int j = 2; j <= MY - 1; ++j)
-            for (int i = 2 + (k + j + color) % 2; i <= MX - 1; i += 2) res = res + rr[I3(i, j, k)];
+    if (g_sum_mode == 1) {
+        xsum_t acc;
+        xsum_init(&acc);
+        for (int k = 2; k <= MZ - 1; ++k)
+            for (int j = 2; j <= MY - 1; ++j)
+                for (int i = 2 + (k + j + color) % 2; i <= MX - 1; i += 2) xsum_add(&acc, rr[I3(i, j, k)]);
+        res = xsum_round(&acc);
+    } else {
+        for (int k = 2; k <= MZ - 1; ++k)
+            for (int j = 2; j <= MY - 1; ++j)
+                for (int i = 2 + (k + j + color) % 2; i <= MX - 1; i += 2) res = res + rr[I3(i, j, k)];
+    }
     free(rr);
     return res;
 }
@@ -814,6 +825,55 @@ double orc_rbsor_core(double *x, const double *b, const orc_work_t *wk, const or
                       double omega, int color)
 {
     return rbsor_core(x, b, wk, g, omega, color, 1);
+}
+
+/* resSOR (NonUniform.jl:680-744): sqrt(Σ r²) over every interior cell of
+ * r = (dd + ω(cond_xm + cond_ym + cond_zm))·dp/ω, dp = ((ss - b)/dd - θ)·m,
+ * without updating θ. */
+double orc_res_sor(const double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega)
+{
+    const int MX = g->MX, MY = g->MY, MZ = g->MZ;
+    double *rr = (double *)calloc((size_t)MX * MY * MZ, sizeof(double));
+#pragma omp parallel for schedule(static)  num_threads(orc_get_threads())
+    for (int k = 2; k <= MZ - 1; ++k)
+        for (int j = 2; j <= MY - 1; ++j)
+            for (int i = 2; i <= MX - 1; ++i) {
+                size_t c = I3(i, j, k);
+                stencil_t s = stencil(x, wk->lam, wk->mask, wk->rho, wk->cp, g, i, j, k);
+                double dp = (((s.ss - b[c]) / s.dd - x[c])) * wk->mask[c];
+                double r = (s.dd + omega * (s.cxm + s.cym + s.czm)) * dp / omega;
+                rr[c] = r;
+            }
+    double res = seq_sum_sq(rr, MX, MY, MZ);
+    free(rr);
+    return sqrt(res);
+}
+
+/* rbsor! (NonUniform.jl:936-957): colour 0 then colour 1, returns
+ * sqrt((0 + res_c0) + res_c1). */
+double orc_rbsor(double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega)
+{
+    double res = 0.0;
+    for (int c = 0; c <= 1; ++c) res += rbsor_core(x, b, wk, g, omega, c, 1);
+    return sqrt(res);
+}
+
+/* solveSOR! (NonUniform.jl:1017-1050) on θ and b of wk: res0 = resSOR (1
+ * when 0), then rbsor!/res0 per iteration into hist until < tol.  Returns
+ * 1 when converged; *itr = iterations run. */
+int orc_solve_sor(orc_work_t *wk, const orc_grid_t *g, double omega, double tol, int maxItr, int *itr,
+                  double *res0_out, double *hist, int hist_cap)
+{
+    double res0 = orc_res_sor(wk->theta, wk->b, wk, g, omega);
+    if (res0 == 0.0) res0 = 1.0;
+    *res0_out = res0;
+    for (int n = 1; n <= maxItr; ++n) {
+        double res = orc_rbsor(wk->theta, wk->b, wk, g, omega) / res0;
+        if (hist && n <= hist_cap) hist[n - 1] = res;
+        *itr = n;
+        if (res < tol) return 1;
+    }
+    return 0;
 }
 
 /* Preconditioner! (NonUniform.jl:531-559): :none copies, :gs runs

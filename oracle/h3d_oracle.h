@@ -83,6 +83,10 @@ double orc_calc_rk(double *r, const double *theta, const double *b,
                    const orc_work_t *wk, const orc_grid_t *g);
 double orc_rbsor_core(double *x, const double *b, const orc_work_t *wk,
                       const orc_grid_t *g, double omega, int color);
+double orc_res_sor(const double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega);
+double orc_rbsor(double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega);
+int orc_solve_sor(orc_work_t *wk, const orc_grid_t *g, double omega, double tol, int maxItr, int *itr,
+                  double *res0_out, double *hist, int hist_cap);
 void orc_precond(double *xx, const double *bb, const orc_work_t *wk,
                  const orc_grid_t *g, int smoother);      /* 0 none, 1 gs */
 double orc_fdot1(const double *x, int MX, int MY, int MZ);

@@ -207,6 +207,31 @@ class Case:
         f.argtypes = [_vp, _vp, _vp, _vp, _d, _i]
         return f(_p(self.arr[x]), _p(self.arr[b]), ctypes.byref(self.work), ctypes.byref(self.grid), omega, color)
 
+    def res_sor(self, x, b, omega):
+        """resSOR (src/NonUniform.jl:680-744)"""
+        f = load().orc_res_sor
+        f.argtypes = [_vp, _vp, _vp, _vp, _d]
+        f.restype = _d
+        return f(_p(self.arr[x]), _p(self.arr[b]), ctypes.byref(self.work), ctypes.byref(self.grid), omega)
+
+    def rbsor(self, x, b, omega):
+        """rbsor! (src/NonUniform.jl:936-957)"""
+        f = load().orc_rbsor
+        f.argtypes = [_vp, _vp, _vp, _vp, _d]
+        f.restype = _d
+        return f(_p(self.arr[x]), _p(self.arr[b]), ctypes.byref(self.work), ctypes.byref(self.grid), omega)
+
+    def solve_sor(self, omega, tol=1e-6, maxItr=20000):
+        """solveSOR! (src/NonUniform.jl:1017-1050) on θ, b: (converged, itr, res0, history)"""
+        f = load().orc_solve_sor
+        f.argtypes = [_vp, _vp, _d, _d, _i, ctypes.POINTER(_i), ctypes.POINTER(_d), _vp, _i]
+        f.restype = _i
+        itr, res0 = _i(), _d()
+        hist = np.zeros(maxItr)
+        conv = f(ctypes.byref(self.work), ctypes.byref(self.grid), omega, tol, maxItr, ctypes.byref(itr),
+                 ctypes.byref(res0), _p(hist), maxItr)
+        return bool(conv), itr.value, res0.value, hist[: itr.value].copy()
+
     def precond(self, xx, bb, smoother=1):
         f = load().orc_precond
         f.argtypes = [_vp, _vp, _vp, _vp, _i]
