@@ -22,6 +22,10 @@ Printed JSON line (one, on stdout):
   c3         480 x 480 x 60 with the seeded power map: the same two solves
              to 1e-8 and their maximum temperature difference (K) from an
              FP64 solve to 1e-12
+  cg         CG! + GS(5) on the same 240 x 240 x 30 case to eps = 1e-6
+  c5_step    one implicit-Euler step (dt = 1 ms) of the 960 x 960 x 120
+             transient case from 300 K, FP64 to 1e-8: iterations, time and
+             the reference-equivalent HBM rate
 """
 import argparse
 import json
@@ -49,6 +53,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-refined", action="store_true", help="skip the FP32-refinement comparison")
     ap.add_argument("--no-c3", action="store_true", help="skip the 480x480x60 precision comparison")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 960x960x120 transient step")
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG! solve")
     ap.add_argument("--cpu-iters", type=int, default=200)
     return ap.parse_args()
 
@@ -111,6 +117,28 @@ def precision_c3(np, h2_amd, L, WorkBuffers, load_model):
                      "max_abs_err_K": float("%.3e" % np.max(np.abs(th - ref)))}
         log("c3 %s: %.1f ms, itr %d, converged %s" % (name, t * 1e3, itr, conv))
     out["speedup_fp32_refined"] = round(out["fp64"]["ms"] / out["fp32_refined"]["ms"], 3)
+    wk.close()
+    return out
+
+
+def transient_step_c5(h2_amd, L, WorkBuffers, load_model):
+    """One implicit-Euler step of the 960x960x120 power-map case, FP64."""
+    log("building model 960x960x120 (power map)")
+    m = h2_amd.build_model(960, 960, 120, power="map")
+    wk = WorkBuffers(*m.shape)
+    load_model(wk, m)
+    dt = 1e-3
+    wk.fill("b", 0.0)
+    t0 = time.perf_counter()
+    wk.calc_rhs(dt, False)
+    conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, dt, 1e-8, 20000, L.SMOOTHER_GS, False)
+    t = time.perf_counter() - t0
+    cells = 960 * 960 * 120
+    # a transient iteration reads ap0's class table, no extra array: 560 B/cell
+    out = {"dt": dt, "tol": 1e-8, "itr": itr, "converged": conv, "s": round(t, 3),
+           "ms_per_iteration": round(t / max(itr, 1) * 1e3, 3),
+           "gbs": round(BYTES_PER_CELL_ITER["fp64"] * cells * itr / t / 1e9, 1)}
+    log("c5 step: %.2f s, itr %d" % (t, itr))
     wk.close()
     return out
 
@@ -186,11 +214,23 @@ def main():
                             "outer_residuals": [float("%.3e" % v) for v in wk.history()]})
         except L.H3DError as e:
             refined["fp32_refined_error"] = str(e)
+    # CG! + GS(5) to 1e-6 on the same case
+    cg = None
+    if not args.no_cg:
+        wk.upload("theta", th0)
+        wk.fill("b", 0.0)
+        t0 = time.perf_counter()
+        wk.calc_rhs(dt, True)
+        conv, itr_cg, _ = wk.solve(L.SOLVER_CG, dt, 1e-6, 20000, L.SMOOTHER_GS, True)
+        cg = {"eps": 1e-6, "ms": round((time.perf_counter() - t0) * 1e3, 3), "itr": itr_cg, "converged": conv}
     wk.close()
 
     c3 = None
     if not args.no_c3:
         c3 = precision_c3(np, h2_amd, L, WorkBuffers, load_model)
+    c5 = None
+    if not args.no_c5:
+        c5 = transient_step_c5(h2_amd, L, WorkBuffers, load_model)
 
     cpu = None
     if not args.no_cpu:
@@ -206,7 +246,8 @@ def main():
             "config": {"workload": "modelA %dx%dx%d steady PBiCGSTAB+GS(5) eps=%g, calRHS!+solve"
                                    % (nx, nx, nz, args.eps), "cells": cells, "iterations": itr,
                        "device_solve_ms": round(sum(dev_ms) / len(dev_ms), 3)},
-            "roofline": roofline, "cpu_baseline": cpu, "fp32_refined": refined, "c3": c3}
+            "roofline": roofline, "cpu_baseline": cpu, "fp32_refined": refined, "cg": cg, "c3": c3,
+            "c5_step": c5}
     print(json.dumps(line), flush=True)
 
 

@@ -506,68 +506,88 @@ hipError_t launch_rb_form_rhs(int mode, T *bb, const T *r, const T *q, const RbG
 // operator, residual, BLAS-1 with fused sums
 // ---------------------------------------------------------------------------
 // CalcAX! (src/NonUniform.jl:454-511): ax = A x; with NS >= 1 also
-// s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz.  Pair mapping; offsets are into
-// the whole arrays (both halves).
-template <typename T, int NS>
+// s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz.  Pair mapping (cells 2m and
+// 2m+1 of a row) over R rows per thread (rb_rows).  Offsets are into the
+// whole arrays (both halves).
+template <typename T, int NS, int R>
 __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T *__restrict__ x,
                                                     const T *__restrict__ dy, const T *__restrict__ dz, RbGeo<T> g,
                                                     KState *st, RedBuf rb)
 {
     if (halted(st)) return;
-    const RbThread t = rb_thread(g);
+    const RbRows t = rb_rows<R>(g);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY, HS = (unsigned)g.HS;
-    RbCell c[2];
-    int h[2];
-    rb_pair(g, t, c, h);
     const unsigned sz = (unsigned)sizeof(T);
-    Nb<T> n[2];
-    Cls q[2];
-    T xv[2], av[2], dyv[2], dzv[2];
+    Nb<T> n[R][2];
+    Cls q[R][2];
+    T xv[R][2], av[R][2], dyv[R][2], dzv[R][2];
+    unsigned co[R][2];
+    bool valid[R][2];
+    int jj[R];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        const unsigned co = (h[a] ? HS : 0u) + c[a].co, no = (h[a] ? 0u : HS) + c[a].co;
-        n[a] = rb_nb(x, no, c[a].s, HX, HXY);
-        xv[a] = ldb(x, co * sz);
-        q[a] = rb_cls(g.cls, g.cls, co, no, c[a].s, HX, HXY);
-        dyv[a] = (NS >= 1 && dy) ? ldb(dy, co * sz) : T(0);
-        dzv[a] = NS >= 2 ? ldb(dz, co * sz) : T(0);
-    }
-    const T *stab = rb_stage(g, t.k);
+    for (int r = 0; r < R; ++r) {
+        jj[r] = min(t.j0 + r * BY, g.NY);
+        const bool rowok = t.live && t.j0 + r * BY <= g.NY;
+        const unsigned base = (unsigned)t.m + HX * (unsigned)jj[r] + HXY * (unsigned)t.k;
+        const int h0 = (jj[r] + t.k) & 1;
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        const Cond<T> cd = rb_cond(g, stab, q[a], c[a].i, t.j, t.k);
-        av[a] = rb_apply(g, cd, n[a], xv[a]);
-        if (!(NS >= 1 && dy)) dyv[a] = av[a];
-    }
-    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        if (c[a].valid) {
-            if (NS >= 1) v[0] = dd_add(v[0], (double)av[a] * (double)dyv[a]);
-            if (NS >= 2) v[1] = dd_add(v[1], (double)av[a] * (double)dzv[a]);
+        for (int a = 0; a < 2; ++a) {
+            const int h = h0 ^ a, i = 2 * t.m + a;
+            co[r][a] = (h ? HS : 0u) + base;
+            const unsigned no = (h ? 0u : HS) + base;
+            valid[r][a] = rowok && i >= 1 && i <= g.NX;
+            n[r][a] = rb_nb<T>(x, no, a, HX, HXY);
+            xv[r][a] = ldb(x, co[r][a] * sz);
+            q[r][a] = rb_cls(g.cls, g.cls, co[r][a], no, a, HX, HXY);
+            dyv[r][a] = (NS >= 1 && dy) ? ldb(dy, co[r][a] * sz) : T(0);
+            dzv[r][a] = NS >= 2 ? ldb(dz, co[r][a] * sz) : T(0);
         }
     }
+    const T *stab = rb_stage(g, t.k);
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const Cond<T> cd = rb_cond(g, stab, q[r][a], 2 * t.m + a, jj[r], t.k);
+            av[r][a] = rb_apply(g, cd, n[r][a], xv[r][a]);
+            if (!(NS >= 1 && dy)) dyv[r][a] = av[r][a];
+            if (valid[r][a]) {
+                if (NS >= 1) v[0] = dd_add(v[0], (double)av[r][a] * (double)dyv[r][a]);
+                if (NS >= 2) v[1] = dd_add(v[1], (double)av[r][a] * (double)dzv[r][a]);
+            }
+        }
     // the workgroup sum first, the stores last: a store before the
     // reduction's barrier would hold the wave until it completes
     if (NS > 0) block_partials<NS>(v, rb);
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-        if (c[a].valid) stb(ax, ((h[a] ? HS : 0u) + c[a].co) * sz, av[a]);
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            if (valid[r][a]) stb(ax, co[r][a] * sz, av[r][a]);
 }
+
+// One row per thread: two rows measured slower at 960x960x120 (888 against
+// 838 µs for CalcAX!, 1013 against 975 µs with its dot) and equal at
+// 240x240x30.
+constexpr int kAxRows = 1;
 
 template <typename T>
 hipError_t launch_rb_calc_ax(T *ax, const T *x, const RbGeo<T> &g, const T *dot_y, const T *dot_z, int fin,
                              KState *st, double *hist, const RedBuf &rb, hipStream_t s)
 {
+    constexpr int R = kAxRows;
     const int ns = fin < 0 ? 0 : (dot_z ? 2 : 1);
-    const dim3 grid(g.nb1), block(BX, BY);
-    if (ns == 0) hipLaunchKernelGGL((k_rb_calc_ax<T, 0>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+    const unsigned nb = rb_rows_nb<R>(g);
+    const dim3 grid(nb), block(BX, BY);
+    if (ns == 0)
+        hipLaunchKernelGGL((k_rb_calc_ax<T, 0, R>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     else if (ns == 1)
-        hipLaunchKernelGGL((k_rb_calc_ax<T, 1>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
-    else hipLaunchKernelGGL((k_rb_calc_ax<T, 2>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+        hipLaunchKernelGGL((k_rb_calc_ax<T, 1, R>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+    else hipLaunchKernelGGL((k_rb_calc_ax<T, 2, R>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ns == 0) return e;
-    return launch_finalize_nb(ns, g.nb1, fin, st, hist, rb, s);
+    return launch_finalize_nb(ns, nb, fin, st, hist, rb, s);
 }
 
 // CalcRK! (src/NonUniform.jl:372-434) with θ, b in standard layout; r (and
