@@ -174,11 +174,14 @@ class SteadySensitivity:
         self.volume = np.zeros(m.ID.shape, order="F")
         self.volume[1:-1, 1:-1, 1:-1] = m.dx * m.dy * m.dZ[None, None, 1:-1]
         self.last = None
+        self._solved = False    # the device holds the forward solution of the current parameters
+        self._theta0 = None
 
     def close(self):
         self.wk.close()
 
     def set_params(self, scales=None, hsrc=None):
+        self._solved = False
         if scales is not None:
             self.scales = np.asarray(scales, dtype=np.float64).copy()
         if hsrc is not None:
@@ -199,6 +202,8 @@ class SteadySensitivity:
             self.wk.upload("theta", theta_start)
         conv, itr, res0 = self.wk.solve(self.solver, self.dt, self.tol, self.max_itr, self.smoother, True)
         self.last = {"converged": conv, "itr": itr, "res0": res0, "solve_ms": self.wk.last_solve_ms()}
+        self._solved = True
+        self._theta0 = None       # host copy of the solution, formed on first use
         return self.wk.objective(self.region, self.beta)
 
     def theta(self):
@@ -206,6 +211,8 @@ class SteadySensitivity:
 
     def adjoint(self):
         """Gradients of the last forward solve's J."""
+        if not self._solved:
+            raise RuntimeError("adjoint() needs forward() after the last parameter change or difference solve")
         conv, itr, res0 = self.wk.adjoint_solve(self.solver, self.dt, self.tol, self.max_itr, self.smoother, True)
         adj_ms = self.wk.last_solve_ms()
         glam, ghs = self.wk.gradients()
@@ -228,6 +235,7 @@ class SteadySensitivity:
         FloatMin = 1e-37 (src/common.jl:12), which a converging solve of a
         tiny right-hand side would reach."""
         wk = self.wk
+        self._solved = False
         scale = 2.0 ** (-np.round(np.log2(np.max(np.abs(rhs)))))
         wk.upload("lam", lam)
         wk.upload("theta", self.theta_init)
@@ -249,7 +257,11 @@ class SteadySensitivity:
         to the change instead of the temperature, and the right-hand side
         holds conductance differences times temperature differences."""
         if theta0 is None:
-            theta0 = self.theta()
+            if self._theta0 is None:
+                if not self._solved:
+                    raise RuntimeError("central_difference() needs forward() first (or theta0)")
+                self._theta0 = self.theta()
+            theta0 = self._theta0
         lam0 = scaled_lambda(self.m, self.groups, self.scales)
         thetas = []
         for sign in (1.0, -1.0):

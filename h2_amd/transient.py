@@ -65,6 +65,8 @@ class TransientRun:
         self.slot_seg = SLOT_CKPT + self.nseg + 1
         self.stats = {}
         self.verbose = verbose
+        self._setup_count = 0     # bumped whenever the device fields are set up again
+        self._ckpt_setup = -1     # the set-up whose forward run left checkpoints
 
     def _log(self, *a):
         if self.verbose:
@@ -83,6 +85,7 @@ class TransientRun:
         if hsrc is not None:
             self.hsrc = np.array(hsrc, dtype=np.float64, order="F", copy=True)
         wk = self.wk
+        self._setup_count += 1
         wk.upload("lam", scaled_lambda(self.m, self.groups, self.scales))
         wk.upload("theta", self.theta_init)
         wk.apply_bc()
@@ -120,12 +123,16 @@ class TransientRun:
         self.stats["forward_iters"] = iters
         self.stats["mem_forward"] = wk.mem_info()[0]
         self.J = J
+        self._ckpt_setup = self._setup_count if checkpoints else -1
         return J
 
     def adjoint(self):
         """dJ_total/dscale (13 groups), dJ_total/dλ and dJ_total/dhsrc_base
         per cell, for the last forward run (which must have kept its
         checkpoints)."""
+        if self._ckpt_setup != self._setup_count:
+            raise RuntimeError("adjoint() needs forward(checkpoints=True) after the last parameter change, "
+                               "states() or central_difference()")
         wk = self.wk
         wk.reset_gradients()
         chain = False
@@ -193,6 +200,7 @@ class TransientRun:
         """δ of A(λ) δ = rhs (transient operator, zero halo), solved for a
         power-of-two multiple of rhs of order one."""
         wk = self.wk
+        self._setup_count += 1
         scale = 2.0 ** (-np.round(np.log2(np.max(np.abs(rhs)))))
         wk.upload("lam", lam)
         wk.upload("theta", self.theta_init)

@@ -81,4 +81,7 @@ def test_c4_adjoint_vs_central_differences():
         print(kind, idx, a, fd, abs(a - fd) / abs(fd))
     worst = max(abs(a - fd) / abs(fd) for _, _, a, fd in errs)
     assert worst <= 1e-5, errs
+    # the difference solves replaced the forward solution on the device
+    with pytest.raises(RuntimeError):
+        sens.adjoint()
     sens.close()
