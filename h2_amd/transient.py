@@ -241,6 +241,7 @@ class TransientRun:
                     rhs = self._rhs_change(d) - apply_conductance_change(m, change, thetas[n])
                     d = self._solve_change(rhs, lam)
                     ds.append(d)
+                    self._log("difference group %d sign %+d step %d" % (k, int(sign), n))
                 deltas.append(ds)
         finally:
             self.wk.set_operator_form(L.FORM_REFERENCE)
