@@ -18,14 +18,16 @@ Printed JSON line (one, on stdout):
   cpu_baseline  the oracle (OpenMP C restatement of the reference
              algorithm) on a bounded sample of the same workload
   fp32_refined  the same case solved to a true relative residual of 1e-8
-             with FP64 and with FP32-inner/FP64-refinement, and the speed-up
-  c3         480 x 480 x 60 with the seeded power map: the same two solves
-             to 1e-8 and their maximum temperature difference (K) from an
-             FP64 solve to 1e-12
+             with FP64 and with FP32-inner/FP64-refinement, and the speed-up;
+             the same with CG! (FP32 CG inner solves)
+  c3         480 x 480 x 60 with the seeded power map: the same solves
+             (PBiCGSTAB! and CG!, FP64 and FP32-refined) to 1e-8 and their
+             maximum temperature difference (K) from an FP64 solve to 1e-12
   cg         CG! + GS(5) on the same 240 x 240 x 30 case to eps = 1e-6
   c5_step    one implicit-Euler step (dt = 1 ms) of the 960 x 960 x 120
-             transient case from 300 K, FP64 to 1e-8: iterations, time and
-             the reference-equivalent HBM rate
+             transient case from 300 K, FP64 PBiCGSTAB! to 1e-8: iterations,
+             time and the reference-equivalent HBM rate; the same step with
+             FP32-refined CG!
 """
 import argparse
 import json
@@ -101,22 +103,27 @@ def precision_c3(np, h2_amd, L, WorkBuffers, load_model):
     load_model(wk, m)
     th0 = wk.download("theta")
 
-    def run(prec, tol):
+    def run(prec, tol, solver=L.SOLVER_PBICGSTAB):
         wk.upload("theta", th0)
         wk.fill("b", 0.0)
         t0 = time.perf_counter()
         wk.calc_rhs(10000.0, True)
-        conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, tol, 20000, L.SMOOTHER_GS, True, precision=prec)
+        conv, itr, _ = wk.solve(solver, 10000.0, tol, 20000, L.SMOOTHER_GS, True, precision=prec)
         return time.perf_counter() - t0, conv, itr, wk.download("theta")
 
     _, _, itr_ref, ref = run(L.FP64, 1e-12)
-    out = {"tol": 1e-8, "reference": "FP64 to 1e-12 (%d iterations)" % itr_ref}
-    for name, prec in (("fp64", L.FP64), ("fp32_refined", L.FP32_REFINED)):
-        t, conv, itr, th = run(prec, 1e-8)
+    out = {"tol": 1e-8, "reference": "FP64 PBiCGSTAB to 1e-12 (%d iterations)" % itr_ref}
+    for name, prec, solver in (("fp64", L.FP64, L.SOLVER_PBICGSTAB),
+                               ("fp32_refined", L.FP32_REFINED, L.SOLVER_PBICGSTAB),
+                               ("cg_fp64", L.FP64, L.SOLVER_CG),
+                               ("cg_fp32_refined", L.FP32_REFINED, L.SOLVER_CG)):
+        t, conv, itr, th = run(prec, 1e-8, solver)
         out[name] = {"ms": round(t * 1e3, 2), "itr": itr, "converged": conv,
                      "max_abs_err_K": float("%.3e" % np.max(np.abs(th - ref)))}
         log("c3 %s: %.1f ms, itr %d, converged %s" % (name, t * 1e3, itr, conv))
     out["speedup_fp32_refined"] = round(out["fp64"]["ms"] / out["fp32_refined"]["ms"], 3)
+    out["speedup_cg_fp32_refined_vs_cg_fp64"] = round(out["cg_fp64"]["ms"] / out["cg_fp32_refined"]["ms"], 3)
+    out["speedup_cg_fp32_refined_vs_fp64"] = round(out["fp64"]["ms"] / out["cg_fp32_refined"]["ms"], 3)
     wk.close()
     return out
 
@@ -128,17 +135,27 @@ def transient_step_c5(h2_amd, L, WorkBuffers, load_model):
     wk = WorkBuffers(*m.shape)
     load_model(wk, m)
     dt = 1e-3
-    wk.fill("b", 0.0)
-    t0 = time.perf_counter()
-    wk.calc_rhs(dt, False)
-    conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, dt, 1e-8, 20000, L.SMOOTHER_GS, False)
-    t = time.perf_counter() - t0
+    th0 = wk.download("theta")
+
+    def one(solver, prec):
+        wk.upload("theta", th0)
+        wk.fill("b", 0.0)
+        t0 = time.perf_counter()
+        wk.calc_rhs(dt, False)
+        conv, itr, _ = wk.solve(solver, dt, 1e-8, 20000, L.SMOOTHER_GS, False, precision=prec)
+        return time.perf_counter() - t0, conv, itr
+
+    t, conv, itr = one(L.SOLVER_PBICGSTAB, L.FP64)
     cells = 960 * 960 * 120
     # a transient iteration reads ap0's class table, no extra array: 560 B/cell
     out = {"dt": dt, "tol": 1e-8, "itr": itr, "converged": conv, "s": round(t, 3),
            "ms_per_iteration": round(t / max(itr, 1) * 1e3, 3),
            "gbs": round(BYTES_PER_CELL_ITER["fp64"] * cells * itr / t / 1e9, 1)}
     log("c5 step: %.2f s, itr %d" % (t, itr))
+    t32, conv32, itr32 = one(L.SOLVER_CG, L.FP32_REFINED)
+    out["cg_fp32_refined"] = {"s": round(t32, 3), "inner_itr": itr32, "converged": conv32,
+                              "speedup_vs_fp64_pbicgstab": round(t / t32, 3)}
+    log("c5 step CG FP32-refined: %.2f s, itr %d" % (t32, itr32))
     wk.close()
     return out
 
@@ -163,12 +180,12 @@ def main():
     dt = 10000.0
     cells = nx * nx * nz
 
-    def step(precision=L.FP64, tol=args.eps):
+    def step(precision=L.FP64, tol=args.eps, solver=L.SOLVER_PBICGSTAB):
         wk.upload("theta", th0)
         wk.fill("b", 0.0)
         t0 = time.perf_counter()
         wk.calc_rhs(dt, True)
-        conv, itr, res0 = wk.solve(L.SOLVER_PBICGSTAB, dt, tol, 20000, L.SMOOTHER_GS, True, precision)
+        conv, itr, res0 = wk.solve(solver, dt, tol, 20000, L.SMOOTHER_GS, True, precision)
         t1 = time.perf_counter()
         return t1 - t0, wk.last_solve_ms(), conv, itr, res0
 
@@ -214,6 +231,13 @@ def main():
                             "outer_residuals": [float("%.3e" % v) for v in wk.history()]})
         except L.H3DError as e:
             refined["fp32_refined_error"] = str(e)
+        # the same pair with CG! as the solver (FP32 CG inner solves)
+        c64 = step(L.FP64, 1e-8, L.SOLVER_CG)
+        c32 = step(L.FP32_REFINED, 1e-8, L.SOLVER_CG)
+        refined.update({"cg_fp64_ms": round(c64[0] * 1e3, 3), "cg_fp64_itr": c64[3],
+                        "cg_fp32_refined_ms": round(c32[0] * 1e3, 3), "cg_fp32_refined_inner_itr": c32[3],
+                        "cg_fp32_refined_converged": c32[2], "speedup_cg": round(c64[0] / c32[0], 3),
+                        "speedup_cg_refined_vs_fp64_pbicgstab": round(r64[0] / c32[0], 3)})
     # CG! + GS(5) to 1e-6 on the same case
     cg = None
     if not args.no_cg:

@@ -263,6 +263,7 @@ def main(argv=None):
     ap.add_argument("--dt", type=float, default=1.0e-3)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp32")
+    ap.add_argument("--solver", choices=("pbicgstab", "cg"), default="pbicgstab")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fd-groups", type=str, default="", help="comma-separated groups for central differences")
     ap.add_argument("--fd-step", type=float, default=1e-6)
@@ -273,12 +274,13 @@ def main(argv=None):
     m = build_model(a.nx, a.ny, a.nz, power="map")
     print("model built in %.1f s" % (time.perf_counter() - t0), flush=True)
     prec = L.FP32_REFINED if a.precision == "fp32" else L.FP64
+    solver = L.SOLVER_CG if a.solver == "cg" else L.SOLVER_PBICGSTAB
     run = TransientRun(m, dt=a.dt, nsteps=a.steps, checkpoint_every=a.every, tol=a.tol, precision=prec,
-                       verbose=True)
+                       solver=solver, verbose=True)
     used0, total = run.wk.mem_info()
     J = run.forward()
     out = {"grid": [a.nx, a.ny, a.nz], "steps": a.steps, "dt": a.dt, "checkpoint_every": a.every, "tol": a.tol,
-           "precision": a.precision, "J_total": J, "forward_s": run.stats["forward_s"],
+           "precision": a.precision, "solver": a.solver, "J_total": J, "forward_s": run.stats["forward_s"],
            "forward_iters": run.stats["forward_iters"], "device_mem_before_run": used0,
            "device_mem_after_forward": run.stats["mem_forward"], "device_total": total}
     if not a.no_adjoint:

@@ -3,7 +3,8 @@ Zcase2 cell split in two): the FP64 solve to a relative residual of 1e-8
 against an FP64 solve to 1e-12 (maximum temperature difference in K), and
 the FP32-refined solve at the same 1e-8.  The refined solve must either
 converge to the same accuracy or report that it did not (its guard keeps
-the best correction; the temperature stays finite)."""
+the best correction; the temperature stays finite).  The FP32-refined
+solve with CG! inner solves converges to the same accuracy."""
 import numpy as np
 import pytest
 
@@ -22,11 +23,11 @@ def test_c3_fp64_and_refined_against_tight_fp64():
     load_model(wk, m)
     th0 = wk.download("theta")
 
-    def run(prec, tol):
+    def run(prec, tol, solver=L.SOLVER_PBICGSTAB):
         wk.upload("theta", th0)
         wk.fill("b", 0.0)
         wk.calc_rhs(10000.0, True)
-        conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, tol, 20000, L.SMOOTHER_GS, True, precision=prec)
+        conv, itr, _ = wk.solve(solver, 10000.0, tol, 20000, L.SMOOTHER_GS, True, precision=prec)
         return conv, itr, wk.last_solve_ms(), wk.download("theta")
 
     conv, itr, ms, ref = run(L.FP64, 1e-12)
@@ -42,4 +43,9 @@ def test_c3_fp64_and_refined_against_tight_fp64():
     assert np.all(np.isfinite(t32))
     if conv:
         assert err32 < 1e-5
+    # CG! inner solves converge in FP32 on this case
+    conv, itr, ms, tc = run(L.FP32_REFINED, 1e-8, L.SOLVER_CG)
+    errc = np.max(np.abs(tc - ref))
+    print("fp32-refined CG 1e-8: converged %s, inner itr %d, %.1f ms, max |dθ| %.3e K" % (conv, itr, ms, errc))
+    assert conv and errc < 1e-5
     wk.close()
