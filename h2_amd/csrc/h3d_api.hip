@@ -104,6 +104,7 @@ struct h3d_ctx {
     std::vector<double *> slots;             // device copies of fields (h3d_store_field)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evb[2] = {nullptr, nullptr};
     double last_ms = 0.0;
+    int refine_floor = 0;        // the last refined solve stopped at the residual's rounding floor
     int verbose = 0;             // H3D_VERBOSE=1: refinement diagnostics on stderr
     std::string err;
 };
@@ -798,6 +799,15 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
     double res0 = 0.0;
     bool converged = false, nan = false;
     const int max_outer = 100;
+    // unit roundoff of FP64; a true residual within kFloorUlps·u·‖s‖ (s the
+    // per-cell rounding scale, launch_rk_scale) is below the error bound of
+    // its own evaluation (7 products and 7 sums per cell, plus the rounding
+    // of θ), so no FP64 iterate can be shown to have a smaller one
+    const double unit_roundoff = std::ldexp(1.0, -53);
+    const double kFloorUlps = 9.0;
+    double rn_prev = 0.0;
+    int slow = 0;
+    c->refine_floor = 0;
     for (int outer_it = 0; outer_it <= max_outer; ++outer_it) {
         // true FP64 residual r = b - Aθ
         HIPCHK(c, launch_calc_rk<double>(r64, nullptr, c->f[H3D_F_THETA], c->f[H3D_F_B], g64, FIN_STORE, c->d_unit,
@@ -813,6 +823,21 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         }
         if (!std::isfinite(rn)) { nan = true; break; }
         if (outer_it > 0 && rn / res0 < tol) { converged = true; break; }
+        if (outer_it > 0) {
+            HIPCHK(c, launch_rk_scale<double>(c->f[H3D_F_THETA], c->f[H3D_F_B], g64, c->d_unit, c->rb, s));
+            HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            const double floor_rn = kFloorUlps * unit_roundoff * std::sqrt(c->h_state[0].out[0]);
+            if (c->verbose)
+                std::fprintf(stderr, "h3d refine: outer %d true res %.6e is %.3f u|s| (rounding floor %.3e)\n",
+                             outer_it, rn / res0, rn / (unit_roundoff * std::sqrt(c->h_state[0].out[0])),
+                             floor_rn / res0);
+            if (rn <= floor_rn) { converged = true; c->refine_floor = 1; break; }
+            // stagnation: two corrections in a row that did not halve it
+            slow = rn > 0.5 * rn_prev ? slow + 1 : 0;
+            if (slow >= 2) break;
+        }
+        rn_prev = rn;
         if (outer_it == max_outer || total_inner >= maxItr) break;
         // inner FP32 solve of A d = r from d = 0, guarded: it stops on
         // divergence or stagnation and the correction is its best iterate
@@ -858,6 +883,7 @@ int solve(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int smoothe
     if (smoother != H3D_SMOOTHER_NONE && smoother != H3D_SMOOTHER_GS)
         return fail(c, H3D_ERR_ARG, "Unsupported smoother. Use none (0) or gs (1)");
     if (maxItr < 1) return fail(c, H3D_ERR_ARG, "maxItr must be >= 1");
+    c->refine_floor = 0;
     if (precision == H3D_FP64)
         return solve_fp64(c, solver, dt, tol, maxItr, smoother, is_steady, isconverged, itr, res0);
     if (precision == H3D_FP32_REFINED)
@@ -1187,6 +1213,13 @@ int h3d_last_solve_ms(const h3d_ctx *c, double *ms)
 {
     if (!c || !ms) return H3D_ERR_ARG;
     *ms = c->last_ms;
+    return H3D_OK;
+}
+
+int h3d_last_solve_floor(const h3d_ctx *c, int *at_floor)
+{
+    if (!c || !at_floor) return H3D_ERR_ARG;
+    *at_floor = c->refine_floor;
     return H3D_OK;
 }
 

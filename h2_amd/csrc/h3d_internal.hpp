@@ -168,6 +168,10 @@ hipError_t launch_calc_ax(T *ax, const T *x, const Geo<T> &g, const T *dot_y, co
 template <typename T>
 hipError_t launch_calc_rk(T *r, T *r0copy, const T *th, const T *b, const Geo<T> &g, int fin, KState *st,
                           double *hist, const RedBuf &rb, hipStream_t s);
+// st->out[0] = Σ s², s = |b| + Σ|terms of Aθ| per cell: ε·sqrt(out[0]) is the
+// rounding scale of the computed residual b - Aθ (double only)
+template <typename T>
+hipError_t launch_rk_scale(const T *th, const T *b, const Geo<T> &g, KState *st, const RedBuf &rb, hipStream_t s);
 // rbsor_core! (src/NonUniform.jl:848-915), one colour, ω = 1
 template <typename T>
 hipError_t launch_rbsor(T *x, const T *b, const Geo<T> &g, int color, const KState *st, hipStream_t s);

@@ -526,6 +526,52 @@ hipError_t launch_calc_rk(T *r, T *r0copy, const T *th, const T *b, const Geo<T>
     return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
 }
 
+// Σ s², s = |b| + Σ|terms of A θ| at each cell, the terms taken in the form
+// g.flux selects.  ε‖s‖₂ is the size of the rounding error of the computed
+// residual fl(b - Aθ): a true residual near it is at its attainable floor.
+template <typename T, bool CLS>
+__global__ void __launch_bounds__(256) k_rk_scale(const T *th, const T *b, Geo<T> g, RedBuf rb, int KC)
+{
+    const T *stab = stage_table<T, CLS>(g);
+    INTERIOR_COLUMN(g, KC)
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    if (active) {
+        for (int k = kb; k < ke; ++k) {
+            const size_t c = c0_ + sz_ * k;
+            const Cond<T> o = conductances<T, CLS>(g, stab, i, j, k, c);
+            const T x0 = th[c];
+            T a;
+            if (g.flux) {
+                const T sink = o.cxp + o.cxm + o.cyp + o.cym + o.czp + o.czm + o.ap0;
+                a = fabs(o.xp * (th[c + 1] - x0)) + fabs(o.xm * (th[c - 1] - x0)) +
+                    fabs(o.yp * (th[c + sy_] - x0)) + fabs(o.ym * (th[c - sy_] - x0)) +
+                    fabs(o.zp * (th[c + sz_] - x0)) + fabs(o.zm * (th[c - sz_] - x0)) + fabs(sink * x0);
+            } else {
+                a = fabs(o.xp * th[c + 1]) + fabs(o.xm * th[c - 1]) + fabs(o.yp * th[c + sy_]) +
+                    fabs(o.ym * th[c - sy_]) + fabs(o.zp * th[c + sz_]) + fabs(o.zm * th[c - sz_]) +
+                    fabs(diag(o) * x0);
+            }
+            a += fabs(b[c]);
+            v[0] = dd_add(v[0], (double)a * (double)a);
+        }
+    }
+    block_partials<1>(v, rb);
+}
+
+template <typename T>
+hipError_t launch_rk_scale(const T *th, const T *b, const Geo<T> &g, KState *st, const RedBuf &rb, hipStream_t s)
+{
+    Shape sh = interior_shape(g.NX, g.NY, g.NZ);
+    if (g.cls)
+        hipLaunchKernelGGL((k_rk_scale<T, true>), sh.grid, sh.block, table_bytes(g), s, th, b, g, rb, sh.KC);
+    else
+        hipLaunchKernelGGL((k_rk_scale<T, false>), sh.grid, sh.block, 0, s, th, b, g, rb, sh.KC);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_STORE, st, nullptr, rb, s);
+}
+template hipError_t launch_rk_scale<double>(const double *, const double *, const Geo<double> &, KState *,
+                                            const RedBuf &, hipStream_t);
+
 // Gauss-Seidel update of one cell, ω = 1 (rbsor_core!, NonUniform.jl:907-908):
 // dp = ((ss - b)/dd - pp)·m0; x = pp + ω·dp.  The flux form computes the
 // same correction as dp = ((A x)_c - b)/dd.

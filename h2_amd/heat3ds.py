@@ -204,6 +204,13 @@ class WorkBuffers:
         _check(self.ctx, self.lib.h3d_last_solve_ms(self.ctx, ctypes.byref(ms)))
         return ms.value
 
+    def last_solve_floor(self):
+        """True when the last FP32-refined solve converged by reaching the
+        rounding floor of its FP64 residual rather than the tolerance."""
+        f = ctypes.c_int()
+        _check(self.ctx, self.lib.h3d_last_solve_floor(self.ctx, ctypes.byref(f)))
+        return bool(f.value)
+
     # single operators -----------------------------------------------------
     def calc_ax(self, ax, x, dt=1.0, is_steady=True):
         _check(self.ctx, self.lib.h3d_calc_ax(self.ctx, self._fid(ax), self._fid(x), float(dt), int(bool(is_steady))))

@@ -146,7 +146,12 @@ int h3d_calc_rhs(h3d_ctx *ctx, double dt, int is_steady);
  * reference iteration; H3D_FP32_REFINED runs FP32 inner solves (relative
  * tolerance inner_tol, 1e-4 if <= 0) inside an FP64 refinement loop that
  * stops when the true FP64 ||b - A θ|| / res0 < tol; then *itr counts
- * inner iterations and the history holds the outer true residuals. */
+ * inner iterations and the history holds the outer true residuals.  The
+ * refinement also reports convergence when the true residual is within
+ * 9u||s|| (u = 2^-53, s = |b| + Σ|terms of Aθ| per cell), the error bound
+ * of its own FP64 evaluation (h3d_last_solve_floor then gives 1), and it
+ * stops unconverged after two corrections in a row that did not halve it.
+ * No counterpart in the reference: its solvers test the recursive residual. */
 int h3d_pbicgstab(h3d_ctx *ctx, double dt, double tol, int maxItr, int smoother,
                   int is_steady, int precision, double inner_tol,
                   int *isconverged, int *itr, double *res0);
@@ -160,6 +165,10 @@ int h3d_get_history(const h3d_ctx *ctx, double *res, int cap, int *n);
 /* Device time of the last h3d_pbicgstab/h3d_cg call (HIP events on the
  * context stream, initial residual to final residual). */
 int h3d_last_solve_ms(const h3d_ctx *ctx, double *ms);
+/* 1 when the last H3D_FP32_REFINED solve converged by reaching the rounding
+ * floor of its FP64 residual rather than tol, else 0.  No counterpart in
+ * the reference. */
+int h3d_last_solve_floor(const h3d_ctx *ctx, int *at_floor);
 
 /* Single operators on fields, for unit tests and custom drivers. */
 int h3d_calc_ax(h3d_ctx *ctx, int ax, int x, double dt, int is_steady);

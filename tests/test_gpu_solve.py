@@ -172,11 +172,34 @@ def test_fp32_refined_reaches_fp64_accuracy(oracle):
     wk.calc_rhs(10000.0, True)
     conv, itr, res0 = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-8, 20000, 1, True, precision=L.FP32_REFINED)
     hist = wk.history()
-    assert conv and hist[-1] < 1e-8
+    assert conv and hist[-1] < 1e-8 and not wk.last_solve_floor()
     th32 = wk.download("theta")
     c, conv_o, itr_o, res0_o, _ = oracle_solve(oracle, 40, 40, 30, "pbicgstab", 1, 1e-12, exact=False)
     assert abs(res0 - res0_o) <= 1e-13 * res0_o
     assert np.max(np.abs(th32 - c.arr["theta"])) < 1e-5
+    wk.close()
+
+
+def test_fp32_refined_stops_at_rounding_floor(oracle):
+    """A tolerance below what FP64 can resolve (1e-30): the refinement must
+    stop at the rounding floor of its FP64 residual, report convergence
+    there, and not spend the iteration budget; the answer matches a tight
+    FP64 solve of the oracle."""
+    wk, m = device_case(40, 40, 30)
+    wk.calc_rhs(10000.0, True)
+    conv, itr, res0 = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 20000, 1, True, precision=L.FP32_REFINED)
+    hist = wk.history()
+    assert conv and wk.last_solve_floor()
+    assert itr < 5000 and len(hist) < 20
+    assert 1e-30 < hist[-1] < 1e-10
+    th32 = wk.download("theta")
+    c, conv_o, itr_o, res0_o, _ = oracle_solve(oracle, 40, 40, 30, "pbicgstab", 1, 1e-12, exact=False)
+    assert np.max(np.abs(th32 - c.arr["theta"])) < 1e-6
+    # an FP64 solve afterwards clears the flag
+    wk.fill("b", 0.0)
+    wk.calc_rhs(10000.0, True)
+    wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-6, 20000, 1, True)
+    assert not wk.last_solve_floor()
     wk.close()
 
 
