@@ -32,7 +32,7 @@ EXPORTS = [
     "h3d_create", "h3d_destroy", "h3d_last_error", "h3d_create_error", "h3d_set_grid", "h3d_set_bc",
     "h3d_get_hc", "h3d_upload", "h3d_download", "h3d_fill", "h3d_copy", "h3d_apply_bc", "h3d_calc_rhs",
     "h3d_pbicgstab", "h3d_cg", "h3d_get_history", "h3d_last_solve_ms", "h3d_last_solve_floor",
-    "h3d_calc_ax", "h3d_calc_rk",
+    "h3d_calc_ax", "h3d_calc_rk", "h3d_residual_scale",
     "h3d_rbsor_color", "h3d_precond", "h3d_fdot1", "h3d_fdot2", "h3d_bicg1", "h3d_triad", "h3d_bicg2",
     "h3d_time_kernel", "h3d_set_option", "h3d_get_option",
     "h3d_objective", "h3d_adjoint_solve", "h3d_gradients", "h3d_get_adjoint",
@@ -70,6 +70,7 @@ _SIGS = {
     "h3d_last_solve_floor": (_c_int, [_vp, _ip]),
     "h3d_calc_ax": (_c_int, [_vp, _c_int, _c_int, _c_double, _c_int]),
     "h3d_calc_rk": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_double, _c_int, _dp]),
+    "h3d_residual_scale": (_c_int, [_vp, _c_int, _c_int, _c_double, _c_int, _dp]),
     "h3d_rbsor_color": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_double, _c_int]),
     "h3d_precond": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_double, _c_int]),
     "h3d_fdot1": (_c_int, [_vp, _c_int, _dp]),

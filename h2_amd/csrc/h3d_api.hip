@@ -1252,6 +1252,21 @@ int h3d_calc_rk(h3d_ctx *c, int r, int theta, int b, double dt, int is_steady, d
     return H3D_OK;
 }
 
+int h3d_residual_scale(h3d_ctx *c, int theta, int b, double dt, int is_steady, double *scale)
+{
+    if (!c || !scale) return H3D_ERR_ARG;
+    if (!field_ok(theta) || !field_ok(b) || theta == H3D_F_MASK || b == H3D_F_MASK)
+        return fail(c, H3D_ERR_ARG, "bad field");
+    Geo<double> g;
+    int rc = make_geo64(c, dt, is_steady, &g);
+    if (rc != H3D_OK) return rc;
+    HIPCHK(c, launch_rk_scale<double>(c->f[theta], c->f[b], g, c->d_unit, c->rb, c->stream));
+    double s = 0;
+    if ((rc = unit_reduce_result(c, &s, 1)) != H3D_OK) return rc;
+    *scale = std::sqrt(s);
+    return H3D_OK;
+}
+
 int h3d_rbsor_color(h3d_ctx *c, int x, int b, int color, double dt, int is_steady)
 {
     if (!c) return H3D_ERR_ARG;

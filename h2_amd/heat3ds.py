@@ -221,6 +221,14 @@ class WorkBuffers:
                                               int(bool(is_steady)), ctypes.byref(res)))
         return res.value
 
+    def residual_scale(self, theta, b, dt=1.0, is_steady=True):
+        """||s||, s = |b| + sum|terms of A theta| per cell: 2**-53 times it is
+        the rounding scale of the FP64 residual b - A theta."""
+        out = ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_residual_scale(self.ctx, self._fid(theta), self._fid(b), float(dt),
+                                                     int(bool(is_steady)), ctypes.byref(out)))
+        return out.value
+
     def rbsor_color(self, x, b, color, dt=1.0, is_steady=True):
         _check(self.ctx, self.lib.h3d_rbsor_color(self.ctx, self._fid(x), self._fid(b), int(color), float(dt),
                                                   int(bool(is_steady))))

@@ -45,7 +45,7 @@ class TransientRun:
 
     def __init__(self, m, dt=1.0e-3, nsteps=100, checkpoint_every=10, amp=0.5, period=20.0e-3, beta=BETA,
                  region=None, solver=L.SOLVER_PBICGSTAB, smoother=L.SMOOTHER_GS, tol=1e-8, precision=L.FP64,
-                 adjoint_precision=None, max_itr=20000, device=0, verbose=False):
+                 adjoint_precision=None, max_itr=20000, device=0, verbose=False, true_residual=False):
         if nsteps < 1 or checkpoint_every < 1:
             raise ValueError("nsteps and checkpoint_every must be >= 1")
         self.m = m
@@ -65,6 +65,9 @@ class TransientRun:
         self.slot_seg = SLOT_CKPT + self.nseg + 1
         self.stats = {}
         self.verbose = verbose
+        # after each forward step: the true FP64 residual ||b - Aθ|| relative
+        # to the step's res0, and in units of its rounding scale 2^-53·||s||
+        self.true_residual = bool(true_residual)
         self._setup_count = 0     # bumped whenever the device fields are set up again
         self._ckpt_setup = -1     # the set-up whose forward run left checkpoints
 
@@ -92,16 +95,21 @@ class TransientRun:
         wk.upload("hsrc", self.hsrc)
         wk.store("hsrc", SLOT_HSRC)
 
-    def _step(self, n):
+    def _step(self, n, record=False):
         """θⁿ⁻¹ (in θ) → θⁿ; returns J(θⁿ) and the solve's iteration count."""
         wk = self.wk
         wk.load("hsrc", SLOT_HSRC, self.factor(n))
         wk.fill("b", 0.0)
         wk.calc_rhs(self.dt, False)
-        conv, itr, _ = wk.solve(self.solver, self.dt, self.tol, self.max_itr, self.smoother, False,
-                                precision=self.precision)
+        conv, itr, res0 = wk.solve(self.solver, self.dt, self.tol, self.max_itr, self.smoother, False,
+                                   precision=self.precision)
         if not conv:
             raise RuntimeError("step %d did not converge (%d iterations)" % (n, itr))
+        if record:
+            rt = wk.calc_rk("pcg_r", "theta", "b", self.dt, False)
+            ulp = math.ldexp(wk.residual_scale("theta", "b", self.dt, False), -53)
+            self.stats["true_residual"].append(
+                [n, rt / res0 if res0 > 0 else 0.0, rt / ulp if ulp > 0 else 0.0, bool(wk.last_solve_floor())])
         return wk.objective(self.region, self.beta), itr
 
     def forward(self, scales=None, hsrc=None, checkpoints=True):
@@ -111,12 +119,18 @@ class TransientRun:
         if checkpoints:
             wk.store("theta", SLOT_CKPT)
         J, iters = 0.0, []
+        self.stats["true_residual"] = []
         t0 = time.perf_counter()
         for n in range(1, self.nsteps + 1):
-            Jn, itr = self._step(n)
+            Jn, itr = self._step(n, record=self.true_residual)
             J += self.dt * Jn
             iters.append(itr)
-            self._log("forward step %d J %.9f itr %d" % (n, Jn, itr))
+            if self.true_residual:
+                _, rel, ulps, at_floor = self.stats["true_residual"][-1]
+                self._log("forward step %d J %.9f itr %d true res %.3e (%.2f u|s|%s)" %
+                          (n, Jn, itr, rel, ulps, ", at floor" if at_floor else ""))
+            else:
+                self._log("forward step %d J %.9f itr %d" % (n, Jn, itr))
             if checkpoints and (n % self.every == 0 or n == self.nsteps):
                 wk.store("theta", SLOT_CKPT + (n + self.every - 1) // self.every)
         self.stats["forward_s"] = time.perf_counter() - t0
@@ -265,6 +279,8 @@ def main(argv=None):
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp32")
     ap.add_argument("--solver", choices=("pbicgstab", "cg"), default="pbicgstab")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--true-residual", action="store_true",
+                    help="log each step's true FP64 residual and its rounding scale")
     ap.add_argument("--fd-groups", type=str, default="", help="comma-separated groups for central differences")
     ap.add_argument("--fd-step", type=float, default=1e-6)
     ap.add_argument("--out", type=str, default="")
@@ -276,13 +292,15 @@ def main(argv=None):
     prec = L.FP32_REFINED if a.precision == "fp32" else L.FP64
     solver = L.SOLVER_CG if a.solver == "cg" else L.SOLVER_PBICGSTAB
     run = TransientRun(m, dt=a.dt, nsteps=a.steps, checkpoint_every=a.every, tol=a.tol, precision=prec,
-                       solver=solver, verbose=True)
+                       solver=solver, verbose=True, true_residual=a.true_residual)
     used0, total = run.wk.mem_info()
     J = run.forward()
     out = {"grid": [a.nx, a.ny, a.nz], "steps": a.steps, "dt": a.dt, "checkpoint_every": a.every, "tol": a.tol,
            "precision": a.precision, "solver": a.solver, "J_total": J, "forward_s": run.stats["forward_s"],
            "forward_iters": run.stats["forward_iters"], "device_mem_before_run": used0,
            "device_mem_after_forward": run.stats["mem_forward"], "device_total": total}
+    if a.true_residual:
+        out["true_residual"] = run.stats["true_residual"]
     if not a.no_adjoint:
         g = run.adjoint()
         out.update({"adjoint_s": run.stats["adjoint_s"], "adjoint_iters": run.stats["adjoint_iters"],

@@ -173,6 +173,11 @@ int h3d_last_solve_floor(const h3d_ctx *ctx, int *at_floor);
 /* Single operators on fields, for unit tests and custom drivers. */
 int h3d_calc_ax(h3d_ctx *ctx, int ax, int x, double dt, int is_steady);
 int h3d_calc_rk(h3d_ctx *ctx, int r, int theta, int b, double dt, int is_steady, double *res);
+/* ||s||_2 with s = |b| + Σ|terms of Aθ| per interior cell, the terms in the
+ * current operator form: 2^-53·||s|| is the scale of the rounding error of
+ * the FP64 residual b - Aθ (the refinement's floor test).  No counterpart
+ * in the reference. */
+int h3d_residual_scale(h3d_ctx *ctx, int theta, int b, double dt, int is_steady, double *scale);
 int h3d_rbsor_color(h3d_ctx *ctx, int x, int b, int color, double dt, int is_steady);
 int h3d_precond(h3d_ctx *ctx, int xx, int bb, int smoother, double dt, int is_steady);
 int h3d_fdot1(h3d_ctx *ctx, int x, double *out);

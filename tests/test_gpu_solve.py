@@ -203,6 +203,42 @@ def test_fp32_refined_stops_at_rounding_floor(oracle):
     wk.close()
 
 
+@pytest.mark.parametrize("form", [L.FORM_REFERENCE, L.FORM_FLUX])
+def test_residual_scale_terms(form):
+    """||s||, s = |b| + sum|terms of A theta|: with theta = 0 it is ||b||;
+    with one unit cell and b = 0 every s entry is one term of A e_c (the
+    diagonal at c, a conductance at each neighbour), so it equals ||A e_c||
+    bitwise, at an interior cell and at cells on convective and isothermal
+    faces; it is homogeneous and bounds ||b - A theta||."""
+    import math
+    wk, m = device_case(40, 40, 30)
+    wk.set_operator_form(form)
+    sh = m.shape
+    rng = np.random.default_rng(3)
+    b = np.asfortranarray(rng.normal(0.0, 1e-3, sh))
+    wk.upload("b", b)
+    wk.upload("theta", np.zeros(sh))
+    inner = b[1:-1, 1:-1, 1:-1].ravel()
+    assert wk.residual_scale("theta", "b", 10000.0, True) == math.sqrt(math.fsum(inner * inner))
+    wk.fill("b", 0.0)
+    for c in [(20, 20, 15), (1, 7, 1), (40, 40, 30), (5, 1, 30)]:
+        e = np.zeros(sh, order="F")
+        e[c] = 1.0
+        wk.upload("theta", e)
+        s = wk.residual_scale("theta", "b", 10000.0, True)
+        r = wk.calc_rk("pcg_r", "theta", "b", 10000.0, True)
+        assert s == r and s > 0
+    th = np.asfortranarray(rng.uniform(290.0, 310.0, sh))
+    wk.upload("theta", th)
+    wk.upload("b", b)
+    s1 = wk.residual_scale("theta", "b", 10000.0, False)
+    assert wk.calc_rk("pcg_r", "theta", "b", 10000.0, False) <= s1
+    wk.upload("theta", 2.0 * th)
+    wk.upload("b", 2.0 * b)
+    assert wk.residual_scale("theta", "b", 10000.0, False) == 2.0 * s1
+    wk.close()
+
+
 def test_q3d_drop_in(capsys):
     r = h2_amd.q3d(40, 40, 30, "pbicgstab", "gs", epsilon=1e-6, par="hip", is_steady=True)
     out = capsys.readouterr().out
