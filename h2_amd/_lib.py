@@ -21,6 +21,7 @@ FP64, FP32_REFINED = 0, 1
 OPT_COEF_FORM, OPT_NUM_CLASSES, OPT_LAYOUT = 0, 1, 2
 LAYOUT_AUTO, LAYOUT_STANDARD = 0, 1
 OPT_OPERATOR_FORM, FORM_REFERENCE, FORM_FLUX = 3, 0, 1
+OPT_SWEEP_CHUNK = 4
 COEF_AUTO, COEF_DIRECT = 0, 1
 
 FIELDS = ["theta", "b", "mask", "rho", "lam", "cp", "pcg_p", "pcg_p_", "pcg_r", "pcg_r0",

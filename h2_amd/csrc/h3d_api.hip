@@ -68,6 +68,9 @@ struct h3d_ctx {
     // whenever the class table form is active (H3D_OPT_LAYOUT)
     int layout = H3D_LAYOUT_AUTO;
     int op_form = H3D_FORM_REFERENCE;     // H3D_OPT_OPERATOR_FORM of the FP64 operator
+    int sweep_chunk = -1;                 // H3D_OPT_SWEEP_CHUNK
+    double *rbswap64 = nullptr;           // second buffer of the fused (out-of-place) sweeps
+    float *rbswap32 = nullptr;
     unsigned long grid_version = 1;       // bumped by h3d_set_grid / h3d_set_bc
     unsigned long rb_lam = 0, rb_grid = 0; // class build and grid version the RB tables were built for
     std::vector<double> hZC, hdZ;         // host copies of the Z metrics
@@ -332,6 +335,10 @@ struct Path {
     RbGeo<T> rg{};
     bool rb = false;
     size_t n = 0;  // Krylov vector length
+    // colour-split path: z-chunk of the two-colour sweep launches (0 = one
+    // launch per colour) and the second buffer they alternate with
+    int chunk = 0;
+    T *swap = nullptr;
 };
 
 template <typename T>
@@ -389,6 +396,16 @@ hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T
              : launch_rbsor_first<T>(mode, xx, bb, r, q, P.g, st, s);
     if (e != hipSuccess) return e;
     if ((e = p_sweep<T>(P, xx, bb, 1, st, s)) != hipSuccess) return e;
+    if (P.rb && P.chunk > 0) {
+        // sweeps 2-5, each colour 0 (half 1) then colour 1 (half 0) in one
+        // launch, out of place: xx -> swap -> xx -> swap -> xx
+        T *src = xx, *dst = P.swap;
+        for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
+            e = launch_rb_sweep2<T>(src, dst, bb, P.rg, 1, P.chunk, st, s);
+            std::swap(src, dst);
+        }
+        return e;
+    }
     for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
         e = p_sweep<T>(P, xx, bb, 0, st, s);
         if (e == hipSuccess) e = p_sweep<T>(P, xx, bb, 1, st, s);
@@ -490,8 +507,9 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
     if ((rc = reset_state(c, tol, maxItr, v.xbest != nullptr)) != H3D_OK) return rc;
     // The Krylov vectors keep a zero halo, as in the reference where only
     // their interiors are ever written (WorkBuffers starts them at zero).
-    T *vecs[] = {v.p, v.p_, v.r, v.r0, v.q, v.s, v.s_, v.t};
-    for (T *a : vecs) HIPCHK(c, launch_fill<T>(a, T(0), P.n, s));
+    T *vecs[] = {v.p, v.p_, v.r, v.r0, v.q, v.s, v.s_, v.t, P.swap};
+    for (T *a : vecs)
+        if (a) HIPCHK(c, launch_fill<T>(a, T(0), P.n, s));
     // res0 = CalcRK!(r, θ, b, ...); PBiCGSTAB also copies r0 = r
     const bool bicg = (solver == H3D_SOLVER_PBICGSTAB);
     HIPCHK(c, p_calc_rk<T>(P, v.r, bicg ? v.r0 : nullptr, v.th, v.b, bicg ? FIN_RES0_BICG : FIN_RES0, c->d_state,
@@ -622,6 +640,26 @@ int make_rbgeo(h3d_ctx *c, const Geo<T> &g, RbGeo<T> *out)
 }
 
 // The path a solve runs on: RB when the class table form is active.
+// The second buffer of the two-colour sweeps, zeroed when allocated (its
+// halo is never written and must equal the vectors' zero halo).
+template <typename T>
+T **swap_slot(h3d_ctx *c);
+template <>
+double **swap_slot<double>(h3d_ctx *c) { return &c->rbswap64; }
+template <>
+float **swap_slot<float>(h3d_ctx *c) { return &c->rbswap32; }
+
+// z-chunk of the two-colour sweeps for the operator g (0: one launch per
+// colour).  Chosen from the grid unless H3D_OPT_SWEEP_CHUNK fixes it; not
+// used when the tables of four planes would not fit the LDS budget.
+template <typename T>
+int sweep_chunk(const h3d_ctx *c, const RbGeo<T> &g)
+{
+    if (c->sweep_chunk == 0 || g.K < 1 || rb_sweep2_lds_bytes(g) > kSweep2LdsMax) return 0;
+    if (c->sweep_chunk > 0) return std::min(std::min(c->sweep_chunk, g.NZ), kSweep2ChunkMax);
+    return sweep_chunk_auto(g.HX, g.NY, g.NZ, (int)sizeof(T));
+}
+
 template <typename T>
 int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
 {
@@ -632,6 +670,15 @@ int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
         int rc = make_rbgeo<T>(c, g, &P->rg);
         if (rc != H3D_OK) return rc;
         P->n = 2 * c->HS;
+        P->chunk = sweep_chunk<T>(c, P->rg);
+        if (P->chunk > 0) {
+            T **sw = swap_slot<T>(c);
+            if (!*sw) {
+                HIPCHK(c, hipMalloc(sw, P->n * sizeof(T)));
+                HIPCHK(c, launch_fill<T>(*sw, T(0), P->n, c->stream));
+            }
+            P->swap = *sw;
+        }
     }
     return H3D_OK;
 }
@@ -976,6 +1023,8 @@ int h3d_destroy(h3d_ctx *c)
         if (c->rb64[a]) (void)hipFree(c->rb64[a]);
         if (c->rb32[a]) (void)hipFree(c->rb32[a]);
     }
+    if (c->rbswap64) (void)hipFree(c->rbswap64);
+    if (c->rbswap32) (void)hipFree(c->rbswap32);
     void *rbs[] = {c->d_cls_rb, c->d_ZF64,  c->d_ZF32,      c->d_region,
                    c->d_seed,   c->d_mu,    c->d_save[0],   c->d_save[1],   c->d_bmax,     c->d_scalar};
     for (void *p : rbs)
@@ -1019,6 +1068,12 @@ int h3d_set_option(h3d_ctx *c, int option, int value)
         if (value != H3D_FORM_REFERENCE && value != H3D_FORM_FLUX) return fail(c, H3D_ERR_ARG, "bad form");
         c->op_form = value;
         c->grid_version++;  // captured iterations carry the form
+        return H3D_OK;
+    }
+    if (option == H3D_OPT_SWEEP_CHUNK) {
+        if (value < -1) return fail(c, H3D_ERR_ARG, "bad sweep chunk");
+        c->sweep_chunk = value;
+        c->grid_version++;  // captured iterations carry the sweep sequence
         return H3D_OK;
     }
     return fail(c, H3D_ERR_ARG, "unknown option");
@@ -1335,7 +1390,7 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
 {
     if (!c) return H3D_ERR_ARG;
     if (!ms || reps < 1) return fail(c, H3D_ERR_ARG, "bad arguments");
-    if (which < 0 || which > 4) return fail(c, H3D_ERR_ARG, "unknown kernel id");
+    if (which < 0 || which > 6) return fail(c, H3D_ERR_ARG, "unknown kernel id");
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
@@ -1347,15 +1402,23 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
     double *x = w[1], *b = w[0], *q = w[4], *r0 = w[3];
     Path<float> P32;
     float *w32[8] = {};
-    if (which >= 3) {
+    const bool fp32 = which == 3 || which == 4 || which == 6;
+    if (which == 5 && P.chunk < 1) return fail(c, H3D_ERR_ARG, "no two-colour sweep for this operator");
+    if (fp32) {
         Geo<float> g32{};
         if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;
         if ((rc = make_path<float>(c, g32, &P32)) != H3D_OK) return rc;
         if ((rc = path_vecs<float>(c, P32, w32)) != H3D_OK) return rc;
         HIPCHK(c, launch_convert<float, double>(w32[1], x, P.n, c->stream));
         HIPCHK(c, launch_convert<float, double>(w32[0], b, P.n, c->stream));
+        if (which == 6 && P32.chunk < 1) return fail(c, H3D_ERR_ARG, "no two-colour sweep for this operator");
     }
     auto once = [&]() -> hipError_t {
+        // one full sweep as one two-colour launch, x -> swap (the result is
+        // not copied back: every launch computes the same sweep)
+        if (which == 5) return launch_rb_sweep2<double>(x, P.swap, b, P.rg, 1, P.chunk, c->d_unit, c->stream);
+        if (which == 6)
+            return launch_rb_sweep2<float>(w32[1], P32.swap, w32[0], P32.rg, 1, P32.chunk, c->d_unit, c->stream);
         if (which == 3) {
             hipError_t e = p_sweep<float>(P32, w32[1], w32[0], 0, c->d_unit, c->stream);
             return e != hipSuccess ? e : p_sweep<float>(P32, w32[1], w32[0], 1, c->d_unit, c->stream);

@@ -276,6 +276,21 @@ hipError_t launch_rb_calc_rk(T *r, T *r0copy, const T *th, const T *b, const RbG
                              double *hist, const RedBuf &rb, hipStream_t s);
 template <typename T>
 hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s);
+// Colour passes of halves hA then 1 - hA in one launch, xin -> xout (both
+// whole RB arrays with equal halos), z-chunks of C planes; the result is
+// bitwise that of the two in-place passes.  Dynamic LDS: rb_sweep2_lds_bytes.
+template <typename T>
+hipError_t launch_rb_sweep2(const T *xin, T *xout, const T *b, const RbGeo<T> &g, int hA, int C, const KState *st,
+                            hipStream_t s);
+template <typename T>
+size_t rb_sweep2_lds_bytes(const RbGeo<T> &g);
+// LDS budget of the two-colour sweep (above it the per-colour launches run)
+constexpr size_t kSweep2LdsMax = 64 * 1024;
+// longest z-chunk of the two-colour sweep
+constexpr int kSweep2ChunkMax = 125;
+// z-chunk the two-colour sweep uses on an HX x NY x NZ colour-split grid of
+// elem_bytes-sized values when none is set (0: the per-colour launches)
+int sweep_chunk_auto(int HX, int NY, int NZ, int elem_bytes);
 template <typename T>
 hipError_t launch_rb_first(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const KState *st,
                            hipStream_t s);

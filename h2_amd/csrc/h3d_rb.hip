@@ -71,32 +71,39 @@ __device__ __forceinline__ RbThread rb_thread(const RbGeo<T> &g)
     return r;
 }
 
-// Stage into LDS the conductance tables of the workgroup's plane k, each
-// 2K² entries indexed [(m ? K : 0) + neighbour class][centre class]:
+// The conductance tables of plane k into t (6K² + K entries), threads
+// tid = 0..nthr-1 sharing the work, no barrier: XT and YT, each 2K²
+// entries indexed [(m ? K : 0) + neighbour class][centre class],
 //   XT = ((λf * dy0) * ΔZ[k]) * ddx,   YT = ((λf * dx0) * ΔZ[k]) * ddy,
 // the products CalcAX! forms per face, in its order; then the z-face
 // conductances of faces k-1 and k (K² each); for a transient operator the
 // K values ap0 = ((((ρ·cp)·dx)·dy)·ΔZ[k])·ddt of the plane, formed in FP64
-// as k_ap0 forms them per cell and rounded to T.  All threads must call.
-// Kernels issue their loads of x, b and the class bytes first and stage
-// afterwards, so that the table loads (L2 hits) and the barrier overlap
-// the HBM latency instead of preceding it.
+// as k_ap0 forms them per cell and rounded to T.
 template <typename T>
-__device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g, int k)
+__device__ __forceinline__ void rb_stage_into(const RbGeo<T> &g, int k, T *t, int tid, int nthr)
 {
-    extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
-    T *t = reinterpret_cast<T *>(h3d_dyn_smem);
     const int n = 2 * g.K * g.K;
     const T dz_k = g.dZ[k];
     const T *zf = g.ZF + (size_t)(k - 1) * g.K * g.K;
-    const int tid = threadIdx.x + BX * threadIdx.y;
-    for (int a = tid; a < n; a += NT) {
+    for (int a = tid; a < n; a += nthr) {
         const T f = g.tab[a];
         t[a] = f * g.dy0 * dz_k * g.ddx;
         t[n + a] = f * g.dx0 * dz_k * g.ddy;
         t[2 * n + a] = zf[a];
     }
     if (g.ap && tid < g.K) t[3 * n + tid] = (T)(g.capc[tid] * g.dx0d * g.dy0d * g.dZd[k] * g.ddt);
+}
+
+// The tables of the workgroup's plane k into LDS, then an LDS barrier; all
+// threads must call.  Kernels issue their loads of x, b and the class
+// bytes first and stage afterwards, so that the table loads (L2 hits) and
+// the barrier overlap the HBM latency instead of preceding it.
+template <typename T>
+__device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g, int k)
+{
+    extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
+    T *t = reinterpret_cast<T *>(h3d_dyn_smem);
+    rb_stage_into(g, k, t, (int)(threadIdx.x + BX * threadIdx.y), NT);
     // LDS-only: the kernels' HBM loads, issued before the staging, stay
     // in flight across the barrier
     lds_barrier();
@@ -104,9 +111,15 @@ __device__ __forceinline__ const T *rb_stage(const RbGeo<T> &g, int k)
 }
 
 template <typename T>
+static size_t rb_table_entries(const RbGeo<T> &g)
+{
+    return (size_t)6 * g.K * g.K + (size_t)g.K;
+}
+
+template <typename T>
 static size_t rb_lds(const RbGeo<T> &g)
 {
-    return ((size_t)6 * g.K * g.K + (size_t)g.K) * sizeof(T);
+    return rb_table_entries(g) * sizeof(T);
 }
 
 // Byte offsets of a whole RB array (both halves) fit in 32 bits (the host
@@ -177,9 +190,13 @@ __device__ __forceinline__ Cls rb_cls(const uint8_t *__restrict__ clo, const uin
 }
 
 // The conductances of cell (i,j,k) from its classes and the staged tables;
-// the same values conductances() forms in h3d_kernels.hip.
-template <typename T>
-__device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, const Cls &q, int i, int j, int k)
+// the same values conductances() forms in h3d_kernels.hip.  ΔZ[k] comes
+// from dzk(), called for boundary cells only.  (A separate interior path,
+// which skips the convective zeros in the diagonal, measured slower at
+// 240x240x30: there every wave holds an i = 1 or i = NX cell and ran both.)
+template <typename T, typename DZ>
+__device__ __forceinline__ Cond<T> rb_cond_dz(const RbGeo<T> &g, const T *stab, const Cls &q, int i, int j, int k,
+                                              DZ dzk)
 {
     const int K = g.K, n2 = 2 * K * K;
     const int b = q.c;
@@ -195,7 +212,7 @@ __device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, con
     // convective terms: exact zeros away from the boundary (skipped there)
     o.cxm = o.cxp = o.cym = o.cyp = o.czm = o.czp = T(0);
     if (!(ixm && ixp && iym && iyp && k > 1 && k < g.NZ)) {
-        const T dz_k = g.dZ[k];
+        const T dz_k = dzk();
         o.cxm = ixm ? T(0) : g.HC[0] * g.dy0 * dz_k * T(1);
         o.cxp = ixp ? T(0) : g.HC[1] * g.dy0 * dz_k * T(1);
         o.cym = iym ? T(0) : g.HC[2] * g.dx0 * dz_k * T(1);
@@ -205,6 +222,13 @@ __device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, con
     }
     o.ap0 = g.ap ? stab[3 * n2 + b] : T(0);
     return o;
+}
+
+// rb_cond_dz reading ΔZ[k] from global memory
+template <typename T>
+__device__ __forceinline__ Cond<T> rb_cond(const RbGeo<T> &g, const T *stab, const Cls &q, int i, int j, int k)
+{
+    return rb_cond_dz(g, stab, q, i, j, k, [&] { return g.dZ[k]; });
 }
 
 // ss = Σ cond_n x_n in the reference's order
@@ -396,6 +420,272 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
     hipLaunchKernelGGL((k_rb_sweep<T, R>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own, x + oth,
                        b + own, g.cls + own, g.cls + oth, g, h, st);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Two colours of rbsor_core! in one pass: half hA, then half hB = 1 - hA,
+// from xin into xout (whole RB arrays, both halves; their halos must be
+// equal, as the zero halos of the Krylov vectors are).
+//
+// A workgroup of S2X x S2Y threads covers a footprint of S2X half-row
+// indices by S2Y rows and owns its inner (S2X-2) x (S2Y-2) cells over a
+// chunk of planes [kb, ke], which it marches in z.  At step ka (from kb-1
+// to ke+1) every thread updates half A at its (m, j, ka) into the LDS plane
+// slot of ka; after the barrier the owned threads update half B at plane
+// ka-1 from the A planes ka-2, ka-1, ka in LDS, and store both halves of
+// plane ka-1.  The footprint's ring of A cells (one cell wide in m, j and,
+// through the chunk's first and last steps, k) belongs to neighbouring
+// workgroups, which compute the same values with the same operations.
+//
+// Every cell's update is k_rb_sweep's, so the result is bitwise that of the
+// two in-place colour passes.  The pass is out of place because a ring cell
+// must be computed from the old values its owner overwrites.  Compared with
+// the two passes it reads x and the class field once instead of twice and
+// launches once; it recomputes the ring, about (S2X·S2Y)/((S2X-2)(S2Y-2))
+// × (C+2)/C of the A updates for a chunk of C planes.
+//
+// Four LDS slots (plane index & 3) for the A planes, their class bytes and
+// the conductance tables let one barrier per step order everything: a slot
+// written at step ka was last read at step ka-2, before the barrier of step
+// ka-1.
+constexpr int S2X = 64, S2Y = 16, S2OX = S2X - 2, S2OY = S2Y - 2;
+// longest z-chunk; its planes kb-1..ke+1 keep their ΔZ in LDS, so the loop
+// issues no global load besides the prefetched ones (a vector load's wait
+// would also wait for them)
+constexpr int S2CMAX = kSweep2ChunkMax, S2ZMAX = S2CMAX + 3;
+
+template <typename T>
+struct S2Loads {
+    Nb<T> n;
+    T ppA, bcA, ppB, bcB;
+    T zf;  // this thread's z-face table entry of the plane after
+    Cls q;
+    int sA, clA, clB;
+};
+
+// rb_stage_into for a workgroup of at least 2K² threads, each holding its
+// λf table entry f (plane-independent), its z-face entry zf of plane k and,
+// for a transient operator, capc = ρ·cp of class tid (tid < K)
+// dz_k = ΔZ[k] and dzd_k = its FP64 value are read by the caller outside
+// any divergent branch (scalar loads, which do not wait for the vector
+// loads in flight).
+template <typename T>
+__device__ __forceinline__ void rb_stage_regs(const RbGeo<T> &g, T *t, int tid, T f, T zf, double capc, T dz_k,
+                                              double dzd_k)
+{
+    const int n = 2 * g.K * g.K;
+    if (tid < n) {
+        t[tid] = f * g.dy0 * dz_k * g.ddx;
+        t[n + tid] = f * g.dx0 * dz_k * g.ddy;
+        t[2 * n + tid] = zf;
+    }
+    if (g.ap && tid < g.K) t[3 * n + tid] = (T)(capc * g.dx0d * g.dy0d * dzd_k * g.ddt);
+}
+
+template <typename T>
+static size_t rb_sweep2_lds(const RbGeo<T> &g)
+{
+    return S2ZMAX * (sizeof(double) + sizeof(T)) + 4 * rb_table_entries(g) * sizeof(T) +
+           4 * (size_t)(S2X * S2Y) * (sizeof(T) + 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(S2X *S2Y) k_rb_sweep2(const T *__restrict__ xin, T *__restrict__ xout,
+                                                         const T *__restrict__ b, const uint8_t *__restrict__ cls,
+                                                         RbGeo<T> g, int hA, int C, unsigned nbx, unsigned nby,
+                                                         unsigned nch, const KState *st)
+{
+    if (halted(st)) return;
+    const unsigned nreal = nbx * nby * nch;
+    const unsigned nb = (nreal + 7u) & ~7u;
+    const unsigned p = blockIdx.x;
+    const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
+    if (L >= nreal) return;  // the whole workgroup
+    // m-tile fastest, then the row tile, then the chunk: an XCD's workgroups
+    // in flight are neighbours in m and j and share their rings in its L2
+    const unsigned tm = L % nbx, t1 = L / nbx, tj = t1 % nby, ch = t1 / nby;
+
+    extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
+    const int TB = 6 * g.K * g.K + g.K;
+    double *zd = reinterpret_cast<double *>(h3d_dyn_smem);  // [S2ZMAX] FP64 ΔZ of planes kb-1..
+    T *zt = reinterpret_cast<T *>(zd + S2ZMAX);              // [S2ZMAX] ΔZ in T
+    T *tabs = zt + S2ZMAX;                                   // [4][TB]
+    T *as = tabs + 4 * TB;                                   // [4][S2Y][S2X]
+    uint8_t *acl = reinterpret_cast<uint8_t *>(as + 4 * S2X * S2Y);
+
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + S2X * ty;
+    const int HX = g.HX, MY = g.MY, NX = g.NX, NY = g.NY, NZ = g.NZ;
+    const int mr = (int)tm * S2OX - 1 + tx, jr = (int)tj * S2OY + ty;
+    const int mc = min(max(mr, 0), HX - 1), jc = min(jr, MY - 1);
+    const bool col = (mr == mc) && (jr == jc);
+    const bool own = col && tx >= 1 && tx <= S2OX && ty >= 1 && ty <= S2OY;
+    // rows of the loads: a neighbour of any row in 1..NY is in the array
+    const int jl = min(max(jc, 1), NY);
+    const int kb = 1 + (int)ch * C, ke = min(kb + C - 1, NZ);
+    const int hB = 1 - hA;
+    const unsigned HXu = (unsigned)HX, HXY = HXu * (unsigned)MY;
+    const size_t oA = hA ? g.HS : 0, oB = hB ? g.HS : 0;
+    const T *xA = xin + oA, *xB = xin + oB, *bA = b + oA, *bB = b + oB;
+    T *yA = xout + oA, *yB = xout + oB;
+    const uint8_t *cA = g.cls + oA, *cB = g.cls + oB;
+    const int slot0 = tid;  // (ty, tx) within a plane slot
+
+    // table entries this thread stages (2K² <= S2X·S2Y: the host keeps K
+    // within the LDS budget, which is far smaller)
+    const int KK = g.K * g.K, ntab = min(tid, 2 * KK - 1);
+    // the global loads of step ka: the A cell at plane ka (at its load
+    // position (mc, jl, kl), whose neighbours are all in the array, and at
+    // its true position for the value kept where it is not updated) and
+    // the B cell at plane ka-1; issued one step ahead of their use
+    auto load = [&](int ka, S2Loads<T> &v) {
+        const int kl = min(max(ka, 1), NZ);
+        v.sA = (hA + jl + kl) & 1;
+        // (the loop's last steps load planes past the chunk, clamped into
+        // the array, and use nothing of them)
+        const unsigned ra = (unsigned)mc + HXu * (unsigned)jc + HXY * (unsigned)min(ka, g.MZ - 1);
+        const unsigned rl = (unsigned)mc + HXu * (unsigned)jl + HXY * (unsigned)kl;
+        v.n = rb_nb(xB, rl, v.sA, HXu, HXY);
+        v.ppA = xA[ra];
+        v.bcA = bA[rl];
+        v.q = rb_cls(cA, cB, rl, rl, v.sA, HXu, HXY);
+        v.clA = cA[ra];
+        const unsigned rb_ = (unsigned)mc + HXu * (unsigned)jl + HXY * (unsigned)min(max(ka - 1, 1), NZ);
+        v.ppB = xB[rb_];
+        v.bcB = bB[rb_];
+        v.clB = cB[rb_];
+        v.zf = g.ZF[(size_t)(min(ka + 1, NZ) - 1) * KK + ntab];
+    };
+    const T ftab = g.tab[ntab];
+    const double capc = g.ap ? g.capc[min(tid, g.K - 1)] : 0.0;
+    if (tid <= ke - kb + 2) {
+        zd[tid] = g.dZd[kb - 1 + tid];
+        zt[tid] = g.dZ[kb - 1 + tid];
+    }
+    S2Loads<T> v0, v1;
+    load(kb - 1, v0);
+    lds_barrier();
+    if (kb - 1 >= 1)
+        rb_stage_regs(g, tabs + ((kb - 1) & 3) * TB, tid, ftab, g.ZF[(size_t)(kb - 2) * KK + ntab], capc, zt[0],
+                      zd[0]);
+    lds_barrier();
+    // step ka uses cur (loaded one step earlier) and loads step ka+1 into
+    // nxt; the loop alternates the two register sets instead of copying
+    // (a copy would wait for the loads in flight)
+    auto step = [&](int ka, const S2Loads<T> &cur, S2Loads<T> &nxt) {
+        const int kt = min(ka + 1, min(NZ, ke + 1));  // the plane whose table this step stages
+        const T dz_t = zt[kt - (kb - 1)];
+        const double dzd_t = zd[kt - (kb - 1)];
+        load(ka + 1, nxt);  // unconditional: a skipped load would make later waits drain everything
+        const int kl = min(max(ka, 1), NZ);
+        const int sA = cur.sA;
+        const Nb<T> n = cur.n;
+        const T ppA = cur.ppA, bcA = cur.bcA, ppB = cur.ppB, bcB = cur.bcB;
+        const Cls q = cur.q;
+        const uint8_t clA = (uint8_t)cur.clA;
+        const int clB = cur.clB;
+        const int kB = ka - 1;
+        const bool doB = own && kB >= kb && kB <= ke;
+        const unsigned rb_ = (unsigned)mc + HXu * (unsigned)jl + HXY * (unsigned)min(max(kB, 1), NZ);
+        // ---- the table of plane ka+1, used from the next step on
+        if (ka + 1 >= 1 && ka + 1 <= NZ && ka + 1 <= ke + 1)
+            rb_stage_regs(g, tabs + ((ka + 1) & 3) * TB, tid, ftab, cur.zf, capc, dz_t, dzd_t);
+        // ---- A at (mc, jc, ka) into its slot (the old value where the cell
+        // is not an interior cell of this footprint position)
+        const int iA = 2 * mc + sA;
+        const bool validA = col && jc == jl && ka == kl && ka <= ke + 1 && iA >= 1 && iA <= NX;
+        T anew = ppA;
+        if (validA) {
+            const T *stab = tabs + (ka & 3) * TB;
+            const Cond<T> o = rb_cond_dz(g, stab, q, iA, jc, ka, [&] { return zt[ka - (kb - 1)]; });
+            const T dd = diag(o);
+            if (g.flux) {
+                anew = ppA + (rb_flux(o, n, ppA) - bcA) / dd;
+            } else {
+                const T dp = (((rb_offdiag(o, n) - bcA) / dd - ppA)) * T(1);
+                anew = ppA + T(1) * dp;
+            }
+        }
+        as[(ka & 3) * (S2X * S2Y) + slot0] = anew;
+        acl[(ka & 3) * (S2X * S2Y) + slot0] = clA;
+        lds_barrier();
+        // ---- B at (mc, jc, kB) from the A planes kB-1..kB+1
+        if (doB) {
+            const int sB = (hB + jc + kB) & 1;
+            const int iB = 2 * mc + sB;
+            const T *a0 = as + (kB & 3) * (S2X * S2Y);
+            const uint8_t *c0 = acl + (kB & 3) * (S2X * S2Y);
+            const int zm = ((kB - 1) & 3) * (S2X * S2Y) + slot0, zp = ((kB + 1) & 3) * (S2X * S2Y) + slot0;
+            Nb<T> nb2;
+            nb2.xm = a0[slot0 - 1 + sB];
+            nb2.xp = a0[slot0 + sB];
+            nb2.ym = a0[slot0 - S2X];
+            nb2.yp = a0[slot0 + S2X];
+            nb2.zm = as[zm];
+            nb2.zp = as[zp];
+            Cls qb;
+            qb.c = clB;
+            qb.xm = c0[slot0 - 1 + sB];
+            qb.xp = c0[slot0 + sB];
+            qb.ym = c0[slot0 - S2X];
+            qb.yp = c0[slot0 + S2X];
+            qb.zm = acl[zm];
+            qb.zp = acl[zp];
+            if (jc == jl && iB >= 1 && iB <= NX) {
+                const T *stab = tabs + (kB & 3) * TB;
+                const Cond<T> o = rb_cond_dz(g, stab, qb, iB, jc, kB, [&] { return zt[kB - (kb - 1)]; });
+                const T dd = diag(o);
+                T xn;
+                if (g.flux) {
+                    xn = ppB + (rb_flux(o, nb2, ppB) - bcB) / dd;
+                } else {
+                    const T dp = (((rb_offdiag(o, nb2) - bcB) / dd - ppB)) * T(1);
+                    xn = ppB + T(1) * dp;
+                }
+                yB[rb_] = xn;
+            }
+            // the owned A cell of plane kB (interior cells only: halos stay)
+            const int sA2 = (hA + jc + kB) & 1;
+            const int iA2 = 2 * mc + sA2;
+            if (jc == jl && iA2 >= 1 && iA2 <= NX) yA[rb_] = a0[slot0];
+        }
+    };
+    // steps kb-1 .. ke+1 in pairs; an odd count ends with a step that
+    // updates and stores nothing
+    for (int ka = kb - 1; ka <= ke + 1; ka += 2) {
+        step(ka, v0, v1);
+        step(ka + 1, v1, v0);
+    }
+}
+
+template <typename T>
+hipError_t launch_rb_sweep2(const T *xin, T *xout, const T *b, const RbGeo<T> &g, int hA, int C, const KState *st,
+                            hipStream_t s)
+{
+    if (C < 1) return hipErrorInvalidValue;
+    const unsigned nbx = (unsigned)((g.HX + S2OX - 1) / S2OX), nby = (unsigned)((g.NY + S2OY - 1) / S2OY);
+    const unsigned nch = (unsigned)((g.NZ + C - 1) / C);
+    const unsigned nb = (nbx * nby * nch + 7u) & ~7u;
+    hipLaunchKernelGGL(k_rb_sweep2<T>, dim3(nb), dim3(S2X, S2Y), rb_sweep2_lds(g), s, xin, xout, b, g.cls, g, hA, C,
+                       nbx, nby, nch, st);
+    return hipGetLastError();
+}
+
+template <typename T>
+size_t rb_sweep2_lds_bytes(const RbGeo<T> &g)
+{
+    return rb_sweep2_lds(g);
+}
+
+// Measured on one MI355X (HIP events, one full sweep): FP64 two-colour
+// launches with 30-plane chunks take 969 us at 960x960x120 (transient),
+// against 1084 us for the per-colour launches; at 480x480x60 and
+// 240x240x30 every chunk length was slower (137 against 100 us, 24 against
+// 19 us), and in FP32 at every size (610 against 560 us at 960x960x120).
+int sweep_chunk_auto(int HX, int NY, int NZ, int elem_bytes)
+{
+    (void)HX;
+    (void)NY;
+    return (elem_bytes == 8 && NZ >= 120) ? 30 : 0;
 }
 
 // Colour 0 of the first sweep from x = 0, fused with forming its right-hand
@@ -874,6 +1164,9 @@ hipError_t launch_rb_copy_interior(T *dst, const T *src, const RbGeo<T> &g, cons
     template hipError_t launch_rb_calc_rk<T>(T *, T *, const T *, const T *, const RbGeo<T> &, int, KState *,    \
                                              double *, const RedBuf &, hipStream_t);                            \
     template hipError_t launch_rb_sweep<T>(T *, const T *, const RbGeo<T> &, int, const KState *, hipStream_t);  \
+    template hipError_t launch_rb_sweep2<T>(const T *, T *, const T *, const RbGeo<T> &, int, int, const KState *, \
+                                            hipStream_t);                                                       \
+    template size_t rb_sweep2_lds_bytes<T>(const RbGeo<T> &);                                                  \
     template hipError_t launch_rb_first<T>(int, T *, T *, const T *, const T *, const RbGeo<T> &, const KState *, \
                                            hipStream_t);                                                        \
     template hipError_t launch_rb_form_rhs<T>(int, T *, const T *, const T *, const RbGeo<T> &, const KState *,  \

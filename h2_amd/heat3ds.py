@@ -269,6 +269,12 @@ class WorkBuffers:
         """L.FORM_REFERENCE (the reference's expression, default) or L.FORM_FLUX."""
         _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_OPERATOR_FORM, int(form)))
 
+    def set_sweep_chunk(self, chunk):
+        """Red-black sweeps of the colour-split layout: 0 = one launch per
+        colour, C >= 1 = both colours in one launch over z-chunks of C
+        planes, -1 = chosen from the grid (default).  Bitwise identical."""
+        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_SWEEP_CHUNK, int(chunk)))
+
     def num_classes(self):
         v = ctypes.c_int()
         _check(self.ctx, self.lib.h3d_get_option(self.ctx, L.OPT_NUM_CLASSES, ctypes.byref(v)))

@@ -90,6 +90,9 @@ extern "C" {
 #define H3D_FORM_FLUX 1        /* Σ c_n (x_n - x0) - (Σconv + ap0)·x0: the same operator with rounding
                                   relative to temperature differences instead of temperatures (used for
                                   finite-difference checks of the adjoint) */
+#define H3D_OPT_SWEEP_CHUNK 4  /* Gauss-Seidel sweeps of the colour-split layout: 0 = one launch per colour,
+                                  C >= 1 = the two colours of a sweep in one launch marching chunks of C
+                                  z-planes, -1 = chosen from the grid (default); bitwise identical */
 
 typedef struct h3d_ctx h3d_ctx;
 
@@ -263,7 +266,9 @@ int h3d_rbsor_solve(h3d_ctx *ctx, double dt, double omega, double tol, int maxIt
  * reps back-to-back launches after one warm-up launch; *ms = mean per
  * launch).  which: 0 = one full red-black GS sweep (both colours),
  * 1 = CalcAX!, 2 = CalcAX! with its fused dot; 3 and 4 = the FP32 operator's
- * sweep and CalcAX! (the kernels of the H3D_FP32_REFINED inner solve). */
+ * sweep and CalcAX! (the kernels of the H3D_FP32_REFINED inner solve);
+ * 5 and 6 = one full sweep as one two-colour launch, FP64 and FP32
+ * (H3D_OPT_SWEEP_CHUNK; H3D_ERR_ARG when the operator has none). */
 int h3d_time_kernel(h3d_ctx *ctx, int which, int reps, double dt, int is_steady, double *ms);
 
 #ifdef __cplusplus
