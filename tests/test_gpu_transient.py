@@ -98,3 +98,27 @@ def test_fp32_refined_steps_match_fp64():
     assert abs(J32 - J64) <= 1e-9 * abs(J64)
     assert np.max(np.abs(s32 - s64)) <= 1e-6 * np.max(np.abs(s64))
     assert np.max(np.abs(h32 - h64)) <= 1e-6 * np.max(np.abs(h64))
+
+
+def test_true_residual_log():
+    """With true_residual the forward run records, per step, the true
+    residual ||b - A theta|| / res0 and its size in units of 2^-53·||s||.
+    A converged FP64 step is at 1e-8 of res0 or within a few tens of
+    rounding units of it (the recursive residual drifts from the true one
+    by rounding); a refined step is below 1e-8 or reports the floor."""
+    m = h2_amd.build_model(40, 40, 30, power="map")
+    for prec in (L.FP64, L.FP32_REFINED):
+        run = TransientRun(m, dt=1e-3, nsteps=3, checkpoint_every=3, tol=1e-8, precision=prec,
+                           solver=L.SOLVER_CG, true_residual=True)
+        run.forward()
+        tr = run.stats["true_residual"]
+        print("precision", prec, tr)
+        assert [t[0] for t in tr] == [1, 2, 3]
+        for n, rel, ulps, at_floor in tr:
+            assert np.isfinite(rel) and np.isfinite(ulps) and ulps > 0
+            if prec == L.FP64:
+                assert rel < 1e-8 or ulps < 64
+                assert not at_floor
+            else:
+                assert rel < 1e-8 or (at_floor and ulps <= 9.0)
+        run.close()
