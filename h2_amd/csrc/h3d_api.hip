@@ -655,7 +655,7 @@ float **swap_slot<float>(h3d_ctx *c) { return &c->rbswap32; }
 template <typename T>
 int sweep_chunk(const h3d_ctx *c, const RbGeo<T> &g)
 {
-    if (c->sweep_chunk == 0 || g.K < 1 || rb_sweep2_lds_bytes(g) > kSweep2LdsMax) return 0;
+    if (c->sweep_chunk == 0 || !rb_sweep2_fits(g)) return 0;
     if (c->sweep_chunk > 0) return std::min(std::min(c->sweep_chunk, g.NZ), kSweep2ChunkMax);
     return sweep_chunk_auto(g.HX, g.NY, g.NZ, (int)sizeof(T));
 }

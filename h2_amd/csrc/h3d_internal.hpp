@@ -278,12 +278,13 @@ template <typename T>
 hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s);
 // Colour passes of halves hA then 1 - hA in one launch, xin -> xout (both
 // whole RB arrays with equal halos), z-chunks of C planes; the result is
-// bitwise that of the two in-place passes.  Dynamic LDS: rb_sweep2_lds_bytes.
+// bitwise that of the two in-place passes.  Only for operators for which
+// rb_sweep2_fits (the class count bounds its LDS tables).
 template <typename T>
 hipError_t launch_rb_sweep2(const T *xin, T *xout, const T *b, const RbGeo<T> &g, int hA, int C, const KState *st,
                             hipStream_t s);
 template <typename T>
-size_t rb_sweep2_lds_bytes(const RbGeo<T> &g);
+bool rb_sweep2_fits(const RbGeo<T> &g);
 // LDS budget of the two-colour sweep (above it the per-colour launches run)
 constexpr size_t kSweep2LdsMax = 64 * 1024;
 // longest z-chunk of the two-colour sweep

@@ -427,9 +427,9 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
 // from xin into xout (whole RB arrays, both halves; their halos must be
 // equal, as the zero halos of the Krylov vectors are).
 //
-// A workgroup of S2X x S2Y threads covers a footprint of S2X half-row
-// indices by S2Y rows and owns its inner (S2X-2) x (S2Y-2) cells over a
-// chunk of planes [kb, ke], which it marches in z.  At step ka (from kb-1
+// A workgroup of S2X x S2Y (64 x 8) threads covers a footprint of S2X
+// half-row indices by S2Y rows and owns its inner (S2X-2) x (S2Y-2) cells
+// over a chunk of planes [kb, ke], which it marches in z.  At step ka (from kb-1
 // to ke+1) every thread updates half A at its (m, j, ka) into the LDS plane
 // slot of ka; after the barrier the owned threads update half B at plane
 // ka-1 from the A planes ka-2, ka-1, ka in LDS, and store both halves of
@@ -442,13 +442,15 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
 // must be computed from the old values its owner overwrites.  Compared with
 // the two passes it reads x and the class field once instead of twice and
 // launches once; it recomputes the ring, about (S2X·S2Y)/((S2X-2)(S2Y-2))
-// × (C+2)/C of the A updates for a chunk of C planes.
+// × (C+2)/C of the A updates for a chunk of C planes.  A 64 x 16 footprint
+// (fewer ring cells, one 16-wave workgroup per CU) measured slower than
+// 64 x 8 (two 8-wave workgroups per CU, whose lockstep steps interleave).
 //
 // Four LDS slots (plane index & 3) for the A planes, their class bytes and
 // the conductance tables let one barrier per step order everything: a slot
 // written at step ka was last read at step ka-2, before the barrier of step
 // ka-1.
-constexpr int S2X = 64, S2Y = 16, S2OX = S2X - 2, S2OY = S2Y - 2;
+constexpr int S2X = 64, S2Y = 8, S2OX = S2X - 2, S2OY = S2Y - 2;
 // longest z-chunk; its planes kb-1..ke+1 keep their ΔZ in LDS, so the loop
 // issues no global load besides the prefetched ones (a vector load's wait
 // would also wait for them)
@@ -671,21 +673,24 @@ hipError_t launch_rb_sweep2(const T *xin, T *xout, const T *b, const RbGeo<T> &g
 }
 
 template <typename T>
-size_t rb_sweep2_lds_bytes(const RbGeo<T> &g)
+bool rb_sweep2_fits(const RbGeo<T> &g)
 {
-    return rb_sweep2_lds(g);
+    // the LDS budget, and one staged table entry per thread (rb_stage_regs)
+    return g.K >= 1 && rb_sweep2_lds(g) <= kSweep2LdsMax && 2 * g.K * g.K <= S2X * S2Y;
 }
 
-// Measured on one MI355X (HIP events, one full sweep): FP64 two-colour
-// launches with 30-plane chunks take 969 us at 960x960x120 (transient),
-// against 1084 us for the per-colour launches; at 480x480x60 and
-// 240x240x30 every chunk length was slower (137 against 100 us, 24 against
-// 19 us), and in FP32 at every size (610 against 560 us at 960x960x120).
+// Measured on one MI355X (HIP events, one full sweep, 64 x 8 footprint):
+// at 960x960x120 (transient) the two-colour launches with 60-plane chunks
+// take 933 us in FP64 and 520 us in FP32, against 1079 and 559 us for the
+// per-colour launches (40-plane chunks: 951 and 522 us; 120: 929 and
+// 527 us); at 480x480x60 and 240x240x30 every chunk length was slower
+// (best 132 against 96 us and 69 against 58 us at 480x480x60).
 int sweep_chunk_auto(int HX, int NY, int NZ, int elem_bytes)
 {
     (void)HX;
     (void)NY;
-    return (elem_bytes == 8 && NZ >= 120) ? 30 : 0;
+    (void)elem_bytes;
+    return NZ >= 120 ? 60 : 0;
 }
 
 // Colour 0 of the first sweep from x = 0, fused with forming its right-hand
@@ -1166,7 +1171,7 @@ hipError_t launch_rb_copy_interior(T *dst, const T *src, const RbGeo<T> &g, cons
     template hipError_t launch_rb_sweep<T>(T *, const T *, const RbGeo<T> &, int, const KState *, hipStream_t);  \
     template hipError_t launch_rb_sweep2<T>(const T *, T *, const T *, const RbGeo<T> &, int, int, const KState *, \
                                             hipStream_t);                                                       \
-    template size_t rb_sweep2_lds_bytes<T>(const RbGeo<T> &);                                                  \
+    template bool rb_sweep2_fits<T>(const RbGeo<T> &);                                                         \
     template hipError_t launch_rb_first<T>(int, T *, T *, const T *, const T *, const RbGeo<T> &, const KState *, \
                                            hipStream_t);                                                        \
     template hipError_t launch_rb_form_rhs<T>(int, T *, const T *, const T *, const RbGeo<T> &, const KState *,  \
