@@ -16,7 +16,8 @@
 // which does not change a sum's bits.
 //
 // Work decomposition: a workgroup is 64 (half-row index m) by 4 (rows j)
-// threads of one plane.  "Pair" kernels handle the two cells i = 2m and
+// threads of one plane (the two-colour sweep, k_rb_sweep2, marches a 64 x 8
+// footprint through a chunk of planes instead).  "Pair" kernels handle the two cells i = 2m and
 // 2m+1 of a thread (one in each half); the sweep handles the one cell of
 // the colour being updated.  Every thread computes from clamped, always
 // in-bounds indices and only its stores and sums are predicated on the
