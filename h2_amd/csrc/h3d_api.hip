@@ -1098,6 +1098,10 @@ int h3d_get_option(const h3d_ctx *c, int option, int *value)
         *value = c->op_form;
         return H3D_OK;
     }
+    if (option == H3D_OPT_SWEEP_CHUNK) {
+        *value = c->sweep_chunk;
+        return H3D_OK;
+    }
     return H3D_ERR_ARG;
 }
 

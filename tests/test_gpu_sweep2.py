@@ -5,6 +5,8 @@ hold bitwise to the oracle): same convergence flag, iteration count, res0,
 residual history and θ, for FP64 and FP32-refined solves, steady and
 transient operators, chunk lengths from 1 plane to the whole column, and
 grids whose extents are not multiples of the 62 x 14 tile or of the chunk."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -71,6 +73,8 @@ def test_timing_entry_points():
     with pytest.raises(L.H3DError):
         wk.time_kernel(5, 3)
     wk.set_sweep_chunk(8)
+    v = ctypes.c_int()
+    assert wk.lib.h3d_get_option(wk.ctx, L.OPT_SWEEP_CHUNK, ctypes.byref(v)) == 0 and v.value == 8
     assert wk.time_kernel(5, 3) > 0 and wk.time_kernel(6, 3) > 0
     with pytest.raises(L.H3DError):
         wk.set_sweep_chunk(-2)
