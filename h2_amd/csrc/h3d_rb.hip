@@ -409,17 +409,25 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
     }
 }
 
-// Rows per thread of the sweep: two for FP32 (measured faster at
-// 480x480x60 and 960x960x120), one for FP64 (two rows were slower at
-// 240x240x30 and equal at the larger grids).
+// Rows per thread of the sweep, by precision and grid size (measured, HIP
+// events, one full sweep): FP64 one row on small grids (17.7 against
+// 18.4 us at 240x240x30), two on large ones (91.0 against 95.8 us at
+// 480x480x60, 1025 against 1078 us at 960x960x120); FP32 two rows on
+// small grids (12.2 against 13.4 us with four), four on large ones (56.4
+// against 58.0 us, 540 against 559 us).  Large: at least 2^21 cells per
+// half.
 template <typename T>
 hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s)
 {
     constexpr int R = sizeof(T) == 4 ? 2 : 1;
     const int h = 1 - color;
     const size_t own = h ? g.HS : 0, oth = h ? 0 : g.HS;
-    hipLaunchKernelGGL((k_rb_sweep<T, R>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own, x + oth,
-                       b + own, g.cls + own, g.cls + oth, g, h, st);
+    if (g.HS >= ((size_t)1 << 21))
+        hipLaunchKernelGGL((k_rb_sweep<T, 2 * R>), dim3(rb_rows_nb<2 * R>(g)), dim3(BX, BY), rb_lds(g), s, x + own,
+                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+    else
+        hipLaunchKernelGGL((k_rb_sweep<T, R>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own, x + oth,
+                           b + own, g.cls + own, g.cls + oth, g, h, st);
     return hipGetLastError();
 }
 

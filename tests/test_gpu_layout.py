@@ -48,3 +48,30 @@ def test_fp32_refined_layouts_bitwise(shape):
     assert a[0] == b[0]
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])
+
+
+def run_limited(shape, layout, solver, steady, precision, max_itr):
+    m = h2_amd.build_model(*shape)
+    wk = WorkBuffers(*m.shape)
+    wk.set_layout(layout)
+    load_model(wk, m)
+    dt = 10000.0 if steady else 1e-3
+    wk.calc_rhs(dt, steady)
+    res = wk.solve(solver, dt, 1e-12, max_itr, L.SMOOTHER_GS, steady, precision=precision)
+    out = (res, wk.history(), wk.download("theta"))
+    wk.close()
+    return out
+
+
+@pytest.mark.parametrize("solver,steady,precision", [(L.SOLVER_PBICGSTAB, True, L.FP64),
+                                                     (L.SOLVER_CG, False, L.FP64),
+                                                     (L.SOLVER_CG, False, L.FP32_REFINED)])
+def test_large_grid_layouts_bitwise(solver, steady, precision):
+    """260x260x60 has 2.13 M cells per colour half, where the colour-split
+    sweep takes more rows per thread; 30 iterations of each path."""
+    print("case 260x260x60 solver", solver, "steady", steady, "precision", precision, flush=True)
+    a = run_limited((260, 260, 60), L.LAYOUT_AUTO, solver, steady, precision, 30)
+    b = run_limited((260, 260, 60), L.LAYOUT_STANDARD, solver, steady, precision, 30)
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
