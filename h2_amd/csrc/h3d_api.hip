@@ -325,6 +325,7 @@ struct KVecs {
     double *x;
     double *xbest;  // guarded solves: the iterate with the smallest residual (else nullptr)
     T *th, *b, *p, *p_, *r, *r0, *q, *s, *s_, *t;
+    int guard = 0;  // divergence and stall guard (always on with xbest)
 };
 
 // The operator of a solve in the layout it runs in: the reference's layout
@@ -504,7 +505,7 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
     hipStream_t s = c->stream;
     int rc;
     if ((rc = ensure_hist(c, std::max(maxItr, 1))) != H3D_OK) return rc;
-    if ((rc = reset_state(c, tol, maxItr, v.xbest != nullptr)) != H3D_OK) return rc;
+    if ((rc = reset_state(c, tol, maxItr, v.guard != 0 || v.xbest != nullptr)) != H3D_OK) return rc;
     // The Krylov vectors keep a zero halo, as in the reference where only
     // their interiors are ever written (WorkBuffers starts them at zero).
     T *vecs[] = {v.p, v.p_, v.r, v.r0, v.q, v.s, v.s_, v.t, P.swap};
@@ -835,7 +836,13 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
     if ((rc = path_vecs<float>(c, P32, w)) != H3D_OK) return rc;
     KVecs<float> v;
     v.x = c->d_corr;
-    v.xbest = c->d_best;
+    // CG!'s iterates decrease the error in the A-norm, so its last iterate
+    // is the one to apply and no best-iterate copy is kept (8 B/cell less
+    // per iteration); PBiCGSTAB!'s residuals are erratic in FP32 and its
+    // best iterate is kept
+    const bool snap = solver != H3D_SOLVER_CG;
+    v.xbest = snap ? c->d_best : nullptr;
+    v.guard = 1;
     v.th = c->g32[F32_X]; v.b = c->g32[F32_B];
     set_krylov(v, w);
     HIPCHK(c, launch_fill<float>(v.b, 0.f, c->H, s));
@@ -887,11 +894,12 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         rn_prev = rn;
         if (outer_it == max_outer || total_inner >= maxItr) break;
         // inner FP32 solve of A d = r from d = 0, guarded: it stops on
-        // divergence or stagnation and the correction is its best iterate
+        // divergence or stagnation; the correction is its best iterate
+        // (PBiCGSTAB!) or its last one (CG!)
         HIPCHK(c, launch_convert<float, double>(v.b, r64, c->H, s));
         HIPCHK(c, launch_fill<float>(v.th, 0.f, c->H, s));
         HIPCHK(c, launch_fill<double>(v.x, 0.0, c->H, s));
-        HIPCHK(c, launch_fill<double>(v.xbest, 0.0, c->H, s));
+        if (snap) HIPCHK(c, launch_fill<double>(v.xbest, 0.0, c->H, s));
         KState st{};
         rc = krylov<float>(c, solver, v, P32, inner_tol, std::max(1, maxItr - total_inner), smoother, &st);
         if (rc != H3D_OK) return rc;
@@ -907,8 +915,10 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
             }
         }
         if (!(st.res_min < 1.0)) break;  // the inner solve made no progress: stop refining
+        // without a copy, the last iterate only where it reduced the residual
+        if (!snap && !(st.res < 1.0)) break;
         // st.improved: the last iterate is the best one; else the snapshot is
-        HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], st.improved ? v.x : v.xbest, g64, s));
+        HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], (snap && !st.improved) ? v.xbest : v.x, g64, s));
     }
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipEventSynchronize(c->ev1));
