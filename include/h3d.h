@@ -92,7 +92,10 @@ extern "C" {
                                   finite-difference checks of the adjoint) */
 #define H3D_OPT_SWEEP_CHUNK 4  /* Gauss-Seidel sweeps of the colour-split layout: 0 = one launch per colour,
                                   C >= 1 = the two colours of a sweep in one launch marching chunks of C
-                                  z-planes, -1 = chosen from the grid (default); bitwise identical */
+                                  z-planes (C capped at NZ and 125; operators with more than 16 classes
+                                  keep one launch per colour), -1 = chosen from the grid (default: chunks
+                                  of 60 planes when NZ >= 120, else one launch per colour); bitwise
+                                  identical */
 
 typedef struct h3d_ctx h3d_ctx;
 
