@@ -44,3 +44,35 @@ def test_mixed_bc_solve_bitwise(oracle, size, solver, steady, chunk):
     assert np.array_equal(wk.history(), hist)
     assert np.array_equal(wk.download("theta"), c.arr["theta"])
     wk.close()
+
+
+@pytest.mark.parametrize("solver", ["pbicgstab", "cg"])
+def test_mixed_bc_fp32_refined(oracle, solver):
+    """The FP32-refined solve on the same mixed-condition case (tol 1e-8).
+    The oracle recomputes the true residual of the returned θ, which must be
+    below 1e-8 of res0.  θ is compared with the oracle's FP64 CG solve to
+    1e-12: on this case an FP64 PBiCGSTAB solve stopped at 1e-8 (residual
+    4.0e-9) is 2.1e-4 K from it and one stopped at 1e-9 is 3.7e-5 K away, so
+    the bound is 1e-3 K."""
+    dt = 0.25
+    wk, c, rng = make_pair(oracle, 40, 40, 30, seed=12, bc_set=mixed_bc(), random_lam="classes", steady=True, dt=dt)
+    shape = c.shape
+    put(wk, c, "hsrc", rng.uniform(0.0, 2.0e11, shape))
+    put(wk, c, "b", np.zeros(shape))
+    wk.calc_rhs(dt, True)
+    with oracle.sum_mode(oracle.EXACT):
+        c.cal_rhs()
+        conv_o, itr_o, res0_o, _ = c.solve("cg", tol=1e-12, maxItr=5000, smoother=1)
+    assert conv_o
+    s = L.SOLVER_CG if solver == "cg" else L.SOLVER_PBICGSTAB
+    conv, itr, res0 = wk.solve(s, dt, 1e-8, 5000, L.SMOOTHER_GS, True, precision=L.FP32_REFINED)
+    print("solver", solver, "refined", conv, itr, wk.history(), flush=True)
+    assert conv and res0 == res0_o
+    th = wk.download("theta")
+    dth = np.max(np.abs(th - c.arr["theta"]))
+    c.arr["theta"][:] = th
+    rel = c.calc_rk("r", "theta", "b") / res0_o
+    print("  true residual", rel, "max dθ", dth, flush=True)
+    assert rel < 1e-8
+    assert dth < 1e-3
+    wk.close()
