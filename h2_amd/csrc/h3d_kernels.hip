@@ -125,7 +125,7 @@ static size_t table_bytes(const Geo<T> &g)
 // ---------------------------------------------------------------------------
 // scalar steps (one thread, after the sums are known)
 // ---------------------------------------------------------------------------
-__device__ void fin_res0(double s0, KState *st)  // CalcRK! result; NonUniform.jl:263-272
+__device__ __forceinline__ void fin_res0(double s0, KState *st)  // CalcRK! result; NonUniform.jl:263-272
 {
     const double r0 = sqrt(s0);
     st->res0 = r0;
@@ -134,7 +134,7 @@ __device__ void fin_res0(double s0, KState *st)  // CalcRK! result; NonUniform.j
     else if (!isfinite(r0)) st->status = ST_NAN;
 }
 
-__device__ void fin_bicg_rho(double rho, KState *st)  // NonUniform.jl:287-300
+__device__ __forceinline__ void fin_bicg_rho(double rho, KState *st)  // NonUniform.jl:287-300
 {
     st->itr += 1;
     st->rho = rho;
@@ -147,7 +147,7 @@ __device__ void fin_bicg_rho(double rho, KState *st)  // NonUniform.jl:287-300
 
 // Divergence guard of a guarded solve (no reference counterpart): returns
 // false after stopping the solve.
-__device__ bool guard_step(double res, KState *st)
+__device__ __forceinline__ bool guard_step(double res, KState *st)
 {
     st->improved = 0;
     if (!isfinite(res) || res > kDivergeFactor * st->res_min || st->since_best >= kStallIters) {
@@ -164,7 +164,7 @@ __device__ bool guard_step(double res, KState *st)
     return true;
 }
 
-__device__ void fin_bicg_res(double s0, KState *st, double *hist)  // NonUniform.jl:331-346
+__device__ __forceinline__ void fin_bicg_res(double s0, KState *st, double *hist)  // NonUniform.jl:331-346
 {
     double res = sqrt(s0);
     res /= st->res0;
@@ -179,7 +179,7 @@ __device__ void fin_bicg_res(double s0, KState *st, double *hist)  // NonUniform
     }
 }
 
-__device__ void finalize(int fin, const double *s, KState *st, double *hist)
+__device__ __forceinline__ void finalize(int fin, const double *s, KState *st, double *hist)
 {
     switch (fin) {
     case FIN_STORE:
@@ -273,14 +273,18 @@ __device__ void finalize(int fin, const double *s, KState *st, double *hist)
 
 // Sum nb workgroup partials in a fixed order and run the finaliser.  One
 // workgroup of NT threads; each thread issues its loads four at a time
-// before adding them.
+// before adding them.  The first loads and the leader's copy of the state
+// are issued before the halted test, so the three reads overlap instead of
+// following one another; the finaliser works on the copy, which the leader
+// stores back whole.
 template <int NS>
 __global__ void __launch_bounds__(256) k_finalize(const double *partials, unsigned nb, int fin, KState *st,
                                                   double *hist)
 {
-    if (halted(st)) return;
     const int tid = threadIdx.x;
     dd acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    KState sv;
+    if (block_leader()) sv = *st;
     unsigned b = tid;
     for (; b + 3 * NT < nb; b += 4 * NT) {
         double v[4][3][2];
@@ -301,13 +305,16 @@ __global__ void __launch_bounds__(256) k_finalize(const double *partials, unsign
         for (int s = 0; s < NS; ++s)
             acc[s] = dd_add(acc[s], dd{partials[6 * (size_t)b + 2 * s], partials[6 * (size_t)b + 2 * s + 1]});
     }
+    // uniform over the workgroup: every thread leaves, or none
+    if (halted(st)) return;
     dd t[3];
     block_sum<NS>(acc, t);
     if (block_leader()) {
         double tot[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < NS; ++s) tot[s] = t[s].hi + t[s].lo;
-        finalize(fin, tot, st, hist);
+        finalize(fin, tot, &sv, hist);
+        *st = sv;
     }
 }
 
