@@ -373,9 +373,10 @@ hipError_t p_copy_interior(const Path<T> &P, T *dst, const T *src, const KState 
 }
 
 template <typename T>
-hipError_t p_sweep(const Path<T> &P, T *x, const T *b, int color, const KState *st, hipStream_t s)
+hipError_t p_sweep(const Path<T> &P, T *x, const T *b, int color, const KState *st, hipStream_t s,
+                   bool from_zero = false)
 {
-    return P.rb ? launch_rb_sweep<T>(x, b, P.rg, color, st, s) : launch_rbsor<T>(x, b, P.g, color, st, s);
+    return P.rb ? launch_rb_sweep<T>(x, b, P.rg, color, st, s, from_zero) : launch_rbsor<T>(x, b, P.g, color, st, s);
 }
 
 // Preconditioner! applied to the vector the mode forms (BiCG1!, Triad! or
@@ -396,7 +397,8 @@ hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T
     e = P.rb ? launch_rb_first<T>(mode, xx, bb, r, q, P.rg, st, s)
              : launch_rbsor_first<T>(mode, xx, bb, r, q, P.g, st, s);
     if (e != hipSuccess) return e;
-    if ((e = p_sweep<T>(P, xx, bb, 1, st, s)) != hipSuccess) return e;
+    // colour 1 of sweep 1 starts from xx = 0 (k_rb_first leaves those cells unwritten)
+    if ((e = p_sweep<T>(P, xx, bb, 1, st, s, true)) != hipSuccess) return e;
     if (P.rb && P.chunk > 0) {
         // sweeps 2-5, each colour 0 (half 1) then colour 1 (half 0) in one
         // launch, out of place: xx -> swap -> xx -> swap -> xx

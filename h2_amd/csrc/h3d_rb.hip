@@ -367,7 +367,7 @@ static unsigned rb_rows_nb(const RbGeo<T> &g)
 // it reads the neighbours from; bo = b's half h; clo/clt = the class
 // field's halves.  Same arithmetic as gs_update().  Each thread updates one cell in each of R rows; all their
 // loads are issued before the table staging.
-template <typename T, int R>
+template <typename T, int R, bool ZERO>
 __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *__restrict__ xt,
                                                   const T *__restrict__ bo, const uint8_t *__restrict__ clo,
                                                   const uint8_t *__restrict__ clt, RbGeo<T> g, int h, const KState *st)
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
         rr[a] = (unsigned)t.m + HX * (unsigned)jj[a] + HXY * (unsigned)t.k;
         const unsigned rb = rr[a] * (unsigned)sizeof(T);
         n[a] = rb_nb(xt, rr[a], s[a], HX, HXY);
-        pp[a] = ldb(xo, rb);
+        pp[a] = ZERO ? T(0) : ldb(xo, rb);
         bc[a] = ldb(bo, rb);
         q[a] = rb_cls(clo, clt, rr[a], rr[a], s[a], HX, HXY);
     }
@@ -416,18 +416,30 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
 // small grids (12.2 against 13.4 us with four), four on large ones (56.4
 // against 58.0 us, 540 against 559 us).  Large: at least 2^21 cells per
 // half.
+template <typename T, int R>
+static void rb_sweep_rows(T *x, const T *b, const RbGeo<T> &g, int h, bool from_zero, const KState *st,
+                          hipStream_t s)
+{
+    const size_t own = h ? g.HS : 0, oth = h ? 0 : g.HS;
+    if (from_zero)
+        hipLaunchKernelGGL((k_rb_sweep<T, R, true>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own,
+                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+    else
+        hipLaunchKernelGGL((k_rb_sweep<T, R, false>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own,
+                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+}
+
+// from_zero: the colour's own cells of x are zero (the first colour-1 pass
+// of a preconditioner that starts from x = 0); the kernel takes them as 0
+// instead of reading them, so they need not have been written.
 template <typename T>
-hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s)
+hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s,
+                           bool from_zero)
 {
     constexpr int R = sizeof(T) == 4 ? 2 : 1;
     const int h = 1 - color;
-    const size_t own = h ? g.HS : 0, oth = h ? 0 : g.HS;
-    if (g.HS >= ((size_t)1 << 21))
-        hipLaunchKernelGGL((k_rb_sweep<T, 2 * R>), dim3(rb_rows_nb<2 * R>(g)), dim3(BX, BY), rb_lds(g), s, x + own,
-                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
-    else
-        hipLaunchKernelGGL((k_rb_sweep<T, R>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own, x + oth,
-                           b + own, g.cls + own, g.cls + oth, g, h, st);
+    if (g.HS >= ((size_t)1 << 21)) rb_sweep_rows<T, 2 * R>(x, b, g, h, from_zero, st, s);
+    else rb_sweep_rows<T, R>(x, b, g, h, from_zero, st, s);
     return hipGetLastError();
 }
 
@@ -703,11 +715,13 @@ int sweep_chunk_auto(int HX, int NY, int NZ, int elem_bytes)
 }
 
 // Colour 0 of the first sweep from x = 0, fused with forming its right-hand
-// side bb (BiCG1! / Triad! / given) and with the fill: the colour-0 cell of
-// the pair (half 1) gets the Gauss-Seidel value from all-zero neighbours,
-// the colour-1 cell 0.  Same arithmetic as k_rbsor_first.
+// side bb (BiCG1! / Triad! / given): the colour-0 cell of the pair (half 1)
+// gets the Gauss-Seidel value from all-zero neighbours.  Same arithmetic as
+// k_rbsor_first.  The colour-1 cell of x is not written: the colour-1 pass
+// that follows (launch_rb_sweep with from_zero) takes it as zero without
+// reading it and writes every interior one.
 template <typename T>
-__global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x0, T *__restrict__ x1,
+__global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x1,
                                                   T *__restrict__ b0, T *__restrict__ b1, const T *__restrict__ r0,
                                                   const T *__restrict__ r1, const T *__restrict__ q0,
                                                   const T *__restrict__ q1, const uint8_t *__restrict__ c0,
@@ -762,7 +776,6 @@ __global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x0, 
         if (v0) stb(b0, ob, bv[0]);
         if (v1) stb(b1, ob, bv[1]);
     }
-    if (v0) stb(x0, ob, T(0));
     if (v1) stb(x1, ob, xn);
 }
 
@@ -773,7 +786,7 @@ hipError_t launch_rb_first(int mode, T *x, T *bb, const T *r, const T *q, const 
     const size_t HS = g.HS;
     // q is not read in FIRST_COPY mode (nullptr); r neither
     const T *r1 = r ? r + HS : nullptr, *q1 = q ? q + HS : nullptr;
-    hipLaunchKernelGGL(k_rb_first<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, mode, x, x + HS, bb, bb + HS, r, r1, q,
+    hipLaunchKernelGGL(k_rb_first<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, mode, x + HS, bb, bb + HS, r, r1, q,
                        q1, g.cls, g.cls + HS, g, st);
     return hipGetLastError();
 }
@@ -1177,7 +1190,8 @@ hipError_t launch_rb_copy_interior(T *dst, const T *src, const RbGeo<T> &g, cons
                                              double *, const RedBuf &, hipStream_t);                            \
     template hipError_t launch_rb_calc_rk<T>(T *, T *, const T *, const T *, const RbGeo<T> &, int, KState *,    \
                                              double *, const RedBuf &, hipStream_t);                            \
-    template hipError_t launch_rb_sweep<T>(T *, const T *, const RbGeo<T> &, int, const KState *, hipStream_t);  \
+    template hipError_t launch_rb_sweep<T>(T *, const T *, const RbGeo<T> &, int, const KState *, hipStream_t,    \
+                                           bool);                                                                  \
     template hipError_t launch_rb_sweep2<T>(const T *, T *, const T *, const RbGeo<T> &, int, int, const KState *, \
                                             hipStream_t);                                                       \
     template bool rb_sweep2_fits<T>(const RbGeo<T> &);                                                         \
