@@ -416,6 +416,31 @@ hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T
     return e;
 }
 
+// Preconditioner!(xx, bb) from zero, then Σ bb·xx finalized with fin (CG!'s
+// ρ = (r, z), NonUniform.jl:201-205).  With per-colour GS sweeps on the
+// colour-split path the sum is formed by the last colour-1 pass itself
+// (launch_rb_sweep_dot); otherwise by a separate dot kernel.  Either way
+// the sum is exactly rounded, so the two are bitwise the same.
+template <typename T>
+hipError_t p_precond_dot(const Path<T> &P, T *xx, T *bb, int smoother, int fin, KState *st, double *hist,
+                         const RedBuf &rb, hipStream_t s)
+{
+    hipError_t e;
+    if (!(P.rb && P.chunk <= 0 && smoother == H3D_SMOOTHER_GS)) {
+        if ((e = p_precond_from_zero<T>(P, FIRST_COPY, xx, bb, nullptr, nullptr, smoother, st, s)) != hipSuccess)
+            return e;
+        return p_dot<T>(P, bb, xx, nullptr, fin, st, hist, rb, s);
+    }
+    if ((e = launch_rb_first<T>(FIRST_COPY, xx, bb, nullptr, nullptr, P.rg, st, s)) != hipSuccess) return e;
+    if ((e = p_sweep<T>(P, xx, bb, 1, st, s, true)) != hipSuccess) return e;
+    for (int sweep = 1; sweep < 5; ++sweep) {
+        if ((e = p_sweep<T>(P, xx, bb, 0, st, s)) != hipSuccess) return e;
+        e = sweep < 4 ? p_sweep<T>(P, xx, bb, 1, st, s) : launch_rb_sweep_dot<T>(xx, bb, P.rg, fin, st, hist, rb, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // Enqueue one PBiCGSTAB! iteration (NonUniform.jl:287-347).  ρ of the
 // iteration was formed by the previous iteration's update (or by CalcRK!
 // for iteration 1) and its FIN_BICG_*RHO step.
@@ -452,9 +477,8 @@ hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, int 
     e = P.rb ? launch_rb_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.rg, st, c->d_hist, c->rb, s)
              : launch_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.g, st, c->d_hist, c->rb, s);
     if (e != hipSuccess) return e;
-    if ((e = p_precond_from_zero<T>(P, FIRST_COPY, v.s, v.r, nullptr, nullptr, smoother, st, s)) != hipSuccess)
-        return e;
-    if ((e = p_dot<T>(P, v.r, v.s, nullptr, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
+    // z = M⁻¹ r; β from ρ = (r, z)
+    if ((e = p_precond_dot<T>(P, v.s, v.r, smoother, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
     return P.rb ? launch_rb_cg_p<T>(v.p, v.s, P.rg, st, s) : launch_cg_p<T>(v.p, v.s, P.g, st, s);
 }
 

@@ -277,6 +277,9 @@ hipError_t launch_rb_calc_rk(T *r, T *r0copy, const T *th, const T *b, const RbG
 template <typename T>
 hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s,
                            bool from_zero = false);
+template <typename T>
+hipError_t launch_rb_sweep_dot(T *x, const T *b, const RbGeo<T> &g, int fin, KState *st, double *hist,
+                               const RedBuf &rb, hipStream_t s);
 // Colour passes of halves hA then 1 - hA in one launch, xin -> xout (both
 // whole RB arrays with equal halos), z-chunks of C planes; the result is
 // bitwise that of the two in-place passes.  Only for operators for which

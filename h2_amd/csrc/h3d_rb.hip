@@ -367,17 +367,18 @@ static unsigned rb_rows_nb(const RbGeo<T> &g)
 // it reads the neighbours from; bo = b's half h; clo/clt = the class
 // field's halves.  Same arithmetic as gs_update().  Each thread updates one cell in each of R rows; all their
 // loads are issued before the table staging.
-template <typename T, int R, bool ZERO>
+template <typename T, int R, bool ZERO, bool DOT>
 __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *__restrict__ xt,
                                                   const T *__restrict__ bo, const uint8_t *__restrict__ clo,
-                                                  const uint8_t *__restrict__ clt, RbGeo<T> g, int h, const KState *st)
+                                                  const uint8_t *__restrict__ clt, RbGeo<T> g, int h, const KState *st,
+                                                  const T *__restrict__ bt, RedBuf rbuf)
 {
     if (halted(st)) return;
     const RbRows t = rb_rows<R>(g);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
     Nb<T> n[R];
     Cls q[R];
-    T pp[R], bc[R];
+    T pp[R], bc[R], bp[R];
     int s[R], jj[R];
     unsigned rr[R];
 #pragma unroll
@@ -389,13 +390,18 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
         n[a] = rb_nb(xt, rr[a], s[a], HX, HXY);
         pp[a] = ZERO ? T(0) : ldb(xo, rb);
         bc[a] = ldb(bo, rb);
+        bp[a] = DOT ? ldb(bt, rb) : T(0);
         q[a] = rb_cls(clo, clt, rr[a], rr[a], s[a], HX, HXY);
     }
     const T *stab = rb_stage(g, t.k);
+    T xv[R];
+    bool valid[R];
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
     for (int a = 0; a < R; ++a) {
         const int i = 2 * t.m + s[a];
-        const bool valid = t.live && t.j0 + a * BY <= g.NY && i >= 1 && i <= g.NX;
+        const bool rowok = t.live && t.j0 + a * BY <= g.NY;
+        valid[a] = rowok && i >= 1 && i <= g.NX;
         const Cond<T> o = rb_cond(g, stab, q[a], i, jj[a], t.k);
         const T dd = diag(o);
         T xn;
@@ -405,8 +411,22 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
             const T dp = (((rb_offdiag(o, n[a]) - bc[a]) / dd - pp[a])) * T(1);
             xn = pp[a] + T(1) * dp;
         }
-        if (valid) stb(xo, rr[a] * (unsigned)sizeof(T), xn);
+        xv[a] = xn;
+        if (DOT) {
+            // this cell, and the other half's cell at the same offset
+            // (i' = 2m + 1 - s, already loaded as the x-neighbour on that side)
+            const int ip = 2 * t.m + 1 - s[a];
+            const T xp = s[a] ? n[a].xm : n[a].xp;
+            if (valid[a]) v[0] = dd_add(v[0], (double)bc[a] * (double)xn);
+            if (rowok && ip >= 1 && ip <= g.NX) v[0] = dd_add(v[0], (double)bp[a] * (double)xp);
+        }
     }
+    // the workgroup sum before the stores (a store ahead of the reduction's
+    // barrier would hold the wave until it completes)
+    if (DOT) block_partials<1>(v, rbuf);
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+        if (valid[a]) stb(xo, rr[a] * (unsigned)sizeof(T), xv[a]);
 }
 
 // Rows per thread of the sweep, by precision and grid size (measured, HIP
@@ -421,12 +441,13 @@ static void rb_sweep_rows(T *x, const T *b, const RbGeo<T> &g, int h, bool from_
                           hipStream_t s)
 {
     const size_t own = h ? g.HS : 0, oth = h ? 0 : g.HS;
+    const RedBuf none{};
     if (from_zero)
-        hipLaunchKernelGGL((k_rb_sweep<T, R, true>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own,
-                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+        hipLaunchKernelGGL((k_rb_sweep<T, R, true, false>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s,
+                           x + own, x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, nullptr, none);
     else
-        hipLaunchKernelGGL((k_rb_sweep<T, R, false>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s, x + own,
-                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+        hipLaunchKernelGGL((k_rb_sweep<T, R, false, false>), dim3(rb_rows_nb<R>(g)), dim3(BX, BY), rb_lds(g), s,
+                           x + own, x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, nullptr, none);
 }
 
 // from_zero: the colour's own cells of x are zero (the first colour-1 pass
@@ -441,6 +462,34 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
     if (g.HS >= ((size_t)1 << 21)) rb_sweep_rows<T, 2 * R>(x, b, g, h, from_zero, st, s);
     else rb_sweep_rows<T, R>(x, b, g, h, from_zero, st, s);
     return hipGetLastError();
+}
+
+// Colour 1 of the preconditioner's last sweep (not from zero) with
+// Σ b·x over every interior cell of both halves, finalized with fin: CG!'s
+// ρ = (r, z) when b is r (NonUniform.jl:205).  The colour-0 half of x is
+// final before this pass, so each thread adds its own cell (new value) and
+// the colour-0 cell at the same offset in the other half, which it has
+// already loaded as a neighbour.  The sum is exactly rounded, so it is
+// bitwise the separate dot kernel's.
+template <typename T>
+hipError_t launch_rb_sweep_dot(T *x, const T *b, const RbGeo<T> &g, int fin, KState *st, double *hist,
+                               const RedBuf &rb, hipStream_t s)
+{
+    constexpr int R0 = sizeof(T) == 4 ? 2 : 1;
+    const int h = 0;  // colour 1
+    const size_t own = 0, oth = g.HS;
+    unsigned nb;
+    if (g.HS >= ((size_t)1 << 21)) {
+        nb = rb_rows_nb<2 * R0>(g);
+        hipLaunchKernelGGL((k_rb_sweep<T, 2 * R0, false, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, b + oth, rb);
+    } else {
+        nb = rb_rows_nb<R0>(g);
+        hipLaunchKernelGGL((k_rb_sweep<T, R0, false, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+                           x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, b + oth, rb);
+    }
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize_nb(1, nb, fin, st, hist, rb, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1192,6 +1241,8 @@ hipError_t launch_rb_copy_interior(T *dst, const T *src, const RbGeo<T> &g, cons
                                              double *, const RedBuf &, hipStream_t);                            \
     template hipError_t launch_rb_sweep<T>(T *, const T *, const RbGeo<T> &, int, const KState *, hipStream_t,    \
                                            bool);                                                                  \
+    template hipError_t launch_rb_sweep_dot<T>(T *, const T *, const RbGeo<T> &, int, KState *, double *,       \
+                                               const RedBuf &, hipStream_t);                                    \
     template hipError_t launch_rb_sweep2<T>(const T *, T *, const T *, const RbGeo<T> &, int, int, const KState *, \
                                             hipStream_t);                                                       \
     template bool rb_sweep2_fits<T>(const RbGeo<T> &);                                                         \
