@@ -379,25 +379,26 @@ hipError_t p_sweep(const Path<T> &P, T *x, const T *b, int color, const KState *
     return P.rb ? launch_rb_sweep<T>(x, b, P.rg, color, st, s, from_zero) : launch_rbsor<T>(x, b, P.g, color, st, s);
 }
 
-// Preconditioner! applied to the vector the mode forms (BiCG1!, Triad! or
-// bb itself), starting from xx = 0 as the callers' myfill!(xx, 0) does.
+// Sweeps 1 to 5 of the GS preconditioner after a first kernel has solved
+// colour 0 of sweep 1 from xx = 0: colour 1 of sweep 1 from zero (the first
+// kernels leave those cells unwritten), then sweeps 2-5.  With dot != null
+// also Σ bb·xx over the interior, finalized with dot->fin (CG!'s
+// ρ = (r, z), NonUniform.jl:201-205): formed inside the last colour-1 pass
+// with per-colour sweeps on the colour-split path (launch_rb_sweep_dot),
+// otherwise by a separate dot kernel.  The sum is exactly rounded either
+// way, so the two are bitwise the same.
+struct DotFin {
+    int fin;
+    KState *st;
+    double *hist;
+    const RedBuf *rb;
+};
+
 template <typename T>
-hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T *r, const T *q, int smoother,
-                               const KState *st, hipStream_t s)
+hipError_t p_sweeps_after_first(const Path<T> &P, T *xx, T *bb, const KState *st, hipStream_t s,
+                                const DotFin *dot = nullptr)
 {
     hipError_t e;
-    if (smoother == H3D_SMOOTHER_NONE) {
-        if (mode != FIRST_COPY) {
-            e = P.rb ? launch_rb_form_rhs<T>(mode, bb, r, q, P.rg, st, s)
-                     : launch_form_rhs<T>(mode, bb, r, q, P.g, st, s);
-            if (e != hipSuccess) return e;
-        }
-        return hipMemcpyAsync(xx, bb, P.n * sizeof(T), hipMemcpyDeviceToDevice, s);
-    }
-    e = P.rb ? launch_rb_first<T>(mode, xx, bb, r, q, P.rg, st, s)
-             : launch_rbsor_first<T>(mode, xx, bb, r, q, P.g, st, s);
-    if (e != hipSuccess) return e;
-    // colour 1 of sweep 1 starts from xx = 0 (k_rb_first leaves those cells unwritten)
     if ((e = p_sweep<T>(P, xx, bb, 1, st, s, true)) != hipSuccess) return e;
     if (P.rb && P.chunk > 0) {
         // sweeps 2-5, each colour 0 (half 1) then colour 1 (half 0) in one
@@ -407,38 +408,41 @@ hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T
             e = launch_rb_sweep2<T>(src, dst, bb, P.rg, 1, P.chunk, st, s);
             std::swap(src, dst);
         }
-        return e;
+    } else {
+        for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
+            e = p_sweep<T>(P, xx, bb, 0, st, s);
+            if (e != hipSuccess) break;
+            if (sweep == 4 && dot && P.rb)
+                return launch_rb_sweep_dot<T>(xx, bb, P.rg, dot->fin, dot->st, dot->hist, *dot->rb, s);
+            e = p_sweep<T>(P, xx, bb, 1, st, s);
+        }
     }
-    for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
-        e = p_sweep<T>(P, xx, bb, 0, st, s);
-        if (e == hipSuccess) e = p_sweep<T>(P, xx, bb, 1, st, s);
-    }
-    return e;
+    if (e != hipSuccess || !dot) return e;
+    return p_dot<T>(P, bb, xx, nullptr, dot->fin, dot->st, dot->hist, *dot->rb, s);
 }
 
-// Preconditioner!(xx, bb) from zero, then Σ bb·xx finalized with fin (CG!'s
-// ρ = (r, z), NonUniform.jl:201-205).  With per-colour GS sweeps on the
-// colour-split path the sum is formed by the last colour-1 pass itself
-// (launch_rb_sweep_dot); otherwise by a separate dot kernel.  Either way
-// the sum is exactly rounded, so the two are bitwise the same.
+// Preconditioner! applied to the vector the mode forms (BiCG1!, Triad! or
+// bb itself), starting from xx = 0 as the callers' myfill!(xx, 0) does;
+// with dot, also Σ bb·xx (see p_sweeps_after_first).
 template <typename T>
-hipError_t p_precond_dot(const Path<T> &P, T *xx, T *bb, int smoother, int fin, KState *st, double *hist,
-                         const RedBuf &rb, hipStream_t s)
+hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T *r, const T *q, int smoother,
+                               const KState *st, hipStream_t s, const DotFin *dot = nullptr)
 {
     hipError_t e;
-    if (!(P.rb && P.chunk <= 0 && smoother == H3D_SMOOTHER_GS)) {
-        if ((e = p_precond_from_zero<T>(P, FIRST_COPY, xx, bb, nullptr, nullptr, smoother, st, s)) != hipSuccess)
-            return e;
-        return p_dot<T>(P, bb, xx, nullptr, fin, st, hist, rb, s);
+    if (smoother == H3D_SMOOTHER_NONE) {
+        if (mode != FIRST_COPY) {
+            e = P.rb ? launch_rb_form_rhs<T>(mode, bb, r, q, P.rg, st, s)
+                     : launch_form_rhs<T>(mode, bb, r, q, P.g, st, s);
+            if (e != hipSuccess) return e;
+        }
+        e = hipMemcpyAsync(xx, bb, P.n * sizeof(T), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess || !dot) return e;
+        return p_dot<T>(P, bb, xx, nullptr, dot->fin, dot->st, dot->hist, *dot->rb, s);
     }
-    if ((e = launch_rb_first<T>(FIRST_COPY, xx, bb, nullptr, nullptr, P.rg, st, s)) != hipSuccess) return e;
-    if ((e = p_sweep<T>(P, xx, bb, 1, st, s, true)) != hipSuccess) return e;
-    for (int sweep = 1; sweep < 5; ++sweep) {
-        if ((e = p_sweep<T>(P, xx, bb, 0, st, s)) != hipSuccess) return e;
-        e = sweep < 4 ? p_sweep<T>(P, xx, bb, 1, st, s) : launch_rb_sweep_dot<T>(xx, bb, P.rg, fin, st, hist, rb, s);
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    e = P.rb ? launch_rb_first<T>(mode, xx, bb, r, q, P.rg, st, s)
+             : launch_rbsor_first<T>(mode, xx, bb, r, q, P.g, st, s);
+    if (e != hipSuccess) return e;
+    return p_sweeps_after_first<T>(P, xx, bb, st, s, dot);
 }
 
 // Enqueue one PBiCGSTAB! iteration (NonUniform.jl:287-347).  ρ of the
@@ -466,7 +470,9 @@ hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, in
                 : launch_bicg_update<T>(v.x, v.xbest, v.r, v.p_, v.s_, v.t, v.s, v.r0, P.g, st, c->d_hist, c->rb, s);
 }
 
-// Enqueue one CG! iteration (NonUniform.jl:160-223).
+// Enqueue one CG! iteration (NonUniform.jl:160-223).  On the colour-split
+// path with GS the update and the preconditioner's first colour share one
+// kernel (k_rb_cg_first).
 template <typename T>
 hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, int smoother)
 {
@@ -474,11 +480,17 @@ hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, int 
     KState *st = c->d_state;
     hipError_t e;
     if ((e = p_calc_ax<T>(P, v.q, v.p, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess) return e;
-    e = P.rb ? launch_rb_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.rg, st, c->d_hist, c->rb, s)
-             : launch_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.g, st, c->d_hist, c->rb, s);
-    if (e != hipSuccess) return e;
     // z = M⁻¹ r; β from ρ = (r, z)
-    if ((e = p_precond_dot<T>(P, v.s, v.r, smoother, FIN_CG_BETA, st, nullptr, c->rb, s)) != hipSuccess) return e;
+    const DotFin dot{FIN_CG_BETA, st, nullptr, &c->rb};
+    if (P.rb && smoother == H3D_SMOOTHER_GS) {
+        e = launch_rb_cg_first<T>(v.x, v.xbest, v.r, v.p, v.q, v.s, P.rg, st, c->d_hist, c->rb, s);
+        if (e == hipSuccess) e = p_sweeps_after_first<T>(P, v.s, v.r, st, s, &dot);
+    } else {
+        e = P.rb ? launch_rb_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.rg, st, c->d_hist, c->rb, s)
+                 : launch_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.g, st, c->d_hist, c->rb, s);
+        if (e == hipSuccess) e = p_precond_from_zero<T>(P, FIRST_COPY, v.s, v.r, nullptr, nullptr, smoother, st, s, &dot);
+    }
+    if (e != hipSuccess) return e;
     return P.rb ? launch_rb_cg_p<T>(v.p, v.s, P.rg, st, s) : launch_cg_p<T>(v.p, v.s, P.g, st, s);
 }
 

@@ -278,6 +278,9 @@ template <typename T>
 hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const KState *st, hipStream_t s,
                            bool from_zero = false);
 template <typename T>
+hipError_t launch_rb_cg_first(double *th, double *xbest, T *r, const T *p, const T *q, T *z, const RbGeo<T> &g,
+                              KState *st, double *hist, const RedBuf &rb, hipStream_t s);
+template <typename T>
 hipError_t launch_rb_sweep_dot(T *x, const T *b, const RbGeo<T> &g, int fin, KState *st, double *hist,
                                const RedBuf &rb, hipStream_t s);
 // Colour passes of halves hA then 1 - hA in one launch, xin -> xout (both

@@ -763,6 +763,20 @@ int sweep_chunk_auto(int HX, int NY, int NZ, int elem_bytes)
     return NZ >= 120 ? 60 : 0;
 }
 
+// The Gauss-Seidel value of a cell whose old value and neighbours are all
+// zero (the first colour of a preconditioner that starts from x = 0): the
+// general update's operations with those zeros, so bitwise the same.
+template <typename T>
+__device__ __forceinline__ T rb_gs_from_zero(const RbGeo<T> &g, const Cond<T> &cd, T bv)
+{
+    const T pp = T(0);
+    const T dd = diag(cd);
+    if (g.flux) return pp + (T(0) - bv) / dd;
+    const T ss = (cd.xp * T(0) + cd.xm * T(0) + cd.yp * T(0) + cd.ym * T(0) + cd.zp * T(0) + cd.zm * T(0));
+    const T dp = (((ss - bv) / dd - pp)) * T(1);
+    return pp + T(1) * dp;
+}
+
 // Colour 0 of the first sweep from x = 0, fused with forming its right-hand
 // side bb (BiCG1! / Triad! / given): the colour-0 cell of the pair (half 1)
 // gets the Gauss-Seidel value from all-zero neighbours.  Same arithmetic as
@@ -810,16 +824,7 @@ __global__ void __launch_bounds__(256) k_rb_first(int mode, T *__restrict__ x1,
     const Cls q = rb_cls(c1, c0, o, o, cr.s, HX, HXY);
     const T *stab = rb_stage(g, t.k);
     const Cond<T> cd = rb_cond(g, stab, q, cr.i, t.j, t.k);
-    const T pp = T(0);
-    const T dd = diag(cd);
-    T xn;
-    if (g.flux) {
-        xn = pp + (T(0) - bv[1]) / dd;
-    } else {
-        const T ss = (cd.xp * T(0) + cd.xm * T(0) + cd.yp * T(0) + cd.ym * T(0) + cd.zp * T(0) + cd.zm * T(0));
-        const T dp = (((ss - bv[1]) / dd - pp)) * T(1);
-        xn = pp + T(1) * dp;
-    }
+    const T xn = rb_gs_from_zero(g, cd, bv[1]);
     const bool v1 = c[a1].valid, v0 = c[1 - a1].valid;
     if (mode != FIRST_COPY) {
         if (v0) stb(b0, ob, bv[0]);
@@ -1183,6 +1188,76 @@ hipError_t launch_rb_cg_update(double *th, double *xbest, T *r, const T *p, cons
     return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, FIN_CG_RES, st, hist, rb, s);
 }
 
+// CG!'s update followed by the first colour of its preconditioner, in one
+// pass: θ += α p, r -= α q and Σ r² (NonUniform.jl:179-186, as
+// k_rb_cg_update), then colour 0 of z = M⁻¹ r from z = 0 with the new r as
+// its right-hand side (NonUniform.jl:201-203, as k_rb_first in FIRST_COPY
+// mode).  The residual's finalize follows this kernel, so a converged
+// iteration has also written colour 0 of z, which nothing reads; the
+// sweeps after the finalize see the halted state and return.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rb_cg_first(double *__restrict__ th, double *__restrict__ xbest,
+                                                     T *__restrict__ r, const T *__restrict__ p,
+                                                     const T *__restrict__ q, T *__restrict__ z1, RbGeo<T> g,
+                                                     KState *st, RedBuf rb)
+{
+    if (halted(st)) return;
+    const RbThread tt = rb_thread(g);
+    const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
+    const double alpha_d = st->alpha;
+    const T alpha = (T)st->alpha;
+    const bool snap = xbest != nullptr && st->improved != 0;
+    const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
+    RbCell c[2];
+    int h[2];
+    rb_pair(g, tt, c, h);
+    const size_t cs0 = (size_t)(2 * tt.m) + sy * (size_t)tt.j + sz * (size_t)tt.k;
+    double xo[2], xn[2];
+    T rv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const size_t o = (h[a] ? g.HS : 0) + c[a].co;
+        const size_t cs = min(cs0 + a, cs0 + (size_t)(g.MX - 1 - 2 * tt.m));
+        xo[a] = th[cs];
+        xn[a] = xo[a] + alpha_d * (double)p[o];
+        rv[a] = r[o] - alpha * q[o];
+    }
+    // the half-1 cell (colour 0) of the pair
+    const int a1 = h[0] == 1 ? 0 : 1;
+    const RbCell &cr = c[a1];
+    const unsigned o = c[0].co;
+    const Cls cq = rb_cls(g.cls + g.HS, g.cls, o, o, cr.s, HX, HXY);
+    const T *stab = rb_stage(g, tt.k);
+    const Cond<T> cd = rb_cond(g, stab, cq, cr.i, tt.j, tt.k);
+    const T zn = rb_gs_from_zero(g, cd, rv[a1]);
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+        if (c[a].valid) v[0] = dd_add(v[0], (double)rv[a] * (double)rv[a]);
+    block_partials<1>(v, rb);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (c[a].valid) {
+            const size_t oa = (h[a] ? g.HS : 0) + c[a].co;
+            const size_t cs = cs0 + a;
+            if (snap) xbest[cs] = xo[a];
+            th[cs] = xn[a];
+            r[oa] = rv[a];
+        }
+    }
+    if (cr.valid) stb(z1, o * (unsigned)sizeof(T), zn);
+}
+
+template <typename T>
+hipError_t launch_rb_cg_first(double *th, double *xbest, T *r, const T *p, const T *q, T *z, const RbGeo<T> &g,
+                              KState *st, double *hist, const RedBuf &rb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rb_cg_first<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, th, xbest, r, p, q, z + g.HS, g, st,
+                       rb);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, FIN_CG_RES, st, hist, rb, s);
+}
+
 // CG!: p = z + β p (NonUniform.jl:218-220)
 template <typename T>
 __global__ void __launch_bounds__(256) k_rb_cg_p(T *__restrict__ p, const T *__restrict__ z, RbGeo<T> g,
@@ -1241,6 +1316,8 @@ hipError_t launch_rb_copy_interior(T *dst, const T *src, const RbGeo<T> &g, cons
                                              double *, const RedBuf &, hipStream_t);                            \
     template hipError_t launch_rb_sweep<T>(T *, const T *, const RbGeo<T> &, int, const KState *, hipStream_t,    \
                                            bool);                                                                  \
+    template hipError_t launch_rb_cg_first<T>(double *, double *, T *, const T *, const T *, T *,                \
+                                              const RbGeo<T> &, KState *, double *, const RedBuf &, hipStream_t); \
     template hipError_t launch_rb_sweep_dot<T>(T *, const T *, const RbGeo<T> &, int, KState *, double *,       \
                                                const RedBuf &, hipStream_t);                                    \
     template hipError_t launch_rb_sweep2<T>(const T *, T *, const T *, const RbGeo<T> &, int, int, const KState *, \
