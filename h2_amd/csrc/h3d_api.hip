@@ -380,13 +380,14 @@ hipError_t p_sweep(const Path<T> &P, T *x, const T *b, int color, const KState *
 }
 
 // Sweeps 1 to 5 of the GS preconditioner after a first kernel has solved
-// colour 0 of sweep 1 from xx = 0: colour 1 of sweep 1 from zero (the first
-// kernels leave those cells unwritten), then sweeps 2-5.  With dot != null
+// colour 0 of sweep 1 from zero into x0 (xx itself, or the path's swap
+// buffer): colour 1 of sweep 1 from zero on x0 (the first kernels leave
+// those cells unwritten), then sweeps 2-5, ending in xx.  With dot != null
 // also Σ bb·xx over the interior, finalized with dot->fin (CG!'s
 // ρ = (r, z), NonUniform.jl:201-205): formed inside the last colour-1 pass
-// with per-colour sweeps on the colour-split path (launch_rb_sweep_dot),
-// otherwise by a separate dot kernel.  The sum is exactly rounded either
-// way, so the two are bitwise the same.
+// on the colour-split path (launch_rb_sweep_dot), by a separate dot kernel
+// otherwise.  The sum is exactly rounded either way, so the two are
+// bitwise the same.
 struct DotFin {
     int fin;
     KState *st;
@@ -395,27 +396,32 @@ struct DotFin {
 };
 
 template <typename T>
-hipError_t p_sweeps_after_first(const Path<T> &P, T *xx, T *bb, const KState *st, hipStream_t s,
+hipError_t p_sweeps_after_first(const Path<T> &P, T *x0, T *xx, T *bb, const KState *st, hipStream_t s,
                                 const DotFin *dot = nullptr)
 {
     hipError_t e;
-    if ((e = p_sweep<T>(P, xx, bb, 1, st, s, true)) != hipSuccess) return e;
+    if ((e = p_sweep<T>(P, x0, bb, 1, st, s, true)) != hipSuccess) return e;
+    // Two-colour launches, out of place, for sweeps 2-5 from x0 = xx
+    // (xx -> swap -> xx -> swap -> xx), or for sweeps 2-4 from x0 = swap
+    // (swap -> xx -> swap -> xx), sweep 5 then running per colour so that
+    // its last pass forms the sum: that costs less than a separate dot
+    // pass (measured at 960x960x120).
+    int n2 = 0;
+    T *src = xx, *dst = P.swap;
     if (P.rb && P.chunk > 0) {
-        // sweeps 2-5, each colour 0 (half 1) then colour 1 (half 0) in one
-        // launch, out of place: xx -> swap -> xx -> swap -> xx
-        T *src = xx, *dst = P.swap;
-        for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
-            e = launch_rb_sweep2<T>(src, dst, bb, P.rg, 1, P.chunk, st, s);
-            std::swap(src, dst);
-        }
-    } else {
-        for (int sweep = 1; sweep < 5 && e == hipSuccess; ++sweep) {
-            e = p_sweep<T>(P, xx, bb, 0, st, s);
-            if (e != hipSuccess) break;
-            if (sweep == 4 && dot && P.rb)
-                return launch_rb_sweep_dot<T>(xx, bb, P.rg, dot->fin, dot->st, dot->hist, *dot->rb, s);
-            e = p_sweep<T>(P, xx, bb, 1, st, s);
-        }
+        n2 = x0 == xx ? 4 : 3;
+        if (x0 != xx) std::swap(src, dst);
+    }
+    for (int k = 0; k < n2 && e == hipSuccess; ++k) {
+        e = launch_rb_sweep2<T>(src, dst, bb, P.rg, 1, P.chunk, st, s);
+        std::swap(src, dst);
+    }
+    for (int sweep = 1 + n2; sweep < 5 && e == hipSuccess; ++sweep) {
+        e = p_sweep<T>(P, xx, bb, 0, st, s);
+        if (e != hipSuccess) break;
+        if (sweep == 4 && dot && P.rb)
+            return launch_rb_sweep_dot<T>(xx, bb, P.rg, dot->fin, dot->st, dot->hist, *dot->rb, s);
+        e = p_sweep<T>(P, xx, bb, 1, st, s);
     }
     if (e != hipSuccess || !dot) return e;
     return p_dot<T>(P, bb, xx, nullptr, dot->fin, dot->st, dot->hist, *dot->rb, s);
@@ -442,7 +448,7 @@ hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T
     e = P.rb ? launch_rb_first<T>(mode, xx, bb, r, q, P.rg, st, s)
              : launch_rbsor_first<T>(mode, xx, bb, r, q, P.g, st, s);
     if (e != hipSuccess) return e;
-    return p_sweeps_after_first<T>(P, xx, bb, st, s, dot);
+    return p_sweeps_after_first<T>(P, xx, xx, bb, st, s, dot);
 }
 
 // Enqueue one PBiCGSTAB! iteration (NonUniform.jl:287-347).  ρ of the
@@ -483,8 +489,11 @@ hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, int 
     // z = M⁻¹ r; β from ρ = (r, z)
     const DotFin dot{FIN_CG_BETA, st, nullptr, &c->rb};
     if (P.rb && smoother == H3D_SMOOTHER_GS) {
-        e = launch_rb_cg_first<T>(v.x, v.xbest, v.r, v.p, v.q, v.s, P.rg, st, c->d_hist, c->rb, s);
-        if (e == hipSuccess) e = p_sweeps_after_first<T>(P, v.s, v.r, st, s, &dot);
+        // with two-colour sweeps, sweep 1 goes into the swap buffer (see
+        // p_sweeps_after_first)
+        T *z0 = P.chunk > 0 ? P.swap : v.s;
+        e = launch_rb_cg_first<T>(v.x, v.xbest, v.r, v.p, v.q, z0, P.rg, st, c->d_hist, c->rb, s);
+        if (e == hipSuccess) e = p_sweeps_after_first<T>(P, z0, v.s, v.r, st, s, &dot);
     } else {
         e = P.rb ? launch_rb_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.rg, st, c->d_hist, c->rb, s)
                  : launch_cg_update<T>(v.x, v.xbest, v.r, v.p, v.q, P.g, st, c->d_hist, c->rb, s);
