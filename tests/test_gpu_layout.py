@@ -29,7 +29,7 @@ def run(shape, layout, solver, smoother, steady, precision, tol):
 
 @pytest.mark.parametrize("shape", [(40, 40, 30), (17, 9, 30), (13, 70, 60)])
 @pytest.mark.parametrize("solver,smoother", [(L.SOLVER_PBICGSTAB, L.SMOOTHER_GS), (L.SOLVER_CG, L.SMOOTHER_GS),
-                                             (L.SOLVER_PBICGSTAB, L.SMOOTHER_NONE)])
+                                             (L.SOLVER_PBICGSTAB, L.SMOOTHER_NONE), (L.SOLVER_CG, L.SMOOTHER_NONE)])
 @pytest.mark.parametrize("steady", [True, False])
 def test_fp64_layouts_bitwise(shape, solver, smoother, steady):
     tol = 1e-8 if smoother == L.SMOOTHER_GS else 1e-5
@@ -42,9 +42,10 @@ def test_fp64_layouts_bitwise(shape, solver, smoother, steady):
 
 
 @pytest.mark.parametrize("shape", [(40, 40, 30), (17, 9, 30)])
-def test_fp32_refined_layouts_bitwise(shape):
-    a = run(shape, L.LAYOUT_AUTO, L.SOLVER_PBICGSTAB, L.SMOOTHER_GS, True, L.FP32_REFINED, 1e-8)
-    b = run(shape, L.LAYOUT_STANDARD, L.SOLVER_PBICGSTAB, L.SMOOTHER_GS, True, L.FP32_REFINED, 1e-8)
+@pytest.mark.parametrize("solver", [L.SOLVER_PBICGSTAB, L.SOLVER_CG])
+def test_fp32_refined_layouts_bitwise(shape, solver):
+    a = run(shape, L.LAYOUT_AUTO, solver, L.SMOOTHER_GS, True, L.FP32_REFINED, 1e-8)
+    b = run(shape, L.LAYOUT_STANDARD, solver, L.SMOOTHER_GS, True, L.FP32_REFINED, 1e-8)
     assert a[0] == b[0]
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])

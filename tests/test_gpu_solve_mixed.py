@@ -76,3 +76,26 @@ def test_mixed_bc_fp32_refined(oracle, solver):
     assert rel < 1e-8
     assert dth < 1e-3
     wk.close()
+
+
+@pytest.mark.parametrize("solver", ["pbicgstab", "cg"])
+def test_mixed_bc_no_smoother_bitwise(oracle, solver):
+    """Preconditioner! :none (a copy) on the mixed-condition case, bitwise
+    against the oracle; CG! then forms ρ with a separate dot pass."""
+    print("case no smoother", solver, flush=True)
+    dt = 0.25
+    wk, c, rng = make_pair(oracle, 40, 40, 30, seed=13, bc_set=mixed_bc(), random_lam="classes", steady=False, dt=dt)
+    shape = c.shape
+    put(wk, c, "hsrc", rng.uniform(0.0, 2.0e11, shape))
+    put(wk, c, "b", np.zeros(shape))
+    wk.calc_rhs(dt, False)
+    with oracle.sum_mode(oracle.EXACT):
+        c.cal_rhs()
+        conv, itr, res0, hist = c.solve(solver, tol=1e-6, maxItr=3000, smoother=0)
+    s = L.SOLVER_CG if solver == "cg" else L.SOLVER_PBICGSTAB
+    out = wk.solve(s, dt, 1e-6, 3000, L.SMOOTHER_NONE, False)
+    print("  oracle", conv, itr, res0, flush=True)
+    assert out == (conv, itr, res0)
+    assert np.array_equal(wk.history(), hist)
+    assert np.array_equal(wk.download("theta"), c.arr["theta"])
+    wk.close()
