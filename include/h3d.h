@@ -94,8 +94,9 @@ extern "C" {
                                   C >= 1 = the two colours of a sweep in one launch marching chunks of C
                                   z-planes (C capped at NZ and 125; operators with more than 16 classes
                                   keep one launch per colour), -1 = chosen from the grid (default: chunks
-                                  of 60 planes when NZ >= 120, else one launch per colour); bitwise
-                                  identical */
+                                  of 60 planes when NZ >= 120, else one launch per colour); h3d_cg
+                                  runs its last sweep per colour either way (that pass also forms
+                                  rho = (r, z)); bitwise identical */
 
 typedef struct h3d_ctx h3d_ctx;
 
