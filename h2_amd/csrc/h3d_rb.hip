@@ -432,10 +432,18 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
 // Rows per thread of the sweep, by precision and grid size (measured, HIP
 // events, one full sweep): FP64 one row on small grids (17.7 against
 // 18.4 us at 240x240x30), two on large ones (91.0 against 95.8 us at
-// 480x480x60, 1025 against 1078 us at 960x960x120); FP32 two rows on
-// small grids (12.2 against 13.4 us with four), four on large ones (56.4
-// against 58.0 us, 540 against 559 us).  Large: at least 2^21 cells per
-// half.
+// 480x480x60, 1025 against 1078 us at 960x960x120; three rows, at 124
+// VGPRs, 86 against 82 us at 480x480x60); FP32 two rows on small grids
+// (12.2 against 13.4 us with four), three on large ones (51.9 against
+// 58.4 us with four at 480x480x60, 548 against 615 us at 960x960x120:
+// 73-77 VGPRs keep six waves per SIMD, four rows' 97 only four).  Large:
+// at least 2^21 cells per half.
+template <typename T>
+constexpr int rb_rows_large()
+{
+    return sizeof(T) == 4 ? 3 : 2;
+}
+
 template <typename T, int R>
 static void rb_sweep_rows(T *x, const T *b, const RbGeo<T> &g, int h, bool from_zero, const KState *st,
                           hipStream_t s)
@@ -459,7 +467,7 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
 {
     constexpr int R = sizeof(T) == 4 ? 2 : 1;
     const int h = 1 - color;
-    if (g.HS >= ((size_t)1 << 21)) rb_sweep_rows<T, 2 * R>(x, b, g, h, from_zero, st, s);
+    if (g.HS >= ((size_t)1 << 21)) rb_sweep_rows<T, rb_rows_large<T>()>(x, b, g, h, from_zero, st, s);
     else rb_sweep_rows<T, R>(x, b, g, h, from_zero, st, s);
     return hipGetLastError();
 }
@@ -480,8 +488,9 @@ hipError_t launch_rb_sweep_dot(T *x, const T *b, const RbGeo<T> &g, int fin, KSt
     const size_t own = 0, oth = g.HS;
     unsigned nb;
     if (g.HS >= ((size_t)1 << 21)) {
-        nb = rb_rows_nb<2 * R0>(g);
-        hipLaunchKernelGGL((k_rb_sweep<T, 2 * R0, false, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+        constexpr int RL = rb_rows_large<T>();
+        nb = rb_rows_nb<RL>(g);
+        hipLaunchKernelGGL((k_rb_sweep<T, RL, false, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
                            x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, b + oth, rb);
     } else {
         nb = rb_rows_nb<R0>(g);
