@@ -64,6 +64,7 @@ struct h3d_ctx {
     float *g32[F32_N] = {};
     double *d_corr = nullptr;    // FP64 correction accumulated by the FP32 inner solve
     double *d_best = nullptr;    // its iterate with the smallest inner residual
+    double *d_prev = nullptr;    // θ before the last correction (taken back if it raised the residual)
     // colour-split (RB) layout of the Krylov vectors: the solver's layout
     // whenever the class table form is active (H3D_OPT_LAYOUT)
     int layout = H3D_LAYOUT_AUTO;
@@ -86,14 +87,15 @@ struct h3d_ctx {
     float *rb32[8] = {};
     // instantiated iteration graphs, reused while nothing they captured changed
     struct GraphKey {
-        int solver, smoother, tsize, steady, rb;
+        int solver, smoother, tsize, steady, rb, seq;
+        uint64_t ddt_bits;                // 1/Δt the colour-split kernels form ap0 from (RbGeo::ddt)
         unsigned long lam_v, grid_v;
         const void *hist, *partials, *x;
         bool operator==(const GraphKey &o) const
         {
             return solver == o.solver && smoother == o.smoother && tsize == o.tsize && steady == o.steady &&
-                   rb == o.rb && lam_v == o.lam_v && grid_v == o.grid_v && hist == o.hist &&
-                   partials == o.partials && x == o.x;
+                   rb == o.rb && seq == o.seq && ddt_bits == o.ddt_bits && lam_v == o.lam_v &&
+                   grid_v == o.grid_v && hist == o.hist && partials == o.partials && x == o.x;
         }
     };
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
@@ -516,8 +518,11 @@ template <typename T>
 int iteration_graph(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, int smoother,
                     hipGraphExec_t *out)
 {
-    h3d_ctx::GraphKey key{solver, smoother, (int)sizeof(T), P.g.ap0 != nullptr, P.rb ? 1 : 0,
-                          c->cls_build,     c->grid_version, c->d_hist, c->rb.partials, v.x};
+    uint64_t ddt_bits = 0;
+    std::memcpy(&ddt_bits, &P.rg.ddt, sizeof(ddt_bits));
+    h3d_ctx::GraphKey key{solver,      smoother,        (int)sizeof(T), P.g.ap0 != nullptr, P.rb ? 1 : 0,
+                          c->rb.seq,   P.rb ? ddt_bits : 0, c->cls_build, c->grid_version,  c->d_hist,
+                          c->rb.partials, v.x};
     for (auto &e : c->graphs)
         if (e.first == key) {
             *out = e.second;
@@ -824,6 +829,7 @@ int ensure_fp32(h3d_ctx *c)
         if (!c->g32[a]) HIPCHK(c, hipMalloc(&c->g32[a], c->H * sizeof(float)));
     if (!c->d_corr) HIPCHK(c, hipMalloc(&c->d_corr, c->H * sizeof(double)));
     if (!c->d_best) HIPCHK(c, hipMalloc(&c->d_best, c->H * sizeof(double)));
+    if (!c->d_prev) HIPCHK(c, hipMalloc(&c->d_prev, c->H * sizeof(double)));
     if (!c->d_ZC32) HIPCHK(c, hipMalloc(&c->d_ZC32, c->MZ * sizeof(float)));
     if (!c->d_dZ32) HIPCHK(c, hipMalloc(&c->d_dZ32, c->MZ * sizeof(float)));
     return H3D_OK;
@@ -908,6 +914,7 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
     const double kFloorUlps = 9.0;
     double rn_prev = 0.0;
     int slow = 0;
+    bool applied = false;  // θ holds a correction and d_prev the iterate before it
     c->refine_floor = 0;
     for (int outer_it = 0; outer_it <= max_outer; ++outer_it) {
         // true FP64 residual r = b - Aθ
@@ -924,6 +931,17 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         }
         if (!std::isfinite(rn)) { nan = true; break; }
         if (outer_it > 0 && rn / res0 < tol) { converged = true; break; }
+        if (applied && rn > rn_prev) {
+            // the last correction raised the true residual: take it back (θ
+            // returns to the iterate of residual rn_prev, the last entry of
+            // the history) and stop unconverged
+            HIPCHK(c, hipMemcpyAsync(c->f[H3D_F_THETA], c->d_prev, c->H * sizeof(double), hipMemcpyDeviceToDevice, s));
+            outer.pop_back();
+            if (c->verbose)
+                std::fprintf(stderr, "h3d refine: outer %d true res %.6e above %.6e: correction taken back\n",
+                             outer_it, rn / res0, rn_prev / res0);
+            break;
+        }
         if (outer_it > 0) {
             HIPCHK(c, launch_rk_scale<double>(c->f[H3D_F_THETA], c->f[H3D_F_B], g64, c->d_unit, c->rb, s));
             HIPCHK(c, hipMemcpyAsync(&c->h_state[0], c->d_unit, sizeof(KState), hipMemcpyDeviceToHost, s));
@@ -965,7 +983,9 @@ int solve_refined(h3d_ctx *c, int solver, double dt, double tol, int maxItr, int
         // without a copy, the last iterate only where it reduced the residual
         if (!snap && !(st.res < 1.0)) break;
         // st.improved: the last iterate is the best one; else the snapshot is
+        HIPCHK(c, hipMemcpyAsync(c->d_prev, c->f[H3D_F_THETA], c->H * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHK(c, launch_add_correction(c->f[H3D_F_THETA], (snap && !st.improved) ? v.xbest : v.x, g64, s));
+        applied = true;
     }
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -1092,7 +1112,7 @@ int h3d_destroy(h3d_ctx *c)
         if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_DZC32, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
-                  c->d_corr, c->d_best};
+                  c->d_corr, c->d_best, c->d_prev};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
@@ -1133,6 +1153,11 @@ int h3d_set_option(h3d_ctx *c, int option, int value)
         c->grid_version++;  // captured iterations carry the sweep sequence
         return H3D_OK;
     }
+    if (option == H3D_OPT_SUM_ORDER) {
+        if (value != H3D_SUM_EXACT && value != H3D_SUM_SEQUENTIAL) return fail(c, H3D_ERR_ARG, "bad sum order");
+        c->rb.seq = value == H3D_SUM_SEQUENTIAL ? 1 : 0;  // in the graph key
+        return H3D_OK;
+    }
     return fail(c, H3D_ERR_ARG, "unknown option");
 }
 
@@ -1157,6 +1182,10 @@ int h3d_get_option(const h3d_ctx *c, int option, int *value)
     }
     if (option == H3D_OPT_SWEEP_CHUNK) {
         *value = c->sweep_chunk;
+        return H3D_OK;
+    }
+    if (option == H3D_OPT_SUM_ORDER) {
+        *value = c->rb.seq ? H3D_SUM_SEQUENTIAL : H3D_SUM_EXACT;
         return H3D_OK;
     }
     return H3D_ERR_ARG;

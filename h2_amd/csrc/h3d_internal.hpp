@@ -65,8 +65,21 @@ struct RedBuf {
     double *partials;       // [nblocks_max][3][2]
     double *partials2;      // [nblocks_max/1024 + 1][3][2]
     int nblocks_max;
+    int seq;                // 1: the finalize steps use reference-order sums (launch_finalize_sums)
 };
 constexpr unsigned kFinalizeMax = 4096;
+
+// Terms of up to two grid sums for the reference-order mode: sum s adds
+// x[s][c]·(y[s] ? y[s][c] : x[s][c]) over the interior cells c in the
+// reference's loop order (k outer, then j, then i), one running FP64 sum
+// from 0.0, as Fdot1/Fdot2 and CalcRK! sum under FLoops' SequentialEx
+// (src/NonUniform.jl:372-434, 567-592; src/common.jl:42).
+template <typename T>
+struct SeqTerms {
+    const T *x[2], *y[2];
+    int NX, NY, NZ, MX, MY, HX;
+    size_t HS;              // 0: the reference's layout; else the colour-split one with this half size
+};
 
 // Per-solve operator constants.  T is the storage/arithmetic type of the
 // stencil (double for the reference iteration, float for the FP32 inner
@@ -153,6 +166,42 @@ template <typename T> void rb_shape(RbGeo<T> &g);
 // Sum nb workgroup partials of ns sums each and run finalizer fin.
 hipError_t launch_finalize_nb(int ns, unsigned nb, int fin, KState *st, double *hist, const RedBuf &rb,
                               hipStream_t s);
+// The ns sums of t in the reference's order (one workgroup), then finalizer fin.
+template <typename T>
+hipError_t launch_finalize_seq(int ns, const SeqTerms<T> &t, int fin, KState *st, double *hist, hipStream_t s);
+
+template <typename T>
+inline SeqTerms<T> seq_terms(const Geo<T> &g, const T *x0, const T *y0, const T *x1 = nullptr,
+                             const T *y1 = nullptr)
+{
+    SeqTerms<T> t{};
+    t.x[0] = x0; t.y[0] = y0; t.x[1] = x1; t.y[1] = y1;
+    t.NX = g.NX; t.NY = g.NY; t.NZ = g.NZ; t.MX = g.MX; t.MY = g.MY;
+    t.HX = 0; t.HS = 0;
+    return t;
+}
+
+template <typename T>
+inline SeqTerms<T> seq_terms(const RbGeo<T> &g, const T *x0, const T *y0, const T *x1 = nullptr,
+                             const T *y1 = nullptr)
+{
+    SeqTerms<T> t{};
+    t.x[0] = x0; t.y[0] = y0; t.x[1] = x1; t.y[1] = y1;
+    t.NX = g.NX; t.NY = g.NY; t.NZ = g.NZ; t.MX = g.MX; t.MY = g.MY;
+    t.HX = g.HX; t.HS = g.HS;
+    return t;
+}
+
+// Finalize step of a reducing launch of nb workgroups: the exactly rounded
+// sums of its partials, or, in the reference-order mode (rb.seq), the
+// sequential sums of the terms t, re-read from the vectors the launch wrote.
+template <typename T>
+inline hipError_t launch_finalize_sums(int ns, unsigned nb, int fin, KState *st, double *hist, const RedBuf &rb,
+                                       hipStream_t s, const SeqTerms<T> &t)
+{
+    if (rb.seq) return launch_finalize_seq<T>(ns, t, fin, st, hist, s);
+    return launch_finalize_nb(ns, nb, fin, st, hist, rb, s);
+}
 template <typename T> hipError_t launch_fill(T *a, T v, size_t n, hipStream_t s);
 template <typename T>
 hipError_t launch_copy_interior(T *dst, const T *src, const Geo<T> &g, const KState *st, hipStream_t s);

@@ -375,6 +375,75 @@ hipError_t launch_finalize_nb(int ns, unsigned nb, int fin, KState *st, double *
     return hipGetLastError();
 }
 
+// Reference-order grid sums (RedBuf::seq), one workgroup of NT threads.
+// Waves 1-3 form the terms of the next tile of kSeqTile interior cells, in
+// the reference's loop order, into one LDS buffer while lane s of wave 0
+// adds the current tile's terms of sum s one after the other from the
+// other buffer; one barrier per tile.  Then the finaliser, as k_finalize.
+// A sequential running sum has no parallel form, so this runs at the rate
+// of one dependent FP64 addition per cell (milliseconds at 240x240x30).
+constexpr int kSeqTile = 1536;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_finalize_seq(SeqTerms<T> t, int ns, int fin, KState *st, double *hist)
+{
+    __shared__ double buf[2][2][kSeqTile];
+    if (halted(st)) return;
+    const int tid = threadIdx.x;
+    const unsigned NX = (unsigned)t.NX, NXY = NX * (unsigned)t.NY;
+    const unsigned n = NXY * (unsigned)t.NZ;
+    const unsigned ntiles = (n + kSeqTile - 1) / kSeqTile;
+    auto form = [&](unsigned tile, int slot) {
+        for (int e = tid - 64; e < kSeqTile; e += NT - 64) {
+            const unsigned q = tile * (unsigned)kSeqTile + (unsigned)e;
+            if (q >= n) break;
+            const unsigned k = q / NXY, rem = q - k * NXY, j = rem / NX, i = rem - j * NX;
+            // stored-array coordinates: the interior starts at 1
+            const unsigned ii = i + 1, jj = j + 1, kk = k + 1;
+            const size_t row = (size_t)jj + (size_t)t.MY * kk;
+            const size_t a = t.HS ? (((ii + jj + kk) & 1u) ? t.HS : 0) + (ii >> 1) + (size_t)t.HX * row
+                                  : ii + (size_t)t.MX * row;
+            for (int s = 0; s < ns; ++s) {
+                const double xv = (double)t.x[s][a];
+                buf[slot][s][e] = xv * (t.y[s] ? (double)t.y[s][a] : xv);
+            }
+        }
+    };
+    if (tid >= 64) form(0, 0);
+    __syncthreads();
+    double acc = 0.0;
+    for (unsigned tile = 0; tile < ntiles; ++tile) {
+        const int slot = (int)(tile & 1u);
+        if (tid >= 64) {
+            if (tile + 1 < ntiles) form(tile + 1, slot ^ 1);
+        } else if (tid < ns) {
+            const unsigned cnt = min((unsigned)kSeqTile, n - tile * (unsigned)kSeqTile);
+            const double *b = buf[slot][tid];
+            for (unsigned e = 0; e < cnt; ++e) acc = acc + b[e];
+        }
+        __syncthreads();
+    }
+    if (tid < 64) {
+        const double s1 = __shfl(acc, 1, 64);
+        if (tid == 0) {
+            KState sv = *st;
+            double tot[3] = {acc, ns > 1 ? s1 : 0.0, 0.0};
+            finalize(fin, tot, &sv, hist);
+            *st = sv;
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_finalize_seq(int ns, const SeqTerms<T> &t, int fin, KState *st, double *hist, hipStream_t s)
+{
+    if (ns < 1 || ns > 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_finalize_seq<T>, dim3(1), dim3(NT), 0, s, t, ns, fin, st, hist);
+    return hipGetLastError();
+}
+template hipError_t launch_finalize_seq<double>(int, const SeqTerms<double> &, int, KState *, double *, hipStream_t);
+template hipError_t launch_finalize_seq<float>(int, const SeqTerms<float> &, int, KState *, double *, hipStream_t);
+
 // ---------------------------------------------------------------------------
 // launch geometry
 // ---------------------------------------------------------------------------
@@ -405,6 +474,14 @@ static hipError_t launch_finalize(const Shape &sh, int fin, KState *st, double *
                                   hipStream_t s)
 {
     return launch_finalize_nb(NS, sh.nb(), fin, st, hist, rb, s);
+}
+
+// The same with the terms for the reference-order mode (RedBuf::seq)
+template <int NS, typename T>
+static hipError_t launch_finalize(const Shape &sh, int fin, KState *st, double *hist, const RedBuf &rb,
+                                  hipStream_t s, const SeqTerms<T> &t)
+{
+    return launch_finalize_sums<T>(NS, sh.nb(), fin, st, hist, rb, s, t);
 }
 
 #define INTERIOR_COLUMN(g, KC)                                              \
@@ -491,7 +568,8 @@ hipError_t launch_calc_ax(T *ax, const T *x, const Geo<T> &g, const T *dot_y, co
 #undef AX_LAUNCH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ns == 0) return e;
-    return ns == 1 ? launch_finalize<1>(sh, fin, st, hist, rb, s) : launch_finalize<2>(sh, fin, st, hist, rb, s);
+    const SeqTerms<T> t = seq_terms<T>(g, ax, dot_y, ax, dot_z);
+    return ns == 1 ? launch_finalize<1>(sh, fin, st, hist, rb, s, t) : launch_finalize<2>(sh, fin, st, hist, rb, s, t);
 }
 
 // CalcRK! (src/NonUniform.jl:372-434): r = (b - A θ)·m and Σ r².  With
@@ -530,7 +608,7 @@ hipError_t launch_calc_rk(T *r, T *r0copy, const T *th, const T *b, const Geo<T>
     else
         hipLaunchKernelGGL((k_calc_rk<T, false>), sh.grid, sh.block, 0, s, r, r0copy, th, b, g, st, rb, sh.KC);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s, seq_terms<T>(g, r, nullptr));
 }
 
 // Σ s², s = |b| + Σ|terms of A θ| at each cell, the terms taken in the form
@@ -857,11 +935,11 @@ hipError_t launch_dot(const T *x, const T *y, const T *z, const Geo<T> &g, int f
     if (z == nullptr) {
         hipLaunchKernelGGL((k_dot<T, 1>), sh.grid, sh.block, 0, s, x, y, z, g, st, rb, sh.KC);
         e = hipGetLastError();
-        return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
+        return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s, seq_terms<T>(g, x, y));
     }
     hipLaunchKernelGGL((k_dot<T, 2>), sh.grid, sh.block, 0, s, x, y, z, g, st, rb, sh.KC);
     e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize<2>(sh, fin, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize<2>(sh, fin, st, hist, rb, s, seq_terms<T>(g, x, y, x, z));
 }
 
 // BiCG1! / Triad! / BICG2! (src/NonUniform.jl:603-659) with host scalars.
@@ -930,12 +1008,15 @@ hipError_t launch_bicg_update(double *th, double *xbest, T *r, const T *p_, cons
         hipLaunchKernelGGL((k_bicg_update<T, 1>), sh.grid, sh.block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st,
                            rb, sh.KC);
         e = hipGetLastError();
-        return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_BICG_RES, st, hist, rb, s);
+        return e != hipSuccess ? e
+                               : launch_finalize<1>(sh, FIN_BICG_RES, st, hist, rb, s, seq_terms<T>(g, r, nullptr));
     }
     hipLaunchKernelGGL((k_bicg_update<T, 2>), sh.grid, sh.block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st, rb,
                        sh.KC);
     e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize<2>(sh, FIN_BICG_RES_RHO, st, hist, rb, s);
+    return e != hipSuccess ? e
+                           : launch_finalize<2>(sh, FIN_BICG_RES_RHO, st, hist, rb, s,
+                                                seq_terms<T>(g, r, nullptr, r, r0));
 }
 
 // CG!: θ += α p, r -= α q, Σ r² (NonUniform.jl:179-186)
@@ -970,7 +1051,7 @@ hipError_t launch_cg_update(double *th, double *xbest, T *r, const T *p, const T
     Shape sh = interior_shape(g.NX, g.NY, g.NZ);
     hipLaunchKernelGGL(k_cg_update<T>, sh.grid, sh.block, 0, s, th, xbest, r, p, q, g, st, rb, sh.KC);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_CG_RES, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize<1>(sh, FIN_CG_RES, st, hist, rb, s, seq_terms<T>(g, r, nullptr));
 }
 
 // CG!: p = z + β p (NonUniform.jl:218-220)

@@ -498,7 +498,7 @@ hipError_t launch_rb_sweep_dot(T *x, const T *b, const RbGeo<T> &g, int fin, KSt
                            x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, b + oth, rb);
     }
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(1, nb, fin, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize_sums<T>(1, nb, fin, st, hist, rb, s, seq_terms<T>(g, b, x));
 }
 
 // ---------------------------------------------------------------------------
@@ -967,7 +967,7 @@ hipError_t launch_rb_calc_ax(T *ax, const T *x, const RbGeo<T> &g, const T *dot_
     else hipLaunchKernelGGL((k_rb_calc_ax<T, 2, R>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ns == 0) return e;
-    return launch_finalize_nb(ns, nb, fin, st, hist, rb, s);
+    return launch_finalize_sums<T>(ns, nb, fin, st, hist, rb, s, seq_terms<T>(g, ax, dot_y, ax, dot_z));
 }
 
 // CalcRK! (src/NonUniform.jl:372-434) with θ, b in standard layout; r (and
@@ -1033,7 +1033,7 @@ hipError_t launch_rb_calc_rk(T *r, T *r0copy, const T *th, const T *b, const RbG
 {
     hipLaunchKernelGGL(k_rb_calc_rk<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, r, r0copy, th, b, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, fin, st, hist, rb, s);
+    return e != hipSuccess ? e : launch_finalize_sums<T>(1, g.nb1, fin, st, hist, rb, s, seq_terms<T>(g, r, nullptr));
 }
 
 // Fdot1/Fdot2 (src/NonUniform.jl:567-592): s0 = Σ x·(y ? y : x), s1 = Σ x·z
@@ -1073,7 +1073,8 @@ hipError_t launch_rb_dot(const T *x, const T *y, const T *z, const RbGeo<T> &g, 
     if (z) hipLaunchKernelGGL((k_rb_dot<T, 2>), grid, block, 0, s, x, y, z, g, st, rb);
     else hipLaunchKernelGGL((k_rb_dot<T, 1>), grid, block, 0, s, x, y, z, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(z ? 2 : 1, g.nb1, fin, st, hist, rb, s);
+    return e != hipSuccess ? e
+                           : launch_finalize_sums<T>(z ? 2 : 1, g.nb1, fin, st, hist, rb, s, seq_terms<T>(g, x, y, x, z));
 }
 
 // BICG2!(θ, p_, s_, alpha, omega); Triad!(r, t_, s, -omega); Σ r², and with
@@ -1138,11 +1139,15 @@ hipError_t launch_rb_bicg_update(double *th, double *xbest, T *r, const T *p_, c
     if (r0 == nullptr) {
         hipLaunchKernelGGL((k_rb_bicg_update<T, 1>), grid, block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st, rb);
         hipError_t e = hipGetLastError();
-        return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, FIN_BICG_RES, st, hist, rb, s);
+        return e != hipSuccess ? e
+                               : launch_finalize_sums<T>(1, g.nb1, FIN_BICG_RES, st, hist, rb, s,
+                                                         seq_terms<T>(g, r, nullptr));
     }
     hipLaunchKernelGGL((k_rb_bicg_update<T, 2>), grid, block, 0, s, th, xbest, r, p_, s_, t, sv, r0, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(2, g.nb1, FIN_BICG_RES_RHO, st, hist, rb, s);
+    return e != hipSuccess ? e
+                           : launch_finalize_sums<T>(2, g.nb1, FIN_BICG_RES_RHO, st, hist, rb, s,
+                                                     seq_terms<T>(g, r, nullptr, r, r0));
 }
 
 // CG!: θ += α p, r -= α q, Σ r² (NonUniform.jl:179-186)
@@ -1194,7 +1199,9 @@ hipError_t launch_rb_cg_update(double *th, double *xbest, T *r, const T *p, cons
 {
     hipLaunchKernelGGL(k_rb_cg_update<T>, dim3(g.nb1), dim3(BX, BY), 0, s, th, xbest, r, p, q, g, st, rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, FIN_CG_RES, st, hist, rb, s);
+    return e != hipSuccess ? e
+                           : launch_finalize_sums<T>(1, g.nb1, FIN_CG_RES, st, hist, rb, s,
+                                                     seq_terms<T>(g, r, nullptr));
 }
 
 // CG!'s update followed by the first colour of its preconditioner, in one
@@ -1264,7 +1271,9 @@ hipError_t launch_rb_cg_first(double *th, double *xbest, T *r, const T *p, const
     hipLaunchKernelGGL(k_rb_cg_first<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, th, xbest, r, p, q, z + g.HS, g, st,
                        rb);
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_finalize_nb(1, g.nb1, FIN_CG_RES, st, hist, rb, s);
+    return e != hipSuccess ? e
+                           : launch_finalize_sums<T>(1, g.nb1, FIN_CG_RES, st, hist, rb, s,
+                                                     seq_terms<T>(g, r, nullptr));
 }
 
 // CG!: p = z + β p (NonUniform.jl:218-220)

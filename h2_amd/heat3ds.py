@@ -275,6 +275,13 @@ class WorkBuffers:
         planes, -1 = chosen from the grid (default).  Bitwise identical."""
         _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_SWEEP_CHUNK, int(chunk)))
 
+    def set_sum_order(self, order):
+        """L.SUM_EXACT (exactly rounded grid sums, default) or L.SUM_SEQUENTIAL
+        (the reference's sequential (k, j, i) running sums: a solve then
+        reproduces the reference's sequential run bit for bit, at the cost of
+        one single-workgroup pass per sum)."""
+        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_SUM_ORDER, int(order)))
+
     def num_classes(self):
         v = ctypes.c_int()
         _check(self.ctx, self.lib.h3d_get_option(self.ctx, L.OPT_NUM_CLASSES, ctypes.byref(v)))

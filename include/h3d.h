@@ -97,6 +97,15 @@ extern "C" {
                                   of 60 planes when NZ >= 120, else one launch per colour); h3d_cg
                                   runs its last sweep per colour either way (that pass also forms
                                   rho = (r, z)); bitwise identical */
+#define H3D_OPT_SUM_ORDER 5    /* order of the Krylov solvers' grid sums (dots, residual norms) */
+#define H3D_SUM_EXACT 0        /* exactly rounded, independent of order (default; parallel, fast) */
+#define H3D_SUM_SEQUENTIAL 1   /* the reference's sequential order: one running FP64 sum over the
+                                  interior cells, k outer, then j, then i (Fdot1/Fdot2/CalcRK! under
+                                  FLoops' SequentialEx, src/NonUniform.jl:372-434, 567-592, with
+                                  src/common.jl:42), so a solve reproduces the reference's sequential
+                                  run bit for bit; each sum runs on one workgroup (milliseconds per
+                                  sum at 240x240x30), for parity runs, not speed.  The RB-SOR
+                                  solver's and the sensitivity extension's sums stay exact. */
 
 typedef struct h3d_ctx h3d_ctx;
 
@@ -157,7 +166,9 @@ int h3d_calc_rhs(h3d_ctx *ctx, double dt, int is_steady);
  * refinement also reports convergence when the true residual is within
  * 9u||s|| (u = 2^-53, s = |b| + Σ|terms of Aθ| per cell), the error bound
  * of its own FP64 evaluation (h3d_last_solve_floor then gives 1), and it
- * stops unconverged after two corrections in a row that did not halve it.
+ * stops unconverged after two corrections in a row that did not halve it,
+ * or after a correction that raised it, which it takes back (θ is then the
+ * iterate whose residual is the history's last entry).
  * No counterpart in the reference: its solvers test the recursive residual. */
 int h3d_pbicgstab(h3d_ctx *ctx, double dt, double tol, int maxItr, int smoother,
                   int is_steady, int precision, double inner_tol,

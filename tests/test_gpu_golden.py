@@ -79,3 +79,50 @@ def test_c2_against_reference_log(c2):
     assert np.max(np.abs(th[mid, mid, :] - g["zline_center"])) < 1e-6
     s = th[1:-1, 1:-1, 1:-1]
     assert round(float(s.max()), 1) == float(pins["theta_max_K"])
+
+
+def test_c2_sequential_sums_reproduce_reference_run(c2):
+    """With the reference's sequential grid sums (H3D_SUM_SEQUENTIAL) the
+    device reproduces the reference's published 240x240x30 run: res0, 208
+    iterations, the README's residual lines of iterations 1, 2 and 208, and
+    θ bit for bit equal to the oracle's sequential run (SHA-256 of the whole
+    array); the default exactly-rounded solve is compared with it in K."""
+    wk, th0 = c2
+    pins = json.load(open(os.path.join(GOLD, "readme_pins.json")))
+    g = np.load(os.path.join(GOLD, "c2_pbicgstab_seq.npz"))
+    wk.set_sum_order(L.SUM_SEQUENTIAL)
+    try:
+        conv, itr, res0, hist, th, b = solve(wk, th0, L.SOLVER_PBICGSTAB, 1e-4)
+    finally:
+        wk.set_sum_order(L.SUM_EXACT)
+    assert conv and itr == pins["converged_at"] == int(g["itr"])
+    assert repr(res0) == pins["res0"]
+    for k, line in pins["history"].items():
+        assert "%24.14E" % hist[int(k) - 1] == "%24s" % line
+    assert np.array_equal(hist, g["history"])
+    assert digest(th) == str(g["theta_sha256"])
+    s = th[1:-1, 1:-1, 1:-1]
+    assert "%.1f" % s.max() == pins["theta_max_K"]
+    # the default (exactly rounded) solve of the same case against it
+    conv_e, itr_e, _, _, th_e, _ = solve(wk, th0, L.SOLVER_PBICGSTAB, 1e-4)
+    err = float(np.max(np.abs(th_e - th)))
+    print("exact-sum solve: %d iterations, max |θ - θ_seq| = %.3e K" % (itr_e, err))
+    assert conv_e and err < 1e-2
+
+
+@pytest.mark.parametrize("name", ["pbicgstab", "cg"])
+def test_c1_sequential_sums_bitwise_vs_seq_fixture(name):
+    """40x40x30 at ε = 1e-6 with sequential sums, against the oracle's
+    sequential run: iteration count, history and θ bit for bit."""
+    g = np.load(os.path.join(GOLD, "c1_%s_seq.npz" % name))
+    m = h2_amd.build_model(40, 40, 30)
+    with WorkBuffers(*m.shape) as wk:
+        load_model(wk, m)
+        wk.set_sum_order(L.SUM_SEQUENTIAL)
+        th0 = wk.download("theta")
+        solver = L.SOLVER_PBICGSTAB if name == "pbicgstab" else L.SOLVER_CG
+        conv, itr, res0, hist, th, b = solve(wk, th0, solver, 1e-6)
+    assert conv and itr == int(g["itr"])
+    assert res0 == float(g["res0"])
+    assert np.array_equal(hist, g["history"])
+    assert digest(th) == str(g["theta_sha256"])

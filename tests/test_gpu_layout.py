@@ -51,10 +51,12 @@ def test_fp32_refined_layouts_bitwise(shape, solver):
     assert np.array_equal(a[2], b[2])
 
 
-def run_limited(shape, layout, solver, steady, precision, max_itr):
+def run_limited(shape, layout, solver, steady, precision, max_itr, chunk=None):
     m = h2_amd.build_model(*shape)
     wk = WorkBuffers(*m.shape)
     wk.set_layout(layout)
+    if chunk is not None:
+        wk.set_sweep_chunk(chunk)
     load_model(wk, m)
     dt = 10000.0 if steady else 1e-3
     wk.calc_rhs(dt, steady)
@@ -76,3 +78,49 @@ def test_large_grid_layouts_bitwise(solver, steady, precision):
     assert a[0] == b[0]
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32_REFINED])
+def test_large_grid_two_colour_sweeps_bitwise(precision):
+    """The two-colour sweep launches (chunks of 30 planes) with CG!, whose
+    preconditioner then starts in the swap buffer and forms ρ in a last
+    per-colour pass, on 260x260x60 (three FP32 rows per thread): bitwise the
+    per-colour launches and the reference's layout, 30 iterations."""
+    print("case 260x260x60 CG transient chunk 30 precision", precision, flush=True)
+    a = run_limited((260, 260, 60), L.LAYOUT_AUTO, L.SOLVER_CG, False, precision, 30, chunk=30)
+    b = run_limited((260, 260, 60), L.LAYOUT_AUTO, L.SOLVER_CG, False, precision, 30, chunk=0)
+    c = run_limited((260, 260, 60), L.LAYOUT_STANDARD, L.SOLVER_CG, False, precision, 30)
+    for o in (b, c):
+        assert a[0] == o[0]
+        assert np.array_equal(a[1], o[1])
+        assert np.array_equal(a[2], o[2])
+
+
+def transient_solve(wk, th0, dt, solver, precision):
+    wk.upload("theta", th0)
+    wk.fill("b", 0.0)
+    wk.calc_rhs(dt, False)
+    res = wk.solve(solver, dt, 1e-10, 3000, L.SMOOTHER_GS, False, precision=precision)
+    return res, wk.history(), wk.download("theta")
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32_REFINED])
+@pytest.mark.parametrize("solver", [L.SOLVER_PBICGSTAB, L.SOLVER_CG])
+def test_transient_dt_change_on_one_context(solver, precision):
+    """A second transient solve with another Δt on the same context (whose
+    captured iteration carries 1/Δt) equals the same solve on a fresh
+    context and in the reference's layout, bit for bit."""
+    m = h2_amd.build_model(40, 40, 30)
+    out = []
+    for layout, dts in ((L.LAYOUT_AUTO, (1e-3, 2e-3)), (L.LAYOUT_AUTO, (2e-3,)), (L.LAYOUT_STANDARD, (2e-3,))):
+        with WorkBuffers(*m.shape) as wk:
+            wk.set_layout(layout)
+            load_model(wk, m)
+            th0 = wk.download("theta")
+            for dt in dts:
+                r = transient_solve(wk, th0, dt, solver, precision)
+            out.append(r)
+    for o in out[1:]:
+        assert out[0][0] == o[0]
+        assert np.array_equal(out[0][1], o[1])
+        assert np.array_equal(out[0][2], o[2])
