@@ -9,7 +9,16 @@ HBM.  θ is reset to the initial state before every step (untimed).
 
 Printed JSON line (one, on stdout):
   value      algorithmic HBM GB/s of the whole solve:
-             560 B/cell/iteration (FP64) x cells x iterations / step time
+             560 B/cell/iteration (FP64, the reference's op sequence: ten
+             sweeps, two SpMV, five dots, BiCG1!, two Triad!, BICG2! and
+             two fills) x cells x iterations / step time; value_fused is
+             the same on the 480 B basis of the fused minimum (no fills)
+  parity     the same case with the reference's sequential grid sums
+             (H3D_SUM_SEQUENTIAL): its iteration count, time, the README
+             residual lines and whether θ's SHA-256 equals the committed
+             fixture of the oracle's sequential run; max_abs_err_K is the
+             largest |θ| difference (K, whole field) of the default solve
+             (exactly rounded sums) from it
   roofline   dominant kernel (one red-black GS colour, k_rb_sweep):
              algorithmic bytes per launch / mean launch time (HIP events
              around 200 launches on the context stream); traffic = HBM
@@ -28,6 +37,17 @@ Printed JSON line (one, on stdout):
              transient case from 300 K, FP64 PBiCGSTAB! to 1e-8: iterations,
              time and the reference-equivalent HBM rate; the same step with
              FP32-refined CG!
+
+Profiling workloads (no JSON line; run under rocprofv3):
+  --profile sweep [--nx 240|480]   the modelA case, a 3-iteration solve,
+             three plain copies of one H-cell FP64 field (the counters'
+             calibration), then 20 full red-black sweeps (40 launches of
+             k_rb_sweep<double>)
+  --profile solve [--nx 240|480]   the headline solve, twice (two graph
+             replays of the same process), for kernel traces
+  --profile transient [--solver cg --precision fp32]   the 960 x 960 x 120
+             power-map case, one implicit-Euler step's first 40 iterations,
+             the copy calibration, then 10 full sweeps
 """
 import argparse
 import json
@@ -41,6 +61,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_ITER = {"fp64": 560.0, "fp32": 280.0}
+FUSED_BYTES_PER_CELL_ITER = 480.0  # the same op sequence without the two fills
 SWEEP_BYTES_PER_CELL = 32.0  # x, λ, rhs read + x written, FP64, one full red-black sweep
 
 
@@ -57,7 +78,12 @@ def parse():
     ap.add_argument("--no-c3", action="store_true", help="skip the 480x480x60 precision comparison")
     ap.add_argument("--no-c5", action="store_true", help="skip the 960x960x120 transient step")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG! solve")
+    ap.add_argument("--no-parity", action="store_true", help="skip the reference-order parity solve")
     ap.add_argument("--cpu-iters", type=int, default=200)
+    ap.add_argument("--profile", choices=("sweep", "solve", "transient"), default=None,
+                    help="run a profiling workload instead of the benchmark")
+    ap.add_argument("--solver", choices=("pbicgstab", "cg"), default="pbicgstab")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     return ap.parse_args()
 
 
@@ -65,16 +91,54 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """(model name, physical cores of the host, CPUs this process may run on)."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    if phys is not None:
+        cores.add((phys, core))
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, len(cores) or (os.cpu_count() or 1), avail
+
+
 def cpu_baseline(nx, nz, iters):
     """Oracle PBiCGSTAB+GS on the same case for a bounded number of
-    iterations, all host cores; returns GB/s in the same algorithmic unit."""
+    iterations, on the host cores this process may use, with threaded grid
+    sums (the reference's par = "thread" path); returns GB/s in the same
+    algorithmic unit."""
     from oracle import oracle as O
     O.build()
     exe = O.EXE
-    cores = min(os.cpu_count() or 1, 16)
+    model, physical, avail = host_cpu()
+    # one thread per physical core this process may use; OMP_NUM_THREADS,
+    # where the environment sets it, is the share of the host given to it
+    cores = min(physical, avail)
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        cores = min(cores, int(share))
     cmd = [exe, str(nx), str(nx), str(nz), "pbicgstab", "gs", "1e-30", "1", str(cores)]
-    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-    # cap the iteration count through a tiny solve budget: run with maxItr via env
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores",
+               ORC_SUM_MODE="2")
+    # a bounded sample: the first iters iterations of the solve
     env["ORC_MAXITR"] = str(iters)
     t0 = time.perf_counter()
     out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env, check=True).stdout
@@ -89,8 +153,10 @@ def cpu_baseline(nx, nz, iters):
     cells = nx * nx * nz
     gbs = BYTES_PER_CELL_ITER["fp64"] * cells * itr / solve_s / 1e9
     return {"value": round(gbs, 4), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": "oracle (OpenMP C restatement of the reference algorithm) PBiCGSTAB+GS FP64 at %dx%dx%d, "
-                      "first %d iterations, %.1f s solve (%.1f s wall incl. geometry)" % (nx, nx, nz, itr, solve_s, wall),
+            "cpu_model": model, "host_physical_cores": physical, "cpus_available": avail,
+            "sample": "oracle (OpenMP C restatement of the reference algorithm, threaded sweeps and grid sums) "
+                      "PBiCGSTAB+GS FP64 at %dx%dx%d, first %d iterations, %.1f s solve (%.1f s wall incl. "
+                      "geometry), %d threads pinned one per core" % (nx, nx, nz, itr, solve_s, wall, cores),
             "s_per_iteration": round(solve_s / itr, 4)}
 
 
@@ -160,12 +226,87 @@ def transient_step_c5(h2_amd, L, WorkBuffers, load_model):
     return out
 
 
+def profile_workload(args):
+    """The workloads the committed rocprofv3 summaries were taken on."""
+    import h2_amd
+    from h2_amd import _lib as L
+    from h2_amd.heat3ds import WorkBuffers, load_model
+    if args.profile in ("sweep", "solve"):
+        nx = args.nx
+        nz = 30 * max(1, nx // 240)
+        m = h2_amd.build_model(nx, nx, nz)
+        wk = WorkBuffers(*m.shape)
+        load_model(wk, m)
+        if args.profile == "sweep":
+            wk.calc_rhs(10000.0, True)
+            wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 3, L.SMOOTHER_GS, True)
+            for _ in range(3):
+                wk.copy("pcg_p", "theta")  # calibration: a plain copy of H*8 bytes
+            print("sweep ms", wk.time_kernel(0, 20, 10000.0, True), flush=True)
+        else:
+            th0 = wk.download("theta")
+            for rep_ in range(2):
+                wk.upload("theta", th0)
+                wk.fill("b", 0.0)
+                wk.calc_rhs(10000.0, True)
+                conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, args.eps, 20000, L.SMOOTHER_GS, True)
+                print("solve %d: itr %d, device ms %.3f" % (rep_, itr, wk.last_solve_ms()), flush=True)
+        wk.close()
+        return
+    solver = L.SOLVER_CG if args.solver == "cg" else L.SOLVER_PBICGSTAB
+    precision = L.FP32_REFINED if args.precision == "fp32" else L.FP64
+    m = h2_amd.build_model(960, 960, 120, power="map")
+    wk = WorkBuffers(*m.shape)
+    load_model(wk, m)
+    wk.fill("b", 0.0)
+    wk.calc_rhs(1e-3, False)
+    conv, itr, res0 = wk.solve(solver, 1e-3, 1e-8, 40, L.SMOOTHER_GS, False, precision=precision)
+    print("solver", args.solver, "precision", args.precision, "iterations", itr, "device ms", wk.last_solve_ms(),
+          flush=True)
+    for _ in range(3):
+        wk.copy("pcg_p", "theta")  # calibration: a plain copy of H*8 bytes
+    print("sweep ms", wk.time_kernel(0, 10, 1e-3, False), flush=True)
+    wk.close()
+
+
+def parity_block(np, L, wk, step, nx, nz, eps, th_exact):
+    """The headline case with the reference's sequential grid sums, and the
+    default solve's temperature difference from it."""
+    import hashlib
+    wk.set_sum_order(L.SUM_SEQUENTIAL)
+    try:
+        t, dev_ms, conv, itr, res0 = step()
+        th_ref = wk.download("theta")
+        hist = wk.history()
+    finally:
+        wk.set_sum_order(L.SUM_EXACT)
+    out = {"sum_order": "sequential (k, j, i) running sums, as the reference's SequentialEx",
+           "itr": itr, "converged": bool(conv), "ms": round(t * 1e3, 3), "device_ms": round(dev_ms, 3),
+           "res0": repr(res0),
+           "max_abs_err_K": float("%.4e" % np.max(np.abs(th_exact - th_ref)))}
+    gold = os.path.join(ROOT, "tests", "golden", "c2_pbicgstab_seq.npz")
+    pins = os.path.join(ROOT, "tests", "golden", "readme_pins.json")
+    if (nx, nz, eps) == (240, 30, 1e-4) and os.path.exists(gold) and os.path.exists(pins):
+        g = np.load(gold)
+        sha = hashlib.sha256(np.asfortranarray(th_ref).tobytes(order="F")).hexdigest()
+        out["theta_sha256_equals_oracle_sequential_run"] = sha == str(g["theta_sha256"])
+        with open(pins) as f:
+            p = json.load(f)
+        out["readme_lines"] = {k: "%.14E" % hist[int(k) - 1] for k in p["history"] if int(k) <= len(hist)}
+        out["readme_lines_match"] = all(out["readme_lines"].get(k) == v for k, v in p["history"].items())
+        out["readme_res0_match"] = repr(res0) == p["res0"]
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
               "multi-GPU (Y-slab) decomposition is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        profile_workload(args)
+        return
     import numpy as np
     import h2_amd
     from h2_amd import _lib as L
@@ -203,6 +344,13 @@ def main():
     ms_per_step = total / args.steps * 1e3
     itr = itrs[-1]
     value = BYTES_PER_CELL_ITER["fp64"] * cells * itr / (ms_per_step * 1e-3) / 1e9
+    value_fused = FUSED_BYTES_PER_CELL_ITER * cells * itr / (ms_per_step * 1e-3) / 1e9
+    th_exact = wk.download("theta")
+    parity = None
+    if not args.no_parity:
+        parity = parity_block(np, L, wk, step, nx, nz, args.eps, th_exact)
+        log("parity: sequential sums %d iterations, %.1f ms, max |dθ| %.3e K"
+            % (parity["itr"], parity["ms"], parity["max_abs_err_K"]))
 
     # dominant kernel: one red-black colour sweep (k_rb_sweep), HIP events on the context stream
     sweep_ms = wk.time_kernel(0, 200, dt, True)  # both colours
@@ -270,6 +418,7 @@ def main():
             "config": {"workload": "modelA %dx%dx%d steady PBiCGSTAB+GS(5) eps=%g, calRHS!+solve"
                                    % (nx, nx, nz, args.eps), "cells": cells, "iterations": itr,
                        "device_solve_ms": round(sum(dev_ms) / len(dev_ms), 3)},
+            "value_fused": round(value_fused, 2), "parity": parity,
             "roofline": roofline, "cpu_baseline": cpu, "fp32_refined": refined, "cg": cg, "c3": c3,
             "c5_step": c5}
     print(json.dumps(line), flush=True)

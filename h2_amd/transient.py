@@ -210,9 +210,10 @@ class TransientRun:
             out.append(wk.download("b"))
         return (out[0] - out[1]) / s
 
-    def _solve_change(self, rhs, lam):
+    def _solve_change(self, rhs, lam, precision=L.FP64):
         """δ of A(λ) δ = rhs (transient operator, zero halo), solved for a
-        power-of-two multiple of rhs of order one."""
+        power-of-two multiple of rhs of order one, in the given precision
+        (FP64, or FP32 inner solves refined to the same FP64 tolerance)."""
         wk = self.wk
         self._setup_count += 1
         scale = 2.0 ** (-np.round(np.log2(np.max(np.abs(rhs)))))
@@ -221,12 +222,13 @@ class TransientRun:
         wk.apply_bc()
         wk.fill("theta", 0.0)
         wk.upload("b", rhs * scale)
-        conv, itr, _ = wk.solve(self.solver, self.dt, self.tol, self.max_itr, self.smoother, False)
+        conv, itr, _ = wk.solve(self.solver, self.dt, self.tol, self.max_itr, self.smoother, False,
+                                precision=precision)
         if not conv:
             raise RuntimeError("perturbation solve did not converge (itr %d)" % itr)
         return wk.download("theta") / scale
 
-    def central_difference(self, k, rel_step=1e-6, thetas=None):
+    def central_difference(self, k, rel_step=1e-6, thetas=None, precision=L.FP64):
         """Central difference of J_total in the scale of group k at relative
         step rel_step.  Each perturbed run θ±ⁿ = θⁿ + δ±ⁿ is computed
         through its change:
@@ -236,7 +238,7 @@ class TransientRun:
         device and FP64 flux-form solves; the J differences come from the
         changes (delta_J_split).  The iterative solves' tolerance then
         applies to δ instead of θ.  thetas: θ⁰..θᴺ of the unperturbed run
-        (recomputed when not given)."""
+        (recomputed when not given); precision: that of the change solves."""
         if thetas is None:
             thetas = self.states()
         m, sc0 = self.m, self.scales.copy()
@@ -253,7 +255,7 @@ class TransientRun:
                 d, ds = np.zeros(thetas[0].shape, order="F"), [None]
                 for n in range(1, self.nsteps + 1):
                     rhs = self._rhs_change(d) - apply_conductance_change(m, change, thetas[n])
-                    d = self._solve_change(rhs, lam)
+                    d = self._solve_change(rhs, lam, precision)
                     ds.append(d)
                     self._log("difference group %d sign %+d step %d" % (k, int(sign), n))
                 deltas.append(ds)

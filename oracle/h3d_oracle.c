@@ -744,6 +744,19 @@ static double grid_sum(const double *x, const double *y, int MX, int MY, int MZ)
         return xsum_round(&acc);
     }
     double res = 0.0;
+    if (g_sum_mode == 2) {
+        /* per-thread running sums over k-slabs, combined by OpenMP: the
+           shape of FLoops' ThreadedEx reduction (src/common.jl:42), whose
+           order is unpinned; used to time the threaded CPU path */
+#pragma omp parallel for schedule(static) num_threads(orc_get_threads()) reduction(+ : res)
+        for (int k = 2; k <= MZ - 1; ++k)
+            for (int j = 2; j <= MY - 1; ++j)
+                for (int i = 2; i <= MX - 1; ++i) {
+                    const size_t c = I3(i, j, k);
+                    res = res + x[c] * (y ? y[c] : x[c]);
+                }
+        return res;
+    }
     for (int k = 2; k <= MZ - 1; ++k)
         for (int j = 2; j <= MY - 1; ++j)
             for (int i = 2; i <= MX - 1; ++i) {

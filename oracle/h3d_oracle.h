@@ -109,7 +109,9 @@ int orc_cg(orc_work_t *wk, const orc_grid_t *g, double tol, int maxItr,
            double *hist, int cap);                         /* NonUniform.jl:115-226 */
 
 /* Grid-sum mode: 0 = the reference's sequential (k,j,i) running sum
- * (default), 1 = exactly rounded sum of the same terms (order-independent). */
+ * (default), 1 = exactly rounded sum of the same terms (order-independent),
+ * 2 = per-thread partial sums combined by OpenMP, the threaded reference
+ * path's shape (order unpinned; timing only). */
 void orc_set_sum_mode(int mode);
 int orc_get_sum_mode(void);
 double orc_exact_sum(const double *t, size_t n);

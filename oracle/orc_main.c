@@ -4,7 +4,9 @@
  * TEST INFRASTRUCTURE ONLY (CPU baseline and fixture generation).
  *
  *   orc_q3d NX NY NZ solver smoother eps [steady=1] [threads=0] [out.bin]
- *   (environment ORC_MAXITR caps the iteration count, default 20000)
+ *   (environment ORC_MAXITR caps the iteration count, default 20000;
+ *    ORC_SUM_MODE selects the grid sums: 0 the reference's sequential order,
+ *    1 exactly rounded, 2 OpenMP partial sums as the threaded path)
  *
  * Prints the reference's log lines ("Initial residual", "%10d %24.14E",
  * "Converged at"), theta min/max over the interior and the solve time of
@@ -39,6 +41,8 @@ int main(int argc, char **argv)
     int threads = argc > 8 ? atoi(argv[8]) : 0;
     const char *out = argc > 9 ? argv[9] : NULL;
     orc_set_threads(threads);
+    const char *sum_mode = getenv("ORC_SUM_MODE");  /* 0 sequential (default), 1 exact, 2 threaded */
+    if (sum_mode) orc_set_sum_mode(atoi(sum_mode));
 
     int MX = NX + 2, MY = NY + 2, MZ = NZ + 2;
     size_t n = (size_t)MX * MY * MZ;

@@ -18,6 +18,8 @@ import os
 import numpy as np
 import pytest
 
+from tests import closed_forms as CF
+
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # Zcase2 cell widths in um, cells 1..30 (src/Zcoord.jl:88-124 with src/modelA.jl:18-30)
@@ -271,3 +273,45 @@ def test_golden_c1_reproducible(oracle):
             assert np.array_equal(hist, g["history"])
             th = np.asfortranarray(c.arr["theta"])
             assert hashlib.sha256(th.tobytes(order="F")).hexdigest() == str(g["theta_sha256"])
+
+
+# ---------------------------------------------------------------------------
+# closed forms of the heat-flux RHS, the transient term and the isothermal
+# halo copy on X/Y faces (cases in tests/closed_forms.py, which the device
+# tests share)
+# ---------------------------------------------------------------------------
+def run_closed_form_case(oracle, case):
+    """The case on the oracle: set-up as q3d() does, calRHS!, a tight
+    PBiCGSTAB+GS solve (or, for a case whose start is the solution, the
+    residual of the start); returns (θ interior, case metrics, residual)."""
+    from h2_amd.heat3ds import BoundaryConditionSet
+    bc = oracle.bc_from_set(BoundaryConditionSet(*case.faces))
+    Z, ZC, dZ = oracle.genZ(case.NZ)
+    dx, dy = oracle.pitch(case.NX), oracle.pitch(case.NY)
+    c = oracle.Case(case.NX + 2, case.NY + 2, case.NZ + 2, dx, dy, ZC, dZ, dt=case.dt, steady=case.steady, bc=bc)
+    c.arr["lam"][:] = 1.0
+    c.arr["lam"][1:-1, 1:-1, 1:-1] = case.lam
+    c.arr["rho"][:] = case.rho
+    c.arr["cp"][:] = case.cp
+    c.arr["hsrc"][1:-1, 1:-1, 1:-1] = case.Q
+    c.arr["theta"][:] = case.theta0
+    c.apply_bc()
+    c.cal_rhs()
+    if case.start_is_solution:
+        return None, (dx, dy, ZC), (c.calc_rk("r", "theta", "b"), np.linalg.norm(c.arr["b"]))
+    conv, itr, res0, hist = c.solve("pbicgstab", tol=1e-14, smoother=1)
+    assert conv
+    return c.arr["theta"][1:-1, 1:-1, 1:-1].copy(), (dx, dy, ZC), c.arr["lam"].copy()
+
+
+@pytest.mark.parametrize("name", CF.CASES)
+def test_closed_form(oracle, name):
+    """Each case of tests/closed_forms.py against its closed form: the flux
+    RHS sign and area (src/RHS.jl:83-134), the transient a_p_0 in the
+    diagonal and the RHS (src/NonUniform.jl:57-91, 454-511), the isothermal
+    halo copy of λ on X/Y faces (src/boundary_conditions.jl:156-215).
+    Planting a flipped flux sign, a doubled ρ·cp in either a_p_0, or an
+    un-copied halo λ in the oracle fails these cases."""
+    case = CF.CASES[name]()
+    th, metrics, extra = run_closed_form_case(oracle, case)
+    CF.check(case, th, metrics, extra)
