@@ -238,6 +238,9 @@ hipError_t launch_rbsor_first(int mode, T *x, T *bb, const T *r, const T *q, con
 template <typename T>
 hipError_t launch_form_rhs(int mode, T *bb, const T *r, const T *q, const Geo<T> &g, const KState *st,
                            hipStream_t s);
+// workgroups (and so partials) of the reference-layout interior kernels
+// (launch_dot, launch_calc_ax, ...) on an NX x NY x NZ interior
+unsigned interior_launch_blocks(int NX, int NY, int NZ);
 // Fdot1/Fdot2 (src/NonUniform.jl:567-592): s0 = Σ x·(y ? y : x), s1 = Σ x·z
 template <typename T>
 hipError_t launch_dot(const T *x, const T *y, const T *z, const Geo<T> &g, int fin, KState *st, double *hist,

@@ -469,6 +469,8 @@ static Shape interior_shape(int nx_threads, int NY, int NZ)
     return sh;
 }
 
+unsigned interior_launch_blocks(int NX, int NY, int NZ) { return interior_shape(NX, NY, NZ).nb(); }
+
 template <int NS>
 static hipError_t launch_finalize(const Shape &sh, int fin, KState *st, double *hist, const RedBuf &rb,
                                   hipStream_t s)

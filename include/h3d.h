@@ -286,6 +286,24 @@ int h3d_rbsor_solve(h3d_ctx *ctx, double dt, double omega, double tol, int maxIt
  * (H3D_OPT_SWEEP_CHUNK; H3D_ERR_ARG when the operator has none). */
 int h3d_time_kernel(h3d_ctx *ctx, int which, int reps, double dt, int is_steady, double *ms);
 
+/* Fdot1/Fdot2 on caller-owned device arrays without a context and without
+ * any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * x, y: device arrays of MX*MY*MZ doubles in the layout above (y NULL:
+ * Σ x², as Fdot1); *out (a device double) receives Σ x·y over the interior,
+ * exactly rounded, bitwise what h3d_fdot1/h3d_fdot2 return for the same
+ * arrays.  stream: a hipStream_t (NULL: the null stream); the call only
+ * enqueues work, so *out is ready when the stream reaches it.  The scratch
+ * memory is one fixed device array per GPU (787 KB, held until the process
+ * ends; its address is looked up on the first call on each device, so make
+ * that call outside graph capture); it serves interiors with
+ * ceil(NX/64)*ceil(NY/4) <= 16384 (H3D_ERR_UNSUPPORTED beyond).  Calls
+ * that overlap on one GPU (several streams or host threads) share it and
+ * race; calls on one stream are ordered.  Returns H3D_ERR_ARG for null
+ * pointers or extents below 3, H3D_ERR_HIP when a launch fails. */
+int h3d_fdot2_noalloc(const double *x, const double *y, int MX, int MY, int MZ, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
