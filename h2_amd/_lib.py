@@ -40,7 +40,7 @@ EXPORTS = [
     "h3d_objective", "h3d_adjoint_solve", "h3d_gradients", "h3d_get_adjoint",
     "h3d_adjoint_step", "h3d_gradients_accumulate", "h3d_gradients_accumulated",
     "h3d_store_field", "h3d_load_field", "h3d_release_slots", "h3d_mem_info",
-    "h3d_res_sor", "h3d_rbsor", "h3d_rbsor_solve", "h3d_fdot2_noalloc",
+    "h3d_res_sor", "h3d_rbsor", "h3d_rbsor_solve", "h3d_sor", "h3d_fdot2_noalloc",
     "h3d_model_pitch", "h3d_model_genz", "h3d_model_fill_id", "h3d_model_properties",
     "h3d_model_heat_src", "h3d_model_mode3_bc", "h3d_model_power_map",
 ]
@@ -97,6 +97,7 @@ _SIGS = {
     "h3d_res_sor": (_c_int, [_vp, _c_int, _c_int, _c_double, _c_double, _c_int, _dp]),
     "h3d_rbsor": (_c_int, [_vp, _c_int, _c_int, _c_double, _c_double, _c_int, _dp]),
     "h3d_rbsor_solve": (_c_int, [_vp, _c_double, _c_double, _c_double, _c_int, _c_int, _ip, _ip, _dp]),
+    "h3d_sor": (_c_int, [_vp, _c_int, _c_int, _c_double, _c_double, _c_int, _dp]),
     "h3d_set_option": (_c_int, [_vp, _c_int, _c_int]),
     "h3d_get_option": (_c_int, [_vp, _c_int, _ip]),
     "h3d_model_pitch": (_c_int, [_c_int, _dp]),

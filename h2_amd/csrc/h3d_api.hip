@@ -65,6 +65,7 @@ struct h3d_ctx {
     double *d_corr = nullptr;    // FP64 correction accumulated by the FP32 inner solve
     double *d_best = nullptr;    // its iterate with the smallest inner residual
     double *d_prev = nullptr;    // θ before the last correction (taken back if it raised the residual)
+    double *d_sor_r = nullptr;   // per-cell residual terms of the lexicographic SOR sweep
     // colour-split (RB) layout of the Krylov vectors: the solver's layout
     // whenever the class table form is active (H3D_OPT_LAYOUT)
     int layout = H3D_LAYOUT_AUTO;
@@ -1112,7 +1113,7 @@ int h3d_destroy(h3d_ctx *c)
         if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_DZC32, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
-                  c->d_corr, c->d_best, c->d_prev};
+                  c->d_corr, c->d_best, c->d_prev, c->d_sor_r};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
@@ -1586,6 +1587,24 @@ int h3d_rbsor(h3d_ctx *c, int x, int b, double omega, double dt, int is_steady, 
     r += sums[0];
     r += sums[1];
     *res = std::sqrt(r);
+    return H3D_OK;
+}
+
+int h3d_sor(h3d_ctx *c, int x, int b, double omega, double dt, int is_steady, double *res)
+{
+    if (!c) return H3D_ERR_ARG;
+    if (!field_ok(x) || !field_ok(b) || x == H3D_F_MASK || b == H3D_F_MASK || x == b || !res)
+        return fail(c, H3D_ERR_ARG, "bad field ids or output");
+    Geo<double> g;
+    int rc = sor_geo(c, dt, is_steady, omega, &g);
+    if (rc != H3D_OK) return rc;
+    if (!c->d_sor_r) HIPCHK(c, hipMalloc(&c->d_sor_r, c->H * sizeof(double)));
+    HIPCHK(c, launch_sor_lex(c->f[x], c->f[b], g, omega, c->d_sor_r, c->stream));
+    // res = Σ r² over the interior, in the sum order of the context
+    HIPCHK(c, launch_dot<double>(c->d_sor_r, nullptr, nullptr, g, FIN_STORE, c->d_unit, nullptr, c->rb, c->stream));
+    double s2 = 0.0;
+    if ((rc = unit_reduce_result(c, &s2, 1)) != H3D_OK) return rc;
+    *res = std::sqrt(s2);
     return H3D_OK;
 }
 

@@ -305,6 +305,12 @@ hipError_t launch_sor_color(double *x, const double *b, const Geo<double> &g, in
 hipError_t launch_res_sor(const double *x, const double *b, const Geo<double> &g, double omega, int fin, KState *st,
                           double *hist, const RedBuf &rb, hipStream_t s);
 
+// sor! (lexicographic point SOR, FP64, the reference's operator form): one
+// sweep in place on x as a hyperplane wavefront; rout receives each interior
+// cell's residual term r (its sum of squares is sor!'s result)
+hipError_t launch_sor_lex(double *x, const double *b, const Geo<double> &g, double omega, double *rout,
+                          hipStream_t s);
+
 // rhs = w·seed + D∘μ_next (μ_next may be nullptr): one step of the transient adjoint
 hipError_t launch_adjoint_rhs(double *rhs, const double *seed, const double *mu_next, const Geo<double> &g, double w,
                               const double hconv[6], hipStream_t s);

@@ -754,6 +754,50 @@ hipError_t launch_sor_color(double *x, const double *b, const Geo<double> &g, in
     return e != hipSuccess ? e : launch_finalize<1>(sh, fin, st, hist, rb, s);
 }
 
+// sor! (src/NonUniform.jl:762-826), the lexicographic point SOR, as a
+// hyperplane wavefront: in the loop order (k outer, then j, then i) a
+// cell's update reads the new values of its i-1, j-1 and k-1 neighbours and
+// the old values of its i+1, j+1 and k+1 neighbours, so the cells of the
+// plane i + j + k = t depend only on planes t-1 (done) and t+1 (not yet
+// touched): one launch per plane t, in increasing t, computes exactly the
+// sequential sweep.  A thread owns (j, k) and updates i = t - j - k; each
+// cell's residual term r goes to rout for the sum after the last plane.
+template <bool CLS>
+__global__ void __launch_bounds__(256) k_sor_plane(double *x, const double *b, Geo<double> g, int t, int k0,
+                                                   int k1, double omega, double *rout)
+{
+    const double *stab = stage_table<double, CLS>(g);
+    const int j = blockIdx.x * BX + threadIdx.x + 1;
+    const int k = k0 + (int)blockIdx.y * BY + threadIdx.y;
+    const int i = t - j - k;
+    if (j > g.NY || k > k1 || i < 1 || i > g.NX) return;
+    const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
+    const size_t c = (size_t)i + sy * (size_t)j + sz * (size_t)k;
+    const Cond<double> o = conductances<double, CLS>(g, stab, i, j, k, c);
+    const double pp = x[c];
+    const double dg = diag(o);
+    const double dp = (((offdiag(o, x, c, sy, sz) - b[c]) / dg - pp)) * 1.0;
+    x[c] = pp + omega * dp;
+    rout[c] = (dg + omega * (o.xm + o.ym + o.zm)) * dp / omega;
+}
+
+hipError_t launch_sor_lex(double *x, const double *b, const Geo<double> &g, double omega, double *rout,
+                          hipStream_t s)
+{
+    for (int t = 3; t <= g.NX + g.NY + g.NZ; ++t) {
+        const int k0 = max(1, t - g.NX - g.NY), k1 = min(g.NZ, t - 2);
+        if (k0 > k1) continue;
+        const dim3 grid((unsigned)((g.NY + BX - 1) / BX), (unsigned)((k1 - k0 + 1 + BY - 1) / BY)), block(BX, BY);
+        if (g.cls)
+            hipLaunchKernelGGL(k_sor_plane<true>, grid, block, table_bytes(g), s, x, b, g, t, k0, k1, omega, rout);
+        else
+            hipLaunchKernelGGL(k_sor_plane<false>, grid, block, 0, s, x, b, g, t, k0, k1, omega, rout);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // resSOR (src/NonUniform.jl:680-744): the residual of rbsor_core! over every
 // interior cell at the current x, without updating it.
 template <bool CLS>

@@ -308,6 +308,14 @@ class WorkBuffers:
                                             int(bool(is_steady)), ctypes.byref(r)))
         return r.value
 
+    def sor(self, x, b, omega, dt=1.0, is_steady=True):
+        """sor! (src/NonUniform.jl:762-826): one lexicographic SOR sweep of x
+        in place; returns sqrt(Σ r²)."""
+        res = ctypes.c_double()
+        _check(self.ctx, self.lib.h3d_sor(self.ctx, self._fid(x), self._fid(b), float(omega), float(dt),
+                                          int(bool(is_steady)), ctypes.byref(res)))
+        return res.value
+
     def solve_sor(self, omega, tol, max_itr=ITR_MAX, dt=1.0, is_steady=True):
         """solveSOR! on θ and b -> (isconverged, itr, res0)."""
         conv, itr, res0 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

@@ -276,6 +276,14 @@ int h3d_res_sor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_stea
 int h3d_rbsor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, double *res);
 int h3d_rbsor_solve(h3d_ctx *ctx, double dt, double omega, double tol, int maxItr, int is_steady,
                     int *isconverged, int *itr, double *res0);
+/* sor! (src/NonUniform.jl:762-826): one lexicographic point-SOR sweep in
+ * place on x (cells in the loop order k, j, i, each seeing the new values
+ * of the cells before it), computed as a wavefront over the planes
+ * i + j + k = t (one launch per plane, NX+NY+NZ-2 launches), so bitwise the
+ * sequential sweep; *res = sqrt(Σ r²), r = (dd + ω(cond_xm + cond_ym +
+ * cond_zm))·dp/ω, exactly rounded (or in the reference's order with
+ * H3D_SUM_SEQUENTIAL). */
+int h3d_sor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, double *res);
 
 /* Timing of one kernel family on the context stream (HIP events around
  * reps back-to-back launches after one warm-up launch; *ms = mean per

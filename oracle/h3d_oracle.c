@@ -840,6 +840,33 @@ double orc_rbsor_core(double *x, const double *b, const orc_work_t *wk, const or
     return rbsor_core(x, b, wk, g, omega, color, 1);
 }
 
+/* sor! (NonUniform.jl:762-826): lexicographic point SOR in place, cells in
+ * the loop order (k outer, then j, then i), each update seeing the new
+ * values of the cells before it; returns sqrt(Σ r²) of
+ * r = (dd + ω(cond_xm + cond_ym + cond_zm))·dp/ω, summed in that order
+ * (mode 0, res += r*r from zero(T)) or exactly rounded (mode 1). */
+double orc_sor(double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega)
+{
+    const int MX = g->MX, MY = g->MY, MZ = g->MZ;
+    double res = 0.0;
+    xsum_t acc;
+    xsum_init(&acc);
+    for (int k = 2; k <= MZ - 1; ++k)
+        for (int j = 2; j <= MY - 1; ++j)
+            for (int i = 2; i <= MX - 1; ++i) {
+                size_t c = I3(i, j, k);
+                double pp = x[c];
+                stencil_t s = stencil(x, wk->lam, wk->mask, wk->rho, wk->cp, g, i, j, k);
+                double dp = (((s.ss - b[c]) / s.dd - pp)) * wk->mask[c];
+                double pn = pp + omega * dp;
+                x[c] = pn;
+                double r = (s.dd + omega * (s.cxm + s.cym + s.czm)) * dp / omega;
+                if (g_sum_mode == 1) xsum_add(&acc, r * r);
+                else res += r * r;
+            }
+    return sqrt(g_sum_mode == 1 ? xsum_round(&acc) : res);
+}
+
 /* resSOR (NonUniform.jl:680-744): sqrt(Σ r²) over every interior cell of
  * r = (dd + ω(cond_xm + cond_ym + cond_zm))·dp/ω, dp = ((ss - b)/dd - θ)·m,
  * without updating θ. */

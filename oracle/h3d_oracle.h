@@ -85,6 +85,8 @@ double orc_rbsor_core(double *x, const double *b, const orc_work_t *wk,
                       const orc_grid_t *g, double omega, int color);
 double orc_res_sor(const double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega);
 double orc_rbsor(double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega);
+/* sor! (NonUniform.jl:762-826): one lexicographic SOR sweep in place */
+double orc_sor(double *x, const double *b, const orc_work_t *wk, const orc_grid_t *g, double omega);
 int orc_solve_sor(orc_work_t *wk, const orc_grid_t *g, double omega, double tol, int maxItr, int *itr,
                   double *res0_out, double *hist, int hist_cap);
 void orc_precond(double *xx, const double *bb, const orc_work_t *wk,

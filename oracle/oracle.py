@@ -221,6 +221,13 @@ class Case:
         f.restype = _d
         return f(_p(self.arr[x]), _p(self.arr[b]), ctypes.byref(self.work), ctypes.byref(self.grid), omega)
 
+    def sor(self, x, b, omega):
+        """sor! (src/NonUniform.jl:762-826): one lexicographic sweep in place"""
+        f = load().orc_sor
+        f.argtypes = [_vp, _vp, _vp, _vp, _d]
+        f.restype = _d
+        return f(_p(self.arr[x]), _p(self.arr[b]), ctypes.byref(self.work), ctypes.byref(self.grid), omega)
+
     def solve_sor(self, omega, tol=1e-6, maxItr=20000):
         """solveSOR! (src/NonUniform.jl:1017-1050) on θ, b: (converged, itr, res0, history)"""
         f = load().orc_solve_sor

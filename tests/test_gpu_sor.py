@@ -75,3 +75,39 @@ def test_sor_argument_errors():
     with pytest.raises(L.H3DError):
         wk.solve_sor(0.0, 1e-6, 10)
     wk.close()
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.7])
+def test_lexicographic_sor_bitwise(oracle, omega):
+    """sor! on the device (a wavefront over the planes i + j + k = t) against
+    the oracle's sequential sweep: θ bit for bit after three sweeps, and each
+    sweep's residual bit for bit, exactly rounded and in the reference's
+    sequential order, on the small random case and on the modelA stack."""
+    c = sor_case(oracle)
+    wk = device_copy(c, set_mode3_bc_parameters())
+    for seq in (False, True):
+        if seq:
+            wk.set_sum_order(L.SUM_SEQUENTIAL)
+        with oracle.sum_mode(0 if seq else 1):
+            for sweep in range(3):
+                r_dev = wk.sor("theta", "b", omega, c.grid.dt, True)
+                r_orc = c.sor("theta", "b", omega)
+                print("omega", omega, "seq", seq, "sweep", sweep, r_dev, r_orc)
+                assert r_dev == r_orc
+        assert np.array_equal(wk.download("theta"), c.arr["theta"])
+    wk.close()
+
+
+def test_lexicographic_sor_on_modelA_bitwise(oracle):
+    c = oracle.modelA_case(24, 20, 30)
+    c.arr["b"][:] = 0.0
+    c.cal_rhs()
+    m = h2_amd.build_model(24, 20, 30)
+    with WorkBuffers(*m.shape) as wk:
+        load_model(wk, m)
+        wk.fill("b", 0.0)
+        wk.calc_rhs(c.grid.dt, True)
+        with oracle.sum_mode(1):
+            for sweep in range(2):
+                assert wk.sor("theta", "b", 1.5, c.grid.dt, True) == c.sor("theta", "b", 1.5)
+        assert np.array_equal(wk.download("theta"), c.arr["theta"])

@@ -71,3 +71,78 @@ def test_over_relaxation_converges_faster(oracle, case):
         assert conv
         its.append(itr)
     assert its[1] < its[0]
+
+
+def test_sor_lexicographic_sweep_is_the_matrix_splitting(oracle, case):
+    """sor! (src/NonUniform.jl:762-826) as a matrix iteration: with A the
+    probed operator (diagonal -dd, off-diagonal conductances), L its part
+    toward neighbours earlier in the (k, j, i) loop order and U the rest, a
+    sweep solves (D - ωL) x' = ((1-ω)D + ωU) x + ω(s_bc - b), s_bc the halo
+    neighbours' contribution; its residual sum is Σ r², r = (dd + ω·c_minus)
+    ·dp/ω with c_minus the conductances toward the x-, y- and z- neighbours
+    (halo ones included).  A red-black or any other order, a dropped ω or a
+    wrong residual term fails it."""
+    import math
+    from oracle import adjoint as OA
+    from scipy.sparse import diags, tril, triu
+    from scipy.sparse.linalg import spsolve_triangular
+    c, exact, th0 = case
+    A = OA.probe_matrix(c).tocsr()
+    dd = -A.diagonal()
+    Lc = tril(A, k=-1).tocsr()
+    Uc = triu(A, k=1).tocsr()
+    shape = c.shape
+    inner = (slice(1, -1),) * 3
+
+    def interior(a):
+        return a[inner].ravel(order="F")
+
+    # halo contributions: CalcAX! of θ with its interior zeroed (ss of the halo cells)
+    hb = th0.copy(order="F")
+    hb[inner] = 0.0
+    ax = np.zeros(shape, order="F")
+    c.calc_ax(ax, hb)
+    s_bc = interior(ax)
+    # conductances toward the minus-side halo neighbours: ones on the x-, y-, z- halo planes
+    ones = np.zeros(shape, order="F")
+    ones[0, :, :] = 1.0
+    ones[:, 0, :] = 1.0
+    ones[:, :, 0] = 1.0
+    ones[inner] = 0.0
+    c.calc_ax(ax, ones)
+    c_minus = np.asarray(Lc.sum(axis=1)).ravel() + interior(ax)
+    b = interior(c.arr["b"])
+    x0 = interior(th0)
+    for omega in (1.0, 1.6):
+        D = diags(dd)
+        rhs = ((1.0 - omega) * D + omega * Uc) @ x0 + omega * (s_bc - b)
+        x1 = spsolve_triangular((D - omega * Lc).tocsr(), rhs, lower=True)
+        dp = (x1 - x0) / omega
+        r = (dd + omega * c_minus) * dp / omega
+        expect = math.sqrt(math.fsum(r * r))
+        c.arr["theta"][:] = th0
+        with oracle.sum_mode(1):
+            res = c.sor("theta", "b", omega)
+        got = interior(c.arr["theta"])
+        assert np.max(np.abs(got - x1)) <= 1e-12 * np.max(np.abs(x1))
+        assert abs(res - expect) <= 1e-9 * expect
+        # the reference's sequential sum of the same terms differs only by rounding
+        c.arr["theta"][:] = th0
+        res_seq = c.sor("theta", "b", omega)
+        assert abs(res_seq - res) <= 1e-12 * res
+
+
+def test_sor_fixed_point_and_convergence(oracle, case):
+    """The exact solution is a fixed point of sor! (its residual vanishes),
+    and repeated sweeps reach it."""
+    c, exact, th0 = case
+    c.arr["theta"][:] = exact
+    res = c.sor("theta", "b", 1.5)
+    assert np.max(np.abs(c.arr["theta"] - exact)) <= 1e-10 * np.max(np.abs(exact))
+    c.arr["theta"][:] = th0
+    res0 = c.sor("theta", "b", 1.5)
+    assert res <= 1e-9 * res0
+    for _ in range(2000):
+        if c.sor("theta", "b", 1.5) < 1e-12 * res0:
+            break
+    assert np.max(np.abs(c.arr["theta"] - exact)) < 1e-8
