@@ -43,8 +43,12 @@ Profiling workloads (no JSON line; run under rocprofv3):
              three plain copies of one H-cell FP64 field (the counters'
              calibration), then 20 full red-black sweeps (40 launches of
              k_rb_sweep<double>)
-  --profile solve [--nx 240|480]   the headline solve, twice (two graph
-             replays of the same process), for kernel traces
+  --profile solve [--nx 240|480]   the headline solve, once, for kernel
+             traces
+  --profile kernels [--nx 240|480]  the modelA case, a 3-iteration solve,
+             then 20 launches each of one full FP64 sweep, FP64 CalcAX!,
+             FP64 CalcAX! with its dot, the FP32 sweep and FP32 CalcAX!
+             (the h3d_time_kernel loops), for per-kernel counters
   --profile transient [--solver cg --precision fp32]   the 960 x 960 x 120
              power-map case, one implicit-Euler step's first 40 iterations,
              the copy calibration, then 10 full sweeps
@@ -80,7 +84,7 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG! solve")
     ap.add_argument("--no-parity", action="store_true", help="skip the reference-order parity solve")
     ap.add_argument("--cpu-iters", type=int, default=200)
-    ap.add_argument("--profile", choices=("sweep", "solve", "transient"), default=None,
+    ap.add_argument("--profile", choices=("sweep", "solve", "transient", "kernels"), default=None,
                     help="run a profiling workload instead of the benchmark")
     ap.add_argument("--solver", choices=("pbicgstab", "cg"), default="pbicgstab")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
@@ -160,6 +164,32 @@ def cpu_baseline(nx, nz, iters):
             "s_per_iteration": round(solve_s / itr, 4)}
 
 
+# kernels timed on the large grids: h3d_time_kernel id, name, algorithmic
+# bytes per interior cell (steady, transient) of one launch set (SURVEY's
+# unit: x, λ, rhs read and x written per full red-black sweep; x, λ read and
+# ax written for CalcAX!, plus one word per fused dot; a transient operator
+# adds ρ·cp), and the precision's word size
+KERNELS = ((0, "sweep_fp64 (both colours)", 32.0, 40.0, 8), (1, "calc_ax_fp64", 24.0, 32.0, 8),
+           (2, "calc_ax_dot_fp64 (with its finalize)", 32.0, 40.0, 8), (3, "sweep_fp32 (both colours)", 32.0, 40.0, 4),
+           (4, "calc_ax_fp32", 24.0, 32.0, 4), (5, "sweep2_fp64 (two-colour launch)", 32.0, 40.0, 8),
+           (6, "sweep2_fp32 (two-colour launch)", 32.0, 40.0, 4))
+
+
+def kernel_fracs(wk, cells, steady, dt, reps, two_colour):
+    """Mean launch time (HIP events on the context stream, best of three
+    sets) and fraction of the 8 TB/s HBM peak in the algorithmic unit."""
+    out = {}
+    for which, name, b_st, b_tr, word in KERNELS:
+        if which >= 5 and not two_colour:
+            continue
+        ms = min(wk.time_kernel(which, reps, dt, steady) for _ in range(3))
+        bpc = (b_st if steady else b_tr) * word / 8.0
+        achieved = bpc * cells / (ms * 1e-3) / 1e9
+        out[name] = {"us": round(ms * 1e3, 2), "alg_bytes_per_cell": bpc, "achieved_GBs": round(achieved, 1),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def precision_c3(np, h2_amd, L, WorkBuffers, load_model):
     """480x480x60, seeded power map: FP64 and FP32-refined solves to 1e-8
     against an FP64 solve to 1e-12."""
@@ -187,6 +217,7 @@ def precision_c3(np, h2_amd, L, WorkBuffers, load_model):
         out[name] = {"ms": round(t * 1e3, 2), "itr": itr, "converged": conv,
                      "max_abs_err_K": float("%.3e" % np.max(np.abs(th - ref)))}
         log("c3 %s: %.1f ms, itr %d, converged %s" % (name, t * 1e3, itr, conv))
+    out["kernels"] = kernel_fracs(wk, 480 * 480 * 60, True, 10000.0, 100, False)
     out["speedup_fp32_refined"] = round(out["fp64"]["ms"] / out["fp32_refined"]["ms"], 3)
     out["speedup_cg_fp32_refined_vs_cg_fp64"] = round(out["cg_fp64"]["ms"] / out["cg_fp32_refined"]["ms"], 3)
     out["speedup_cg_fp32_refined_vs_fp64"] = round(out["fp64"]["ms"] / out["cg_fp32_refined"]["ms"], 3)
@@ -222,6 +253,7 @@ def transient_step_c5(h2_amd, L, WorkBuffers, load_model):
     out["cg_fp32_refined"] = {"s": round(t32, 3), "inner_itr": itr32, "converged": conv32,
                               "speedup_vs_fp64_pbicgstab": round(t / t32, 3)}
     log("c5 step CG FP32-refined: %.2f s, itr %d" % (t32, itr32))
+    out["kernels"] = kernel_fracs(wk, cells, False, dt, 20, True)
     wk.close()
     return out
 
@@ -231,26 +263,28 @@ def profile_workload(args):
     import h2_amd
     from h2_amd import _lib as L
     from h2_amd.heat3ds import WorkBuffers, load_model
-    if args.profile in ("sweep", "solve"):
+    if args.profile in ("sweep", "solve", "kernels"):
         nx = args.nx
         nz = 30 * max(1, nx // 240)
         m = h2_amd.build_model(nx, nx, nz)
         wk = WorkBuffers(*m.shape)
         load_model(wk, m)
-        if args.profile == "sweep":
+        if args.profile == "kernels":
+            wk.calc_rhs(10000.0, True)
+            wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 3, L.SMOOTHER_GS, True)
+            for which in (0, 1, 2, 3, 4):
+                print("kernel %d ms %.4f" % (which, wk.time_kernel(which, 20, 10000.0, True)), flush=True)
+        elif args.profile == "sweep":
             wk.calc_rhs(10000.0, True)
             wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 3, L.SMOOTHER_GS, True)
             for _ in range(3):
                 wk.copy("pcg_p", "theta")  # calibration: a plain copy of H*8 bytes
             print("sweep ms", wk.time_kernel(0, 20, 10000.0, True), flush=True)
         else:
-            th0 = wk.download("theta")
-            for rep_ in range(2):
-                wk.upload("theta", th0)
-                wk.fill("b", 0.0)
-                wk.calc_rhs(10000.0, True)
-                conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, args.eps, 20000, L.SMOOTHER_GS, True)
-                print("solve %d: itr %d, device ms %.3f" % (rep_, itr, wk.last_solve_ms()), flush=True)
+            wk.fill("b", 0.0)
+            wk.calc_rhs(10000.0, True)
+            conv, itr, _ = wk.solve(L.SOLVER_PBICGSTAB, 10000.0, args.eps, 20000, L.SMOOTHER_GS, True)
+            print("solve: itr %d, device ms %.3f" % (itr, wk.last_solve_ms()), flush=True)
         wk.close()
         return
     solver = L.SOLVER_CG if args.solver == "cg" else L.SOLVER_PBICGSTAB
