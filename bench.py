@@ -217,6 +217,22 @@ def precision_c3(np, h2_amd, L, WorkBuffers, load_model):
         out[name] = {"ms": round(t * 1e3, 2), "itr": itr, "converged": conv,
                      "max_abs_err_K": float("%.3e" % np.max(np.abs(th - ref)))}
         log("c3 %s: %.1f ms, itr %d, converged %s" % (name, t * 1e3, itr, conv))
+    # FP32 storage of a vector like the correction: ||A(fl32(d) - d)|| / ||A d||
+    # for d = θ* - θ0, the relative error a matvec of a rounded preconditioned
+    # direction carries before any arithmetic
+    d = ref - th0
+    wk.upload("pcg_p", d)
+    wk.calc_ax("pcg_q", "pcg_p")
+    wk.upload("pcg_p_", d.astype(np.float32).astype(np.float64))
+    wk.calc_ax("pcg_s", "pcg_p_")
+    wk.triad("pcg_s_", "pcg_q", "pcg_s", -1.0)
+    noise = float(np.sqrt(wk.fdot1("pcg_s_") / wk.fdot1("pcg_q")))
+    f32 = out["fp32_refined"]
+    out["configured_form"] = {
+        "form": "FP32 PBiCGSTAB+GS(5) inner solves, inner tol 1e-4, FP64 refinement to 1e-8",
+        "meets_config": bool(f32["converged"] and f32["max_abs_err_K"] <= 1e-5),
+        "fp32_matvec_noise": float("%.4e" % noise),
+        "substitute": "cg_fp32_refined"}
     out["kernels"] = kernel_fracs(wk, 480 * 480 * 60, True, 10000.0, 100, False)
     out["speedup_fp32_refined"] = round(out["fp64"]["ms"] / out["fp32_refined"]["ms"], 3)
     out["speedup_cg_fp32_refined_vs_cg_fp64"] = round(out["cg_fp64"]["ms"] / out["cg_fp32_refined"]["ms"], 3)

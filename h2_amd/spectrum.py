@@ -11,6 +11,9 @@ the Ritz values of the k-step Krylov space), for -A and for its Jacobi
 scaling D^-1/2 (-A) D^-1/2.
 
     python -m h2_amd.spectrum --nx 240 --nz 30 --iters 3000
+    python -m h2_amd.spectrum --device --nx 480 --nz 60 --iters 20000
+
+(the second on a GPU: preconditioned CG with the library's own kernels)
 
 prints, every few hundred iterations, the smallest and largest Ritz values
 and their ratio (a lower bound of κ that rises to κ as the smallest Ritz
@@ -125,6 +128,87 @@ def cg_lanczos(A, iters, seed=1, every=250, log=print):
     return out
 
 
+def _ritz(alphas, betas):
+    """Extreme eigenvalues of the Lanczos tridiagonal matrix that k CG steps
+    with coefficients α_1..α_k, β_1..β_k build: diagonal 1/α_i +
+    β_{i-1}/α_{i-1}, off-diagonal sqrt(β_i)/α_i."""
+    from scipy.linalg import eigvalsh_tridiagonal
+    al, be = np.asarray(alphas, dtype=np.float64), np.asarray(betas, dtype=np.float64)
+    # CG on the negative definite A itself has α < 0: the Ritz values of -A
+    al = np.abs(al)
+    diag = 1.0 / al
+    diag[1:] += be[:-1] / al[:-1]
+    off = np.sqrt(be[:-1]) / al[:-1]
+    n = len(diag)
+    lo = eigvalsh_tridiagonal(diag, off, select="i", select_range=(0, 0))[0]
+    hi = eigvalsh_tridiagonal(diag, off, select="i", select_range=(n - 1, n - 1))[0]
+    return lo, hi
+
+
+def device_pcg_lanczos(wk, iters, smoother, seed=1, every=250, log=print, dt=1.0, is_steady=True, rtol=1e-13):
+    """Ritz values of M^-1 A from preconditioned CG run on the device with
+    the library's own CalcAX!, Preconditioner! (GS(5) or none) and exactly
+    rounded dots, on a random right-hand side (interior cells, zero halo).
+
+    wk is a WorkBuffers with the model loaded; its fields pcg_r, pcg_p,
+    pcg_p_ and pcg_q are overwritten.  A (negative definite) and M^-1 have
+    the same sign, so α, β > 0 and the Ritz values approximate the spectrum
+    of M^-1 A (without a preconditioner, of -A).  Five red-black sweeps
+    from zero are not a symmetric preconditioner, so for GS(5) the values
+    are estimates of the spectrum the reference's CG! iterates with, not
+    exact Lanczos bounds, and the run stops if (r, M^-1 r) changes sign.
+    Yields (k, smallest, largest, relative residual) every `every` steps
+    and at the last step."""
+    rng = np.random.default_rng(seed)
+    b = np.zeros(wk.shape, order="F")
+    b[1:-1, 1:-1, 1:-1] = rng.standard_normal(tuple(m - 2 for m in wk.shape))
+    wk.upload("pcg_r", b)
+    r0 = np.sqrt(wk.fdot1("pcg_r"))
+    wk.fill("pcg_p", 0.0)
+    wk.fill("pcg_q", 0.0)
+    alphas, betas, out = [], [], []
+    rz_old = None
+    for k in range(1, iters + 1):
+        if smoother is None:
+            wk.copy("pcg_p_", "pcg_r")
+        else:
+            # Preconditioner! starts from its output's contents: myfill! first,
+            # as CG! does (src/NonUniform.jl:157-159, 203-205)
+            wk.fill("pcg_p_", 0.0)
+            wk.precond("pcg_p_", "pcg_r", smoother, dt, is_steady)
+        rz = wk.fdot2("pcg_r", "pcg_p_")
+        if rz_old is not None:
+            betas.append(rz / rz_old)
+            wk.bicg1("pcg_p", "pcg_p_", "pcg_q", betas[-1], 0.0)   # p = z + β p
+        else:
+            wk.copy("pcg_p", "pcg_p_")
+        wk.calc_ax("pcg_q", "pcg_p", dt, is_steady)
+        pq = wk.fdot2("pcg_p", "pcg_q")
+        a = rz / pq
+        if not (np.isfinite(a) and a != 0.0 and (not betas or betas[-1] >= 0.0)):
+            log("  k %6d  stopped: (r, z) = %.6e, (p, Ap) = %.6e, beta %s"
+                % (k, rz, pq, "%.6e" % betas[-1] if betas else "-"))
+            if betas:
+                betas.pop()
+            break
+        alphas.append(a)
+        wk.triad("pcg_r", "pcg_q", "pcg_r", -a)                    # Triad!: r = -α q + r
+        rz_old = rz
+        rel = np.sqrt(wk.fdot1("pcg_r")) / r0
+        last = k == iters or rel < rtol
+        if k % every == 0 or last:
+            lo, hi = _ritz(alphas, (betas + [0.0])[:len(alphas)])
+            out.append((k, lo, hi, rel))
+            log("  k %6d  ritz min %.6e  max %.6e  ratio %.4e  rel res %.3e" % (k, lo, hi, hi / lo, rel))
+        if last:
+            break
+    if alphas and (not out or out[-1][0] != len(alphas)):
+        lo, hi = _ritz(alphas, (betas + [0.0])[:len(alphas)])
+        out.append((len(alphas), lo, hi, rel))
+        log("  k %6d  ritz min %.6e  max %.6e  ratio %.4e  rel res %.3e" % (len(alphas), lo, hi, hi / lo, rel))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--nx", type=int, default=240)
@@ -132,7 +216,23 @@ def main(argv=None):
     ap.add_argument("--power", default="map")
     ap.add_argument("--iters", type=int, default=3000)
     ap.add_argument("--every", type=int, default=250)
+    ap.add_argument("--device", action="store_true",
+                    help="CG on the device through the library's operators: M^-1 A for GS(5) and A itself")
     a = ap.parse_args(argv)
+    if a.device:
+        from . import _lib as L
+        from .heat3ds import WorkBuffers, build_model, load_model
+        m = build_model(a.nx, a.nx, a.nz, power=a.power)
+        with WorkBuffers(*m.shape) as wk:
+            load_model(wk, m)
+            for name, sm in (("M^-1 A, M = GS(5)", L.SMOOTHER_GS), ("A (no preconditioner)", None)):
+                print("%dx%dx%d %s:" % (a.nx, a.nx, a.nz, name), flush=True)
+                t0 = time.time()
+                res = device_pcg_lanczos(wk, a.iters, sm, every=a.every, log=lambda s: print(s, flush=True))
+                k, lo, hi, rel = res[-1]
+                print("  after %d steps (%.1f s): kappa >= %.4e, kappa * 2^-24 >= %.3e"
+                      % (k, time.time() - t0, hi / lo, hi / lo * 2.0 ** -24), flush=True)
+        return 0
     from .heat3ds import build_model
     t0 = time.time()
     m = build_model(a.nx, a.nx, a.nz, power=a.power)
