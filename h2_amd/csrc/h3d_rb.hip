@@ -922,6 +922,13 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T 
             dyv[r][a] = (NS >= 1 && dy) ? ldb(dy, co[r][a] * sz) : T(0);
             dzv[r][a] = NS >= 2 ? ldb(dz, co[r][a] * sz) : T(0);
         }
+        // the pair's cells are each other's x-neighbours (cell 2m+1 sits at
+        // the other half's offset of cell 2m and the reverse): reuse their
+        // values and classes instead of loading them again
+        n[r][0].xp = xv[r][1];
+        n[r][1].xm = xv[r][0];
+        q[r][0].xp = q[r][1].c;
+        q[r][1].xm = q[r][0].c;
     }
     const T *stab = rb_stage(g, t.k);
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
