@@ -224,3 +224,36 @@ def test_mask_contract():
         wk.upload("mask", bad)
     assert e.value.code == L.ERR_UNSUPPORTED
     wk.close()
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_fill_and_copy_whole_arrays(oracle, form):
+    """myfill!/mycopy! (src/common.jl:95-113) cover every cell, halo
+    included; a copy into λ takes effect in the next operator (its class
+    tables are rebuilt), as CalcAX! on the oracle with the same λ shows."""
+    print("case fill/copy form %d" % form, flush=True)
+    wk, c, rng = make_pair(oracle, *SIZES[0], seed=11, bc_set=mixed_bc(), random_lam="classes", form=form)
+    a = rng.normal(0, 1, wk.shape)
+    wk.upload("pcg_s", a)
+    wk.copy("pcg_t_", "pcg_s")
+    assert_bitwise(wk.download("pcg_t_"), a, "copy")
+    wk.fill("pcg_t_", 2.5)
+    assert np.all(wk.download("pcg_t_") == 2.5)
+    # new materials through a copy into λ
+    mat = rng.integers(0, 5, size=wk.shape)
+    lam2 = rng.uniform(1.0, 200.0, 5)[mat]
+    wk.upload("pcg_s", lam2)
+    wk.copy("lam", "pcg_s")
+    c.arr["lam"][:] = lam2
+    wk.apply_bc()
+    c.apply_bc()
+    x = rng.normal(0, 1, wk.shape)
+    put(wk, c, "pcg_p_", x)
+    wk.calc_ax("pcg_q", "pcg_p_", c.grid.dt, True)
+    c.calc_ax("q", "p_")
+    assert_bitwise(wk.download("pcg_q"), c.arr["q"], "ax after copying λ")
+    with pytest.raises(L.H3DError):
+        wk.copy("mask", "pcg_s")
+    with pytest.raises(L.H3DError):
+        wk.fill("mask", 1.0)
+    wk.close()
