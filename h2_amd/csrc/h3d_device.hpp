@@ -16,9 +16,12 @@ constexpr double kFloatMin = 1.0e-37;  // src/common.jl:12
 // grid sums: double-double through a fixed reduction tree
 // ---------------------------------------------------------------------------
 // An unevaluated hi + lo pair with error-free TwoSum additions.  Each thread
-// adds at most a few dozen terms, the tree above adds pairs; the pair keeps
-// the sum to far below one ulp of the result, so the final rounding gives
-// the exactly rounded sum of the terms whatever the decomposition.
+// adds at most a few dozen terms, the tree above adds pairs (one TwoSum of
+// the high parts, the low parts added in FP64); the pair keeps the sum
+// within about 2^-100 of the sum of the terms' magnitudes, so the final
+// rounding gives the exactly rounded sum of the terms whatever the
+// decomposition, unless the exact sum lies that close to a rounding
+// boundary.
 struct dd {
     double hi, lo;
 };
@@ -64,12 +67,24 @@ __device__ __forceinline__ double row_shr(double v)
 
 // Inclusive scan within each 16-lane row: lane 15 of a row ends with the
 // row's sum.  All lanes of the wave must be active.
+// Pair addition of the reduction tree: 11 FP64 operations against the 20
+// of dd_add(dd, dd), whose second TwoSum only matters when the high parts
+// cancel to far below the low parts' scale.  Error at most about 2u²
+// (|x| + |y|), u = 2^-53: over the tree's levels it stays within the
+// bound above (measured: whole solves bitwise unchanged, 1.9 % faster at
+// 240x240x30).
+__device__ __forceinline__ dd dd_add_tree(dd x, dd y)
+{
+    const dd s = two_sum(x.hi, y.hi);
+    return quick_two_sum(s.hi, s.lo + (x.lo + y.lo));
+}
+
 __device__ __forceinline__ dd row_sum(dd v)
 {
-    v = dd_add(v, dd{row_shr<1>(v.hi), row_shr<1>(v.lo)});
-    v = dd_add(v, dd{row_shr<2>(v.hi), row_shr<2>(v.lo)});
-    v = dd_add(v, dd{row_shr<4>(v.hi), row_shr<4>(v.lo)});
-    v = dd_add(v, dd{row_shr<8>(v.hi), row_shr<8>(v.lo)});
+    v = dd_add_tree(v, dd{row_shr<1>(v.hi), row_shr<1>(v.lo)});
+    v = dd_add_tree(v, dd{row_shr<2>(v.hi), row_shr<2>(v.lo)});
+    v = dd_add_tree(v, dd{row_shr<4>(v.hi), row_shr<4>(v.lo)});
+    v = dd_add_tree(v, dd{row_shr<8>(v.hi), row_shr<8>(v.lo)});
     return v;
 }
 
