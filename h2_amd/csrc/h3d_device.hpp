@@ -67,6 +67,13 @@ __device__ __forceinline__ double row_shr(double v)
 
 // Inclusive scan within each 16-lane row: lane 15 of a row ends with the
 // row's sum.  All lanes of the wave must be active.
+// A thread's two terms as one exact pair (an invalid term counts as 0):
+// one TwoSum instead of two dd_add steps from zero.
+__device__ __forceinline__ dd pair_sum(bool v0, double t0, bool v1, double t1)
+{
+    return two_sum(v0 ? t0 : 0.0, v1 ? t1 : 0.0);
+}
+
 // Pair addition of the reduction tree: 11 FP64 operations against the 20
 // of dd_add(dd, dd), whose second TwoSum only matters when the high parts
 // cancel to far below the low parts' scale.  Error at most about 2u²

@@ -373,7 +373,10 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
                                                   const uint8_t *__restrict__ clt, RbGeo<T> g, int h, const KState *st,
                                                   const T *__restrict__ bt, RedBuf rbuf)
 {
-    if (halted(st)) return;
+    // small grids (the rows per thread they use): the loads go out before
+    // the status test, whose latency they then hide
+    constexpr bool early = R == (sizeof(T) == 8 ? 1 : 2);
+    if (!early && halted(st)) return;
     const RbRows t = rb_rows<R>(g);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY;
     Nb<T> n[R];
@@ -393,6 +396,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
         bp[a] = DOT ? ldb(bt, rb) : T(0);
         q[a] = rb_cls(clo, clt, rr[a], rr[a], s[a], HX, HXY);
     }
+    if (early && halted(st)) return;
     const T *stab = rb_stage(g, t.k);
     T xv[R];
     bool valid[R];
@@ -894,7 +898,10 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T 
                                                     const T *__restrict__ dy, const T *__restrict__ dz, RbGeo<T> g,
                                                     KState *st, RedBuf rb)
 {
-    if (halted(st)) return;
+    // small grids (fewer than 2^21 cells per half): the loads go out
+    // before the status test, whose latency they then hide
+    const bool early = g.HS < ((size_t)1 << 21);
+    if (!early && halted(st)) return;
     const RbRows t = rb_rows<R>(g);
     const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY, HS = (unsigned)g.HS;
     const unsigned sz = (unsigned)sizeof(T);
@@ -930,6 +937,7 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T 
         q[r][0].xp = q[r][1].c;
         q[r][1].xm = q[r][0].c;
     }
+    if (early && halted(st)) return;
     const T *stab = rb_stage(g, t.k);
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
@@ -939,11 +947,19 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T 
             const Cond<T> cd = rb_cond(g, stab, q[r][a], 2 * t.m + a, jj[r], t.k);
             av[r][a] = rb_apply(g, cd, n[r][a], xv[r][a]);
             if (!(NS >= 1 && dy)) dyv[r][a] = av[r][a];
-            if (valid[r][a]) {
+            if (R > 1 && valid[r][a]) {
                 if (NS >= 1) v[0] = dd_add(v[0], (double)av[r][a] * (double)dyv[r][a]);
                 if (NS >= 2) v[1] = dd_add(v[1], (double)av[r][a] * (double)dzv[r][a]);
             }
         }
+    if (R == 1) {
+        if (NS >= 1)
+            v[0] = pair_sum(valid[0][0], (double)av[0][0] * (double)dyv[0][0], valid[0][1],
+                            (double)av[0][1] * (double)dyv[0][1]);
+        if (NS >= 2)
+            v[1] = pair_sum(valid[0][0], (double)av[0][0] * (double)dzv[0][0], valid[0][1],
+                            (double)av[0][1] * (double)dzv[0][1]);
+    }
     // the workgroup sum first, the stores last: a store before the
     // reduction's barrier would hold the wave until it completes
     if (NS > 0) block_partials<NS>(v, rb);
@@ -1118,13 +1134,9 @@ __global__ void __launch_bounds__(256) k_rb_bicg_update(double *__restrict__ th,
         r0v[a] = NS >= 2 ? ldb(r0, ob) : T(0);
     }
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        if (c[a].valid) {
-            v[0] = dd_add(v[0], (double)rv[a] * (double)rv[a]);
-            if (NS >= 2) v[1] = dd_add(v[1], (double)rv[a] * (double)r0v[a]);
-        }
-    }
+    v[0] = pair_sum(c[0].valid, (double)rv[0] * (double)rv[0], c[1].valid, (double)rv[1] * (double)rv[1]);
+    if (NS >= 2)
+        v[1] = pair_sum(c[0].valid, (double)rv[0] * (double)r0v[0], c[1].valid, (double)rv[1] * (double)r0v[1]);
     block_partials<NS>(v, rb);
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -1184,9 +1196,7 @@ __global__ void __launch_bounds__(256) k_rb_cg_update(double *__restrict__ th, d
         rv[a] = r[o] - alpha * q[o];
     }
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-        if (c[a].valid) v[0] = dd_add(v[0], (double)rv[a] * (double)rv[a]);
+    v[0] = pair_sum(c[0].valid, (double)rv[0] * (double)rv[0], c[1].valid, (double)rv[1] * (double)rv[1]);
     block_partials<1>(v, rb);
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -1254,9 +1264,7 @@ __global__ void __launch_bounds__(256) k_rb_cg_first(double *__restrict__ th, do
     const Cond<T> cd = rb_cond(g, stab, cq, cr.i, tt.j, tt.k);
     const T zn = rb_gs_from_zero(g, cd, rv[a1]);
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-        if (c[a].valid) v[0] = dd_add(v[0], (double)rv[a] * (double)rv[a]);
+    v[0] = pair_sum(c[0].valid, (double)rv[0] * (double)rv[0], c[1].valid, (double)rv[1] * (double)rv[1]);
     block_partials<1>(v, rb);
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
