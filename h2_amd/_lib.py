@@ -23,6 +23,7 @@ LAYOUT_AUTO, LAYOUT_STANDARD = 0, 1
 OPT_OPERATOR_FORM, FORM_REFERENCE, FORM_FLUX = 3, 0, 1
 OPT_SWEEP_CHUNK = 4
 OPT_SUM_ORDER, SUM_EXACT, SUM_SEQUENTIAL = 5, 0, 1
+OPT_RESIDENT, OPT_RESIDENT_WGS = 6, 7
 COEF_AUTO, COEF_DIRECT = 0, 1
 
 FIELDS = ["theta", "b", "mask", "rho", "lam", "cp", "pcg_p", "pcg_p_", "pcg_r", "pcg_r0",

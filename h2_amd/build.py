@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libh3d.so")
-SOURCES = ["h3d_kernels.hip", "h3d_rb.hip", "h3d_adjoint.hip", "h3d_api.hip", "h3d_noalloc.hip", "h3d_model.cpp"]
+SOURCES = ["h3d_kernels.hip", "h3d_rb.hip", "h3d_adjoint.hip", "h3d_api.hip", "h3d_noalloc.hip", "h3d_resident.hip", "h3d_model.cpp"]
 ARCH = "gfx950"
 
 

@@ -71,6 +71,11 @@ struct h3d_ctx {
     int layout = H3D_LAYOUT_AUTO;
     int op_form = H3D_FORM_REFERENCE;     // H3D_OPT_OPERATOR_FORM of the FP64 operator
     int sweep_chunk = -1;                 // H3D_OPT_SWEEP_CHUNK
+    int resident = 1;                     // H3D_OPT_RESIDENT
+    int res_wgs = 0;                      // workgroups of the last solve's resident preconditioner (0: none)
+    int ncu = 0;                          // compute units of the device
+    unsigned *d_resbar = nullptr;         // grid barrier words of the resident preconditioner
+    bool res_prepared[2] = {false, false}; // its kernels' LDS limit raised (FP64, FP32)
     double *rbswap64 = nullptr;           // second buffer of the fused (out-of-place) sweeps
     float *rbswap32 = nullptr;
     unsigned long grid_version = 1;       // bumped by h3d_set_grid / h3d_set_bc
@@ -343,6 +348,9 @@ struct Path {
     // launch per colour) and the second buffer they alternate with
     int chunk = 0;
     T *swap = nullptr;
+    // colour-split path: tile plan of the resident preconditioner (G = 0:
+    // the per-colour launches run)
+    RbRes res{};
 };
 
 template <typename T>
@@ -391,13 +399,6 @@ hipError_t p_sweep(const Path<T> &P, T *x, const T *b, int color, const KState *
 // on the colour-split path (launch_rb_sweep_dot), by a separate dot kernel
 // otherwise.  The sum is exactly rounded either way, so the two are
 // bitwise the same.
-struct DotFin {
-    int fin;
-    KState *st;
-    double *hist;
-    const RedBuf *rb;
-};
-
 template <typename T>
 hipError_t p_sweeps_after_first(const Path<T> &P, T *x0, T *xx, T *bb, const KState *st, hipStream_t s,
                                 const DotFin *dot = nullptr)
@@ -448,6 +449,7 @@ hipError_t p_precond_from_zero(const Path<T> &P, int mode, T *xx, T *bb, const T
         if (e != hipSuccess || !dot) return e;
         return p_dot<T>(P, bb, xx, nullptr, dot->fin, dot->st, dot->hist, *dot->rb, s);
     }
+    if (P.rb && P.res.G > 0) return launch_rb_resident<T>(mode, xx, bb, r, q, P.rg, P.res, st, dot, s);
     e = P.rb ? launch_rb_first<T>(mode, xx, bb, r, q, P.rg, st, s)
              : launch_rbsor_first<T>(mode, xx, bb, r, q, P.g, st, s);
     if (e != hipSuccess) return e;
@@ -491,7 +493,7 @@ hipError_t enqueue_cg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, int 
     if ((e = p_calc_ax<T>(P, v.q, v.p, v.p, nullptr, FIN_CG_ALPHA, st, nullptr, c->rb, s)) != hipSuccess) return e;
     // z = M⁻¹ r; β from ρ = (r, z)
     const DotFin dot{FIN_CG_BETA, st, nullptr, &c->rb};
-    if (P.rb && smoother == H3D_SMOOTHER_GS) {
+    if (P.rb && smoother == H3D_SMOOTHER_GS && P.res.G == 0) {
         // with two-colour sweeps, sweep 1 goes into the swap buffer (see
         // p_sweeps_after_first)
         T *z0 = P.chunk > 0 ? P.swap : v.s;
@@ -602,6 +604,18 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
     }
     HIPCHK(c, hipStreamSynchronize(s));
     *fin_state = c->h_state[1 + ((nbatch - 1) & 1)];
+    c->res_wgs = smoother == H3D_SMOOTHER_GS ? P.res.G : 0;
+    if (P.res.G > 0 && smoother == H3D_SMOOTHER_GS) {
+        // a timed-out wait of the resident preconditioner leaves its error
+        // word set (and the iterates wrong): report it, reset the words
+        unsigned err = 0;
+        HIPCHK(c, hipMemcpy(&err, c->d_resbar + kResBarErrWord, sizeof(err), hipMemcpyDeviceToHost));
+        if (err != 0) {
+            HIPCHK(c, hipMemset(c->d_resbar, 0, kResBarWords * sizeof(unsigned)));
+            return fail(c, H3D_ERR_HIP, "resident preconditioner: a wait for a neighbouring tile timed out "
+                                        "(workgroups not co-resident); set H3D_OPT_RESIDENT to 0");
+        }
+    }
     return H3D_OK;
 }
 
@@ -725,6 +739,22 @@ int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
         if (rc != H3D_OK) return rc;
         P->n = 2 * c->HS;
         P->chunk = sweep_chunk<T>(c, P->rg);
+        // the resident preconditioner when it fits, unless a sweep chunk was set
+        if (c->resident && c->sweep_chunk < 0) {
+            RbRes pl = rb_res_plan<T>(P->rg, c->ncu);
+            bool &prep = c->res_prepared[sizeof(T) == sizeof(double) ? 0 : 1];
+            if (pl.G > 0 && !prep) {
+                HIPCHK(c, rb_res_prepare<T>());
+                prep = true;
+            }
+            if (pl.G > 0) {
+                // the tiles' pass counters start equal (zero) for this plan
+                HIPCHK(c, hipMemsetAsync(c->d_resbar, 0, kResBarWords * sizeof(unsigned), c->stream));
+                pl.bar = c->d_resbar;
+                P->res = pl;
+                P->chunk = 0;  // the resident launch replaces the per-colour and two-colour sweeps
+            }
+        }
         if (P->chunk > 0) {
             T **sw = swap_slot<T>(c);
             if (!*sw) {
@@ -1076,6 +1106,11 @@ int h3d_create(h3d_ctx **out, int MX, int MY, int MZ, int device)
     if ((e = hipMemsetAsync(c->d_unit, 0, sizeof(KState), c->stream)) != hipSuccess) return bail("hipMemset", e);
     if ((e = hipHostMalloc(&c->h_state, 3 * sizeof(KState), hipHostMallocDefault)) != hipSuccess)
         return bail("hipHostMalloc", e);
+    if ((e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+        return bail("hipDeviceGetAttribute", e);
+    if ((e = hipMalloc(&c->d_resbar, kResBarWords * sizeof(unsigned))) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = hipMemsetAsync(c->d_resbar, 0, kResBarWords * sizeof(unsigned), c->stream)) != hipSuccess)
+        return bail("hipMemset", e);
     // reduction scratch: enough partials for any interior launch shape
     const long maxb = ((long)(c->NX + 63) / 64) * ((c->NY + 3) / 4) * (long)c->NZ + 64;
     if ((e = alloc_partials(&c->rb, maxb)) != hipSuccess) return bail("hipMalloc partials", e);
@@ -1113,7 +1148,7 @@ int h3d_destroy(h3d_ctx *c)
         if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_DZC32, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
-                  c->d_corr, c->d_best, c->d_prev, c->d_sor_r};
+                  c->d_corr, c->d_best, c->d_prev, c->d_sor_r, c->d_resbar};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
@@ -1154,6 +1189,12 @@ int h3d_set_option(h3d_ctx *c, int option, int value)
         c->grid_version++;  // captured iterations carry the sweep sequence
         return H3D_OK;
     }
+    if (option == H3D_OPT_RESIDENT) {
+        if (value != 0 && value != 1) return fail(c, H3D_ERR_ARG, "bad resident setting");
+        c->resident = value;
+        c->grid_version++;  // captured iterations carry the preconditioner's launches
+        return H3D_OK;
+    }
     if (option == H3D_OPT_SUM_ORDER) {
         if (value != H3D_SUM_EXACT && value != H3D_SUM_SEQUENTIAL) return fail(c, H3D_ERR_ARG, "bad sum order");
         c->rb.seq = value == H3D_SUM_SEQUENTIAL ? 1 : 0;  // in the graph key
@@ -1187,6 +1228,14 @@ int h3d_get_option(const h3d_ctx *c, int option, int *value)
     }
     if (option == H3D_OPT_SUM_ORDER) {
         *value = c->rb.seq ? H3D_SUM_SEQUENTIAL : H3D_SUM_EXACT;
+        return H3D_OK;
+    }
+    if (option == H3D_OPT_RESIDENT) {
+        *value = c->resident;
+        return H3D_OK;
+    }
+    if (option == H3D_OPT_RESIDENT_WGS) {
+        *value = c->res_wgs;
         return H3D_OK;
     }
     return H3D_ERR_ARG;
@@ -1481,7 +1530,7 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
 {
     if (!c) return H3D_ERR_ARG;
     if (!ms || reps < 1) return fail(c, H3D_ERR_ARG, "bad arguments");
-    if (which < 0 || which > 6) return fail(c, H3D_ERR_ARG, "unknown kernel id");
+    if (which < 0 || which > 8) return fail(c, H3D_ERR_ARG, "unknown kernel id");
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
@@ -1493,8 +1542,9 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
     double *x = w[1], *b = w[0], *q = w[4], *r0 = w[3];
     Path<float> P32;
     float *w32[8] = {};
-    const bool fp32 = which == 3 || which == 4 || which == 6;
+    const bool fp32 = which == 3 || which == 4 || which == 6 || which == 8;
     if (which == 5 && P.chunk < 1) return fail(c, H3D_ERR_ARG, "no two-colour sweep for this operator");
+    if (which == 7 && P.res.G < 1) return fail(c, H3D_ERR_ARG, "no resident preconditioner for this grid");
     if (fp32) {
         Geo<float> g32{};
         if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;
@@ -1503,11 +1553,19 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
         HIPCHK(c, launch_convert<float, double>(w32[1], x, P.n, c->stream));
         HIPCHK(c, launch_convert<float, double>(w32[0], b, P.n, c->stream));
         if (which == 6 && P32.chunk < 1) return fail(c, H3D_ERR_ARG, "no two-colour sweep for this operator");
+        if (which == 8 && P32.res.G < 1) return fail(c, H3D_ERR_ARG, "no resident preconditioner for this grid");
     }
     auto once = [&]() -> hipError_t {
         // one full sweep as one two-colour launch, x -> swap (the result is
         // not copied back: every launch computes the same sweep)
         if (which == 5) return launch_rb_sweep2<double>(x, P.swap, b, P.rg, 1, P.chunk, c->d_unit, c->stream);
+        // the whole preconditioner (five sweeps from zero) as one resident launch
+        if (which == 7)
+            return launch_rb_resident<double>(FIRST_COPY, x, b, nullptr, nullptr, P.rg, P.res, c->d_unit, nullptr,
+                                              c->stream);
+        if (which == 8)
+            return launch_rb_resident<float>(FIRST_COPY, w32[1], w32[0], nullptr, nullptr, P32.rg, P32.res,
+                                             c->d_unit, nullptr, c->stream);
         if (which == 6)
             return launch_rb_sweep2<float>(w32[1], P32.swap, w32[0], P32.rg, 1, P32.chunk, c->d_unit, c->stream);
         if (which == 3) {

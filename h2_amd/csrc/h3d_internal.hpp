@@ -162,6 +162,31 @@ struct RbGeo {
 // fills the launch grid fields of g
 template <typename T> void rb_shape(RbGeo<T> &g);
 
+// Tile plan of the resident Gauss-Seidel preconditioner (h3d_resident.hip):
+// the interior rows (j, k) cut into tiles of TJ x TK rows, ntj x ntk of
+// them, one workgroup per tile, all co-resident (G <= the CU count).
+// G = 0: the grid does not fit and the per-colour launches run.
+struct RbRes {
+    int TJ = 0, TK = 0, ntj = 0, ntk = 0;
+    int G = 0;
+    size_t lds = 0;             // dynamic LDS bytes per workgroup
+    unsigned *bar = nullptr;    // device synchronisation words (kResBarWords)
+};
+constexpr size_t kResLdsMax = 150 * 1024;
+constexpr int kResMaxWG = 1024;
+// synchronisation words: an error word, then a pass counter per workgroup
+// (64 bytes apart); the host zeroes them whenever a solve selects a plan
+constexpr int kResBarWords = 64 + 16 * kResMaxWG;
+constexpr int kResBarErrWord = 0;
+
+// A grid sum formed inside a launch and its finalize step
+struct DotFin {
+    int fin;
+    KState *st;
+    double *hist;
+    const RedBuf *rb;
+};
+
 // ---- launchers (h3d_kernels.hip); all return hipError_t of the launch -----
 // Sum nb workgroup partials of ns sums each and run finalizer fin.
 hipError_t launch_finalize_nb(int ns, unsigned nb, int fin, KState *st, double *hist, const RedBuf &rb,
@@ -376,5 +401,17 @@ hipError_t launch_rb_cg_update(double *th, double *xbest, T *r, const T *p, cons
                                KState *st, double *hist, const RedBuf &rb, hipStream_t s);
 template <typename T>
 hipError_t launch_rb_cg_p(T *p, const T *z, const RbGeo<T> &g, const KState *st, hipStream_t s);
+
+// ---- resident preconditioner (h3d_resident.hip) ------------------------------
+// Tile plan for ncu CUs (G = 0 when the grid does not fit).
+template <typename T> RbRes rb_res_plan(const RbGeo<T> &g, int ncu);
+// Raise the kernels' dynamic LDS limit to kResLdsMax (once, outside capture).
+template <typename T> hipError_t rb_res_prepare();
+// Preconditioner! with :gs (five red-black sweeps) from x = 0 on the
+// right-hand side bb formed per mode (as launch_rb_first), in one launch;
+// with dot also Σ bb·x over the interior, finalized with dot->fin.
+template <typename T>
+hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
+                              const KState *st, const DotFin *dot, hipStream_t s);
 
 }  // namespace h3d

@@ -1,0 +1,520 @@
+// Preconditioner! (src/NonUniform.jl:531-559) with the :gs smoother on
+// small grids as ONE launch: the five red-black sweeps (ten colour passes,
+// rbsor_core! at :848-915 with ω = 1) run inside a grid of co-resident
+// workgroups that keep the iterate in LDS from the first pass to the last.
+//
+// Why: on a grid whose vectors sit in the Infinity Cache (240x240x30 and
+// below) a colour pass launched on its own is latency bound.  Every launch
+// pays the dispatch of two rounds of workgroups, one dependent round trip
+// for x, b and the class bytes, the table staging, and the drain; ten of
+// them per preconditioner are most of a BiCGSTAB iteration.  Here the
+// right-hand side, the diagonal and the conductance table indices of every
+// cell are formed once per launch and held in registers, x lives in LDS,
+// and between passes only the rows a neighbouring tile needs cross HBM.
+//
+// Decomposition: the interior rows (j, k) are cut into tiles of TJ x TK
+// rows (RbRes, planned on the host), one 1024-thread workgroup per tile,
+// all resident at once (at most one tile per CU).  LDS holds both colour
+// halves of the tile's rows plus a one-row ring in j and k (the rows the
+// stencil reads from the neighbouring tiles) and the staged conductance
+// tables of the tile's planes.  Thread t owns the slots t, t+1024, ... of
+// the tile's (row, m) index space in both halves.
+//
+// A pass updates the cells of one half from LDS into LDS, and stores the
+// rows on the tile's edge into x in HBM (every row in the last sweep).  The
+// workgroup then publishes its pass counter and waits for the counters of
+// the (at most four) tiles whose edge rows it reads, and loads those rows
+// of the half just updated.  No grid-wide barrier: each tile waits only
+// for its neighbours.  Cross-workgroup data moves with device-scope (sc1)
+// loads and stores, which the L2s of the eight XCDs keep coherent without
+// the write-back and invalidate a device-scope release/acquire fence
+// costs; each wave waits for its stores before the counter is published.
+// Every wait is bounded: a timed-out wait sets an error word the host
+// checks after the solve, and later waits then return at once.
+//
+// Every cell performs the per-colour kernels' operations in the same
+// order (rb_offdiag / rb_flux / diag, the from-zero start as explicit
+// zeros), so x is bitwise the result of the ten separate launches.
+
+#include "h3d_rb.hpp"
+
+namespace h3d {
+
+constexpr int kResThreads = 1024;
+constexpr int kResSlots = 4;                    // slots per thread and half
+constexpr unsigned kResSpin = 1u << 16;         // polls before a wait gives up
+
+// Words of the synchronisation buffer (RbRes::bar, kResBarWords, zeroed
+// by the host whenever a solve selects a tile plan): the error word, then
+// one pass counter per workgroup, 64 bytes apart.
+constexpr int kBarErr = kResBarErrWord, kBarFlags = 64;
+constexpr int kBarFlagStride = 16;
+
+__device__ __forceinline__ unsigned bar_load(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void bar_store(unsigned *p, unsigned v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void sc_store(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T sc_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The tile of workgroup bid = a + ntj*b: first row (j0, k0), extents
+// tj x tk, and its up to four neighbours in (j, k) (-1: none).
+struct ResTile {
+    int j0, k0, tj, tk;
+    int nb[4];
+};
+
+__device__ __forceinline__ ResTile res_tile(const RbRes &pl, int NY, int NZ)
+{
+    ResTile t;
+    const int a = (int)blockIdx.x % pl.ntj, b = (int)blockIdx.x / pl.ntj;
+    t.j0 = 1 + a * pl.TJ;
+    t.k0 = 1 + b * pl.TK;
+    t.tj = min(pl.TJ, NY - t.j0 + 1);
+    t.tk = min(pl.TK, NZ - t.k0 + 1);
+    t.nb[0] = a > 0 ? (int)blockIdx.x - 1 : -1;
+    t.nb[1] = a + 1 < pl.ntj ? (int)blockIdx.x + 1 : -1;
+    t.nb[2] = b > 0 ? (int)blockIdx.x - pl.ntj : -1;
+    t.nb[3] = b + 1 < pl.ntk ? (int)blockIdx.x + pl.ntj : -1;
+    return t;
+}
+
+// Publish that this workgroup has finished pass count `want` (its stores of
+// the rows its neighbours read are complete, visible to their device-scope
+// loads), then wait until the neighbouring tiles have too.  The counters
+// grow by one per exchanged pass across launches; all start equal.
+__device__ __forceinline__ void res_neighbour_sync(unsigned *bar, const ResTile &tl, unsigned want)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        if (lane == 0) bar_store(bar + kBarFlags + kBarFlagStride * blockIdx.x, want);
+        const int nb = lane < 4 ? tl.nb[lane] : -1;
+        const unsigned *f = bar + kBarFlags + kBarFlagStride * (nb < 0 ? 0 : nb);
+        unsigned n = 0;
+        while (true) {
+            const bool ok = nb < 0 || (int)(bar_load(f) - want) >= 0;
+            if (__all(ok)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if ((++n & 255u) == 0u && (n >= kResSpin || bar_load(bar + kBarErr) != 0u)) {
+                bar_store(bar + kBarErr, 1u);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Row r of a tile's tj x tk rows in the order the passes visit them: the
+// rows on the tile's edge (the ones neighbouring tiles read) first, so that
+// their stores are in flight while the inner rows are computed.
+__device__ __forceinline__ void res_row(int r, int tj, int tk, int &jj, int &kk)
+{
+    const int ij = max(tj - 2, 0), ik = max(tk - 2, 0);
+    const int nedge = tj * tk - ij * ik;
+    if (r >= nedge) {
+        const int q = r - nedge;
+        jj = 2 + q % ij;
+        kk = 2 + q / ij;
+        return;
+    }
+    if (r < tj) {  // first plane of the tile
+        jj = r + 1;
+        kk = 1;
+        return;
+    }
+    r -= tj;
+    if (tk > 1 && r < tj) {  // last plane
+        jj = r + 1;
+        kk = tk;
+        return;
+    }
+    if (tk > 1) r -= tj;
+    // planes in between: first and last row of each
+    if (tj > 1) {
+        kk = 2 + r / 2;
+        jj = (r & 1) ? tj : 1;
+    } else {
+        kk = 2 + r;
+        jj = 1;
+    }
+}
+
+// One slot of a thread: cell (m, j, k) of the tile in both halves.  e = its
+// element offset in an LDS half, base = its offset in an HBM half; edge =
+// its row is on the tile's edge (read by a neighbouring tile).
+struct ResSlot {
+    unsigned e, base;
+    bool own, edge;
+};
+
+template <typename T, bool DOT, bool FLUX>
+__global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__restrict__ x, T *__restrict__ bb,
+                                                             const T *__restrict__ r, const T *__restrict__ q,
+                                                             RbGeo<T> g, RbRes pl, const KState *st, RedBuf rbuf)
+{
+    if (halted(st)) return;  // uniform: every workgroup reads the same state
+    extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
+    const int tid = threadIdx.x;
+    const int HX = g.HX, K = g.K;
+    const unsigned HXY = (unsigned)HX * (unsigned)g.MY;
+    const ResTile tl = res_tile(pl, g.NY, g.NZ);
+    const int RJ = pl.TJ + 2;
+    const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);  // one LDS half (+1 pad)
+    T *xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
+    T *xs1 = xs0 + hsz;
+    T *tabs = xs1 + hsz;
+    const int ntab = 6 * K * K + K;
+    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem + (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) &
+                                                      ~(size_t)15));
+
+    // the tables of the tile's planes, and the class bytes of the tile's
+    // rows and ring (both halves, the domain's halo rows included: their
+    // classes select the boundary conductances) in the x region, laid out
+    // as x (byte offset = element offset); x is zeroed once they are used
+    for (int kk = 1; kk <= tl.tk; ++kk)
+        rb_stage_into(g, tl.k0 + kk - 1, tabs + (size_t)(kk - 1) * ntab, tid, kResThreads);
+    uint8_t *cs0 = reinterpret_cast<uint8_t *>(h3d_dyn_smem);
+    uint8_t *cs1 = cs0 + hsz;
+    const size_t HS = g.HS;
+    {
+        const unsigned rows = (unsigned)((tl.tk + 2) * (tl.tj + 2));
+        for (unsigned e = tid; e < 2 * rows * (unsigned)HX; e += kResThreads) {
+            const unsigned hh = e / (rows * (unsigned)HX), rem = e - hh * rows * (unsigned)HX;
+            const unsigned row = rem / (unsigned)HX, m = rem - row * (unsigned)HX;
+            const unsigned kk = row / (unsigned)(tl.tj + 2), jj = row - kk * (unsigned)(tl.tj + 2);
+            const unsigned j = (unsigned)tl.j0 - 1u + jj, k = (unsigned)tl.k0 - 1u + kk;
+            (hh ? cs1 : cs0)[(kk * (unsigned)RJ + jj) * (unsigned)HX + m] =
+                g.cls[(hh ? HS : 0) + m + (unsigned)HX * j + HXY * k];
+        }
+    }
+
+    // slots and the right-hand side (formed as k_rb_first forms it)
+    T beta = T(0), omg = T(0), r_alpha = T(0);
+    bool first_itr = false;
+    if (mode == FIRST_BICG_P) {
+        first_itr = (st->itr == 1);
+        beta = (T)st->beta;
+        omg = (T)st->omega;
+    } else if (mode == FIRST_BICG_S) {
+        r_alpha = (T)(-st->alpha);
+    }
+    const unsigned nown = (unsigned)(tl.tj * tl.tk * HX);
+    ResSlot sl[kResSlots];
+    T bv[kResSlots][2];
+    unsigned par[kResSlots];  // (j + k) & 1 of the slot
+#pragma unroll
+    for (int u = 0; u < kResSlots; ++u) {
+        const unsigned qi = (unsigned)tid + (unsigned)kResThreads * u;
+        sl[u].own = qi < nown;
+        const unsigned qq = sl[u].own ? qi : 0u;
+        const int row = (int)(qq / (unsigned)HX);
+        const int m = (int)(qq - (unsigned)row * (unsigned)HX);
+        int jj, kk;
+        res_row(row, tl.tj, tl.tk, jj, kk);
+        const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
+        sl[u].edge = row < tl.tj * tl.tk - max(tl.tj - 2, 0) * max(tl.tk - 2, 0);
+        par[u] = (unsigned)((j + k) & 1);
+        sl[u].e = (unsigned)((kk * RJ + jj) * HX + m);
+        sl[u].base = (unsigned)m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k;
+        const unsigned o = sl[u].base;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const size_t oh = (h ? HS : 0) + o;
+            T v;
+            if (mode == FIRST_BICG_P) v = first_itr ? r[oh] : r[oh] + beta * (bb[oh] - omg * q[oh]);
+            else if (mode == FIRST_BICG_S) v = r_alpha * q[oh] + r[oh];
+            else v = bb[oh];
+            bv[u][h] = v;
+        }
+    }
+    __syncthreads();  // tables and class tile staged
+
+    // pass counter at the start of this launch (equal in every workgroup)
+    const unsigned f0 = bar_load(pl.bar + kBarFlags + kBarFlagStride * blockIdx.x);
+
+    // per slot and half: validity, diagonal, sink (flux form) and the LDS
+    // indices of the six conductances, packed two to a word
+    const unsigned zs = (unsigned)(RJ * HX);
+    bool valid[kResSlots][2];
+    T dgl[kResSlots][2], snk[kResSlots][2] = {};
+    unsigned ix[kResSlots][2][3];
+#pragma unroll
+    for (int u = 0; u < kResSlots; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const unsigned e = sl[u].e;
+            const unsigned row = e / (unsigned)HX;
+            const int m = (int)(e - row * (unsigned)HX);
+            const int kk = (int)(row / (unsigned)RJ), jj = (int)(row - (unsigned)kk * (unsigned)RJ);
+            const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
+            const unsigned sx = (unsigned)h ^ par[u];
+            const int i = 2 * m + (int)sx;
+            valid[u][h] = sl[u].own && i >= 1 && i <= g.NX;
+            const uint8_t *co = h ? cs1 : cs0, *ct = h ? cs0 : cs1;
+            Cls c;
+            c.c = co[e];
+            c.xm = ct[e - 1u + sx];
+            c.xp = ct[e + sx];
+            c.ym = ct[e - (unsigned)HX];
+            c.yp = ct[e + (unsigned)HX];
+            c.zm = ct[e - zs];
+            c.zp = ct[e + zs];
+            const T *stab = tabs + (size_t)(kk - 1) * ntab;
+            const Cond<T> o = rb_cond(g, stab, c, i, j, k);
+            dgl[u][h] = diag(o);
+            if (FLUX) snk[u][h] = o.cxp + o.cxm + o.cyp + o.cym + o.czp + o.czm + o.ap0;
+            const int b = c.c, n2 = 2 * K * K;
+            const unsigned tb = (unsigned)((kk - 1) * ntab);
+            const unsigned ixm = tb + ((i > 1 ? K : 0) + c.xm) * K + b;
+            const unsigned ixp = tb + ((i < g.NX ? K : 0) + c.xp) * K + b;
+            const unsigned iym = tb + n2 + ((j > 1 ? K : 0) + c.ym) * K + b;
+            const unsigned iyp = tb + n2 + ((j < g.NY ? K : 0) + c.yp) * K + b;
+            const unsigned izm = tb + 2 * n2 + c.zm * K + b;
+            const unsigned izp = tb + 2 * n2 + K * K + c.zp * K + b;
+            ix[u][h][0] = ixm | (ixp << 16);
+            ix[u][h][1] = iym | (iyp << 16);
+            ix[u][h][2] = izm | (izp << 16);
+            // formed here, one slot at a time: left to the compiler, the
+            // diagonals sink towards their use in the pass loop and the
+            // conductances of all eight stay live until then (spilled)
+            asm volatile("" : "+v"(dgl[u][h]), "+v"(snk[u][h]), "+v"(ix[u][h][0]), "+v"(ix[u][h][1]),
+                         "+v"(ix[u][h][2]));
+        }
+    __syncthreads();  // class tile read
+    // x = 0 everywhere in LDS (the preconditioner starts from zero; the
+    // domain's halo rows and columns stay zero)
+    for (unsigned e = tid; e < 2 * hsz; e += kResThreads) xs0[e] = T(0);
+    // the bb stores of modes that form it (interior cells only)
+    if (mode != FIRST_COPY) {
+#pragma unroll
+        for (int u = 0; u < kResSlots; ++u)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (valid[u][h]) bb[(h ? HS : 0) + sl[u].base] = bv[u][h];
+    }
+    __syncthreads();  // x zeroed
+
+#pragma unroll 1
+    for (int sweep = 0; sweep < 5; ++sweep) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            // one colour pass: colour 0 updates half 1, colour 1 half 0
+            const int h = 1 - half;
+            T *xo = h ? xs1 : xs0;
+            const T *xt = h ? xs0 : xs1;
+            T *xg = x + (h ? HS : 0);
+#pragma unroll
+            for (int u = 0; u < kResSlots; ++u) {
+                if (!sl[u].own) continue;
+                // the offsets and index words are opaque to the compiler
+                // here, so that it forms the addresses inside the loop
+                // instead of hoisting some hundred of them into registers
+                unsigned e = sl[u].e, base = sl[u].base, w0 = ix[u][h][0], w1 = ix[u][h][1],
+                         w2 = ix[u][h][2];
+                asm volatile("" : "+v"(e), "+v"(base), "+v"(w0), "+v"(w1), "+v"(w2));
+                const unsigned sx = (unsigned)h ^ par[u];
+                Nb<T> n;
+                n.xm = xt[e - 1u + sx];
+                n.xp = xt[e + sx];
+                n.ym = xt[e - (unsigned)HX];
+                n.yp = xt[e + (unsigned)HX];
+                n.zm = xt[e - zs];
+                n.zp = xt[e + zs];
+                const T pp = xo[e];
+                Cond<T> o;
+                o.xm = tabs[w0 & 0xffffu];
+                o.xp = tabs[w0 >> 16];
+                o.ym = tabs[w1 & 0xffffu];
+                o.yp = tabs[w1 >> 16];
+                o.zm = tabs[w2 & 0xffffu];
+                o.zp = tabs[w2 >> 16];
+                T xn;
+                if (FLUX) {
+                    const T f = o.xp * (n.xp - pp) + o.xm * (n.xm - pp) + o.yp * (n.yp - pp) + o.ym * (n.ym - pp) +
+                                o.zp * (n.zp - pp) + o.zm * (n.zm - pp);
+                    xn = pp + ((f - snk[u][h] * pp) - bv[u][h]) / dgl[u][h];
+                } else {
+                    const T dp = (((rb_offdiag(o, n) - bv[u][h]) / dgl[u][h] - pp)) * T(1);
+                    xn = pp + T(1) * dp;
+                }
+                if (valid[u][h]) {
+                    xo[e] = xn;
+                    // HBM receives the rows neighbours read, and in the
+                    // last sweep (final values) every row
+                    if (sl[u].edge || sweep == 4) sc_store(xg + base, xn);
+                }
+                // one slot's loads in flight at a time: sixteen waves per CU
+                // hide the LDS latency, and the registers stay within 128
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (sweep == 4 && half == 1) break;
+            // the neighbours' edge rows of half h, once they have stored
+            // them (rows outside the interior are the domain halo: zero,
+            // never loaded)
+            res_neighbour_sync(pl.bar, tl, f0 + (unsigned)(2 * sweep + half + 1));
+            const unsigned nh = (unsigned)(2 * tl.tj + 2 * tl.tk);
+            for (unsigned e = tid; e < nh * (unsigned)HX; e += kResThreads) {
+                const unsigned hr = e / (unsigned)HX, m = e - hr * (unsigned)HX;
+                int jj, kk;
+                if (hr < (unsigned)tl.tj) {
+                    jj = (int)hr + 1;
+                    kk = 0;
+                } else if (hr < 2u * tl.tj) {
+                    jj = (int)hr - tl.tj + 1;
+                    kk = tl.tk + 1;
+                } else if (hr < 2u * tl.tj + tl.tk) {
+                    jj = 0;
+                    kk = (int)hr - 2 * tl.tj + 1;
+                } else {
+                    jj = tl.tj + 1;
+                    kk = (int)hr - 2 * tl.tj - tl.tk + 1;
+                }
+                const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
+                if (j >= 1 && j <= g.NY && k >= 1 && k <= g.NZ)
+                    xo[(kk * RJ + jj) * HX + (int)m] =
+                        sc_load(xg + m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (DOT) {
+        // Σ bb·x over the tile's interior cells of both halves (x final:
+        // half 0 from the last pass, half 1 from the one before; each
+        // thread reads back its own cells)
+        dd v = {0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < kResSlots; ++u) {
+            if (valid[u][0]) v = dd_add(v, (double)bv[u][0] * (double)xs0[sl[u].e]);
+            if (valid[u][1]) v = dd_add(v, (double)bv[u][1] * (double)xs1[sl[u].e]);
+        }
+        const int lane = tid & 63, wave = tid >> 6;
+        const dd w = row_sum(v);
+        if ((lane & 15) == 15) red[wave * 4 + (lane >> 4)] = w;
+        __syncthreads();
+        if (wave == 0) {
+            const dd z = row_sum(red[lane]);
+            if ((lane & 15) == 15) red[64 + (lane >> 4)] = z;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const dd t = dd_add_tree(dd_add_tree(red[64], red[65]), dd_add_tree(red[66], red[67]));
+            rbuf.partials[6 * (size_t)blockIdx.x] = t.hi;
+            rbuf.partials[6 * (size_t)blockIdx.x + 1] = t.lo;
+        }
+    }
+}
+
+template <typename T>
+size_t rb_res_lds(int TJ, int TK, int HX, int K)
+{
+    const size_t hsz = (size_t)(TJ + 2) * (TK + 2) * HX + 1;
+    const size_t ntab = (size_t)6 * K * K + K;
+    size_t n = (2 * hsz + (size_t)TK * ntab) * sizeof(T);
+    n = (n + 15) & ~(size_t)15;
+    return n + 68 * sizeof(dd);
+}
+
+// Tile plan: the tiling with the fewest slot rounds per thread (compute per
+// pass), then the fewest ring elements (HBM reads per pass), then the
+// fewest workgroups (barrier arrivals), within ncu workgroups, the LDS
+// budget and 16-bit table indices.  G = 0 when none fits.
+template <typename T>
+RbRes rb_res_plan(const RbGeo<T> &g, int ncu)
+{
+    RbRes best{};
+    if (g.K < 1 || ncu < 1) return best;
+    const long ntab = 6L * g.K * g.K + g.K;
+    long bc = -1, bh = 0, bg = 0;
+    for (int TK = 1; TK <= g.NZ; ++TK)
+        for (int TJ = 1; TJ <= g.NY; ++TJ) {
+            const long rows = (long)TJ * TK;
+            if (rows * g.HX > (long)kResThreads * kResSlots) break;
+            const int ntj = (g.NY + TJ - 1) / TJ, ntk = (g.NZ + TK - 1) / TK;
+            const long G = (long)ntj * ntk;
+            if (G > ncu || G > kResMaxWG || TK * ntab >= 65536) continue;
+            const size_t lds = rb_res_lds<T>(TJ, TK, g.HX, g.K);
+            if (lds > kResLdsMax) continue;
+            const long rounds = (rows * g.HX + kResThreads - 1) / kResThreads;
+            const long halo = (2L * TJ + 2L * TK) * g.HX;
+            if (bc < 0 || rounds < bc || (rounds == bc && (halo < bh || (halo == bh && G < bg)))) {
+                bc = rounds;
+                bh = halo;
+                bg = G;
+                best.TJ = TJ;
+                best.TK = TK;
+                best.ntj = ntj;
+                best.ntk = ntk;
+                best.G = (int)G;
+                best.lds = lds;
+            }
+        }
+    return best;
+}
+
+template <typename T, bool DOT, bool FLUX>
+static hipError_t res_attr()
+{
+    return hipFuncSetAttribute((const void *)k_rb_resident<T, DOT, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kResLdsMax);
+}
+
+// Allow the resident kernels the LDS budget (above the default dynamic
+// limit); called once per context, outside any stream capture.
+template <typename T>
+hipError_t rb_res_prepare()
+{
+    hipError_t e;
+    if ((e = res_attr<T, false, false>()) != hipSuccess) return e;
+    if ((e = res_attr<T, false, true>()) != hipSuccess) return e;
+    if ((e = res_attr<T, true, false>()) != hipSuccess) return e;
+    return res_attr<T, true, true>();
+}
+
+template <typename T>
+hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
+                              const KState *st, const DotFin *dot, hipStream_t s)
+{
+    if (pl.G <= 0 || !pl.bar || pl.lds > kResLdsMax) return hipErrorInvalidValue;
+    if (dot && pl.G > dot->rb->nblocks_max) return hipErrorInvalidValue;
+    const dim3 grid(pl.G), block(kResThreads);
+    const RedBuf none{};
+    const RedBuf &rb = dot ? *dot->rb : none;
+    if (dot) {
+        if (g.flux) hipLaunchKernelGGL((k_rb_resident<T, true, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
+        else hipLaunchKernelGGL((k_rb_resident<T, true, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
+    } else {
+        if (g.flux) hipLaunchKernelGGL((k_rb_resident<T, false, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
+        else hipLaunchKernelGGL((k_rb_resident<T, false, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !dot) return e;
+    return launch_finalize_sums<T>(1, (unsigned)pl.G, dot->fin, dot->st, dot->hist, *dot->rb, s,
+                                   seq_terms<T>(g, bb, x));
+}
+
+#define RES_INST(T)                                                                                              \
+    template RbRes rb_res_plan<T>(const RbGeo<T> &, int);                                                       \
+    template hipError_t rb_res_prepare<T>();                                                                    \
+    template hipError_t launch_rb_resident<T>(int, T *, T *, const T *, const T *, const RbGeo<T> &,          \
+                                              const RbRes &, const KState *, const DotFin *, hipStream_t);
+
+RES_INST(double)
+RES_INST(float)
+#undef RES_INST
+
+}  // namespace h3d

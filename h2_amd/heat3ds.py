@@ -275,6 +275,19 @@ class WorkBuffers:
         planes, -1 = chosen from the grid (default).  Bitwise identical."""
         _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_SWEEP_CHUNK, int(chunk)))
 
+    def set_resident(self, on):
+        """1 (default): on grids whose iterate fits the CUs' LDS the Gauss-
+        Seidel preconditioner runs as one launch of co-resident workgroups
+        (when the sweep chunk is left at -1); 0: one launch per colour pass.
+        Bitwise identical."""
+        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_RESIDENT, int(on)))
+
+    def resident_workgroups(self):
+        """Workgroups of the last solve's resident preconditioner (0: none)."""
+        v = ctypes.c_int()
+        _check(self.ctx, self.lib.h3d_get_option(self.ctx, L.OPT_RESIDENT_WGS, ctypes.byref(v)))
+        return v.value
+
     def set_sum_order(self, order):
         """L.SUM_EXACT (exactly rounded grid sums, default) or L.SUM_SEQUENTIAL
         (the reference's sequential (k, j, i) running sums: a solve then

@@ -97,6 +97,13 @@ extern "C" {
                                   of 60 planes when NZ >= 120, else one launch per colour); h3d_cg
                                   runs its last sweep per colour either way (that pass also forms
                                   rho = (r, z)); bitwise identical */
+#define H3D_OPT_RESIDENT 6     /* Gauss-Seidel preconditioner of the colour-split layout on grids whose
+                                  iterate fits the CUs' LDS (240x240x30 in FP64 and below): 1 = all five
+                                  sweeps in one launch of co-resident workgroups that keep the iterate
+                                  in LDS (default, used when H3D_OPT_SWEEP_CHUNK is -1 and the grid
+                                  fits), 0 = one launch per colour pass; bitwise identical */
+#define H3D_OPT_RESIDENT_WGS 7 /* read-only: workgroups of the last solve's resident preconditioner
+                                  launches (0 when it did not use one) */
 #define H3D_OPT_SUM_ORDER 5    /* order of the Krylov solvers' grid sums (dots, residual norms) */
 #define H3D_SUM_EXACT 0        /* exactly rounded, independent of order (default; parallel, fast) */
 #define H3D_SUM_SEQUENTIAL 1   /* the reference's sequential order: one running FP64 sum over the
@@ -291,7 +298,9 @@ int h3d_sor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, 
  * 1 = CalcAX!, 2 = CalcAX! with its fused dot; 3 and 4 = the FP32 operator's
  * sweep and CalcAX! (the kernels of the H3D_FP32_REFINED inner solve);
  * 5 and 6 = one full sweep as one two-colour launch, FP64 and FP32
- * (H3D_OPT_SWEEP_CHUNK; H3D_ERR_ARG when the operator has none). */
+ * (H3D_OPT_SWEEP_CHUNK; H3D_ERR_ARG when the operator has none); 7 and 8 =
+ * the whole preconditioner (five sweeps from zero) as one resident launch,
+ * FP64 and FP32 (H3D_OPT_RESIDENT; H3D_ERR_ARG when the grid does not fit). */
 int h3d_time_kernel(h3d_ctx *ctx, int which, int reps, double dt, int is_steady, double *ms);
 
 /* Fdot1/Fdot2 on caller-owned device arrays without a context and without
