@@ -74,7 +74,7 @@ struct h3d_ctx {
     int resident = 1;                     // H3D_OPT_RESIDENT
     int res_wgs = 0;                      // workgroups of the last solve's resident preconditioner (0: none)
     int ncu = 0;                          // compute units of the device
-    unsigned *d_resbar = nullptr;         // grid barrier words of the resident preconditioner
+    unsigned *d_resbar[2] = {};           // resident preconditioner: synchronisation words (FP64, FP32)
     bool res_prepared[2] = {false, false}; // its kernels' LDS limit raised (FP64, FP32)
     double *rbswap64 = nullptr;           // second buffer of the fused (out-of-place) sweeps
     float *rbswap32 = nullptr;
@@ -465,6 +465,20 @@ hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, in
     hipStream_t s = c->stream;
     KState *st = c->d_state;
     hipError_t e;
+    if (P.rb && P.res.G > 0 && smoother == H3D_SMOOTHER_GS) {
+        // the resident preconditioner with CalcAX! and its sums fused:
+        // p formed, p_ = M⁻¹p, q = A p_, alpha
+        const ResAx<T> aq{v.q, v.r0, nullptr, FIN_BICG_ALPHA, st, nullptr, &c->rb};
+        if ((e = launch_rb_resident<T>(FIRST_BICG_P, v.p_, v.p, v.r, v.q, P.rg, P.res, st, nullptr, s, &aq)) !=
+            hipSuccess)
+            return e;
+        // s formed, s_ = M⁻¹s, t = A s_, omega from (t, t) and (t, s)
+        const ResAx<T> at{v.t, nullptr, v.s, FIN_BICG_OMEGA, st, nullptr, &c->rb};
+        if ((e = launch_rb_resident<T>(FIRST_BICG_S, v.s_, v.s, v.r, v.q, P.rg, P.res, st, nullptr, s, &at)) !=
+            hipSuccess)
+            return e;
+        return launch_rb_bicg_update<T>(v.x, v.xbest, v.r, v.p_, v.s_, v.t, v.s, v.r0, P.rg, st, c->d_hist, c->rb, s);
+    }
     // p = r or BiCG1!; myfill!(p_, 0); Preconditioner!(p_, p)
     if ((e = p_precond_from_zero<T>(P, FIRST_BICG_P, v.p_, v.p, v.r, v.q, smoother, st, s)) != hipSuccess) return e;
     // q = A p_; alpha = rho / (q, r0)
@@ -609,9 +623,9 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
         // a timed-out wait of the resident preconditioner leaves its error
         // word set (and the iterates wrong): report it, reset the words
         unsigned err = 0;
-        HIPCHK(c, hipMemcpy(&err, c->d_resbar + kResBarErrWord, sizeof(err), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(&err, P.res.bar + kResBarErrWord, sizeof(err), hipMemcpyDeviceToHost));
         if (err != 0) {
-            HIPCHK(c, hipMemset(c->d_resbar, 0, kResBarWords * sizeof(unsigned)));
+            HIPCHK(c, hipMemset(P.res.bar, 0, kResBarWords * sizeof(unsigned)));
             return fail(c, H3D_ERR_HIP, "resident preconditioner: a wait for a neighbouring tile timed out "
                                         "(workgroups not co-resident); set H3D_OPT_RESIDENT to 0");
         }
@@ -747,10 +761,19 @@ int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
                 HIPCHK(c, rb_res_prepare<T>());
                 prep = true;
             }
+            if (pl.G > 0 && pl.G > c->rb.nblocks_max) {
+                // one partial per tile for the sums fused into the launch
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                (void)hipFree(c->rb.partials);
+                c->rb.partials = c->rb.partials2 = nullptr;
+                if (alloc_partials(&c->rb, pl.G) != hipSuccess) return fail(c, H3D_ERR_HIP, "hipMalloc partials");
+            }
             if (pl.G > 0) {
                 // the tiles' pass counters start equal (zero) for this plan
-                HIPCHK(c, hipMemsetAsync(c->d_resbar, 0, kResBarWords * sizeof(unsigned), c->stream));
-                pl.bar = c->d_resbar;
+                const int pi = sizeof(T) == sizeof(double) ? 0 : 1;
+                if (!c->d_resbar[pi]) HIPCHK(c, hipMalloc(&c->d_resbar[pi], kResBarWords * sizeof(unsigned)));
+                HIPCHK(c, hipMemsetAsync(c->d_resbar[pi], 0, kResBarWords * sizeof(unsigned), c->stream));
+                pl.bar = c->d_resbar[pi];
                 P->res = pl;
                 P->chunk = 0;  // the resident launch replaces the per-colour and two-colour sweeps
             }
@@ -1108,9 +1131,7 @@ int h3d_create(h3d_ctx **out, int MX, int MY, int MZ, int device)
         return bail("hipHostMalloc", e);
     if ((e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return bail("hipDeviceGetAttribute", e);
-    if ((e = hipMalloc(&c->d_resbar, kResBarWords * sizeof(unsigned))) != hipSuccess) return bail("hipMalloc", e);
-    if ((e = hipMemsetAsync(c->d_resbar, 0, kResBarWords * sizeof(unsigned), c->stream)) != hipSuccess)
-        return bail("hipMemset", e);
+
     // reduction scratch: enough partials for any interior launch shape
     const long maxb = ((long)(c->NX + 63) / 64) * ((c->NY + 3) / 4) * (long)c->NZ + 64;
     if ((e = alloc_partials(&c->rb, maxb)) != hipSuccess) return bail("hipMalloc partials", e);
@@ -1148,7 +1169,7 @@ int h3d_destroy(h3d_ctx *c)
         if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_DZC32, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
-                  c->d_corr, c->d_best, c->d_prev, c->d_sor_r, c->d_resbar};
+                  c->d_corr, c->d_best, c->d_prev, c->d_sor_r, c->d_resbar[0], c->d_resbar[1]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);

@@ -407,11 +407,23 @@ hipError_t launch_rb_cg_p(T *p, const T *z, const RbGeo<T> &g, const KState *st,
 template <typename T> RbRes rb_res_plan(const RbGeo<T> &g, int ncu);
 // Raise the kernels' dynamic LDS limit to kResLdsMax (once, outside capture).
 template <typename T> hipError_t rb_res_prepare();
+// CalcAX! fused after the resident preconditioner: ax = A x of its result,
+// s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (when dz), finalized with fin.
+template <typename T>
+struct ResAx {
+    T *ax;
+    const T *dy, *dz;
+    int fin;
+    KState *st;
+    double *hist;
+    const RedBuf *rb;
+};
 // Preconditioner! with :gs (five red-black sweeps) from x = 0 on the
 // right-hand side bb formed per mode (as launch_rb_first), in one launch;
-// with dot also Σ bb·x over the interior, finalized with dot->fin.
+// with dot also Σ bb·x over the interior, finalized with dot->fin; with
+// axf also CalcAX! of the result and its sums (not with dot).
 template <typename T>
 hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
-                              const KState *st, const DotFin *dot, hipStream_t s);
+                              const KState *st, const DotFin *dot, hipStream_t s, const ResAx<T> *axf = nullptr);
 
 }  // namespace h3d

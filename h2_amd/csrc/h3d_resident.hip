@@ -21,16 +21,21 @@
 // the tile's (row, m) index space in both halves.
 //
 // A pass updates the cells of one half from LDS into LDS, and stores the
-// rows on the tile's edge into x in HBM (every row in the last sweep).  The
-// workgroup then publishes its pass counter and waits for the counters of
-// the (at most four) tiles whose edge rows it reads, and loads those rows
-// of the half just updated.  No grid-wide barrier: each tile waits only
-// for its neighbours.  Cross-workgroup data moves with device-scope (sc1)
-// loads and stores, which the L2s of the eight XCDs keep coherent without
-// the write-back and invalidate a device-scope release/acquire fence
-// costs; each wave waits for its stores before the counter is published.
-// Every wait is bounded: a timed-out wait sets an error word the host
-// checks after the solve, and later waits then return at once.
+// rows on the tile's edge into x in HBM (every row in the last sweep); the
+// workgroup then publishes its pass counter.  A pass first updates the
+// rows inside the tile, whose neighbours are all in LDS, while the
+// neighbouring tiles finish the previous pass; then it waits for the
+// counters of the (at most four) tiles whose edge rows it reads, loads
+// those ring rows of the half the previous pass updated, and updates its
+// own edge rows.  No grid-wide barrier: each tile waits only for its
+// neighbours.  Cross-workgroup data moves with device-scope (sc1) loads
+// and stores, which the L2s of the eight XCDs keep coherent without the
+// write-back and invalidate a device-scope release/acquire fence costs;
+// each wave waits for its stores before the counter is published.  Every
+// wait is bounded: a timed-out wait sets an error word the host checks
+// after the solve, and later waits then return at once.  The counters grow
+// by one per pass across launches and start equal (the host zeroes them
+// whenever a solve selects a plan).
 //
 // Every cell performs the per-colour kernels' operations in the same
 // order (rb_offdiag / rb_flux / diag, the from-zero start as explicit
@@ -91,17 +96,22 @@ __device__ __forceinline__ ResTile res_tile(const RbRes &pl, int NY, int NZ)
     return t;
 }
 
-// Publish that this workgroup has finished pass count `want` (its stores of
-// the rows its neighbours read are complete, visible to their device-scope
-// loads), then wait until the neighbouring tiles have too.  The counters
-// grow by one per exchanged pass across launches; all start equal.
-__device__ __forceinline__ void res_neighbour_sync(unsigned *bar, const ResTile &tl, unsigned want)
+// Publish that this workgroup has finished pass count `cnt`: its stores
+// of the rows its neighbours read are complete (visible to their
+// device-scope loads) before the counter moves.  Every thread must call it.
+__device__ __forceinline__ void res_publish(unsigned *bar, unsigned cnt)
 {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
+    if (threadIdx.x == 0) bar_store(bar + kBarFlags + kBarFlagStride * blockIdx.x, cnt);
+}
+
+// Wait until the neighbouring tiles have published pass count `want`.
+// Every thread must call it; ends with a barrier.
+__device__ __forceinline__ void res_wait(unsigned *bar, const ResTile &tl, unsigned want)
+{
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        if (lane == 0) bar_store(bar + kBarFlags + kBarFlagStride * blockIdx.x, want);
         const int nb = lane < 4 ? tl.nb[lane] : -1;
         const unsigned *f = bar + kBarFlags + kBarFlagStride * (nb < 0 ? 0 : nb);
         unsigned n = 0;
@@ -114,6 +124,41 @@ __device__ __forceinline__ void res_neighbour_sync(unsigned *bar, const ResTile 
                 break;
             }
         }
+    }
+    __syncthreads();
+}
+
+// The ring rows of half h (ring row hr = 0..2tj+2tk-1: planes k0-1 and
+// k0+tk, then rows j0-1 and j0+tj of each plane) from x in HBM (xg: that
+// half) into LDS xo.  Only interior cells are read: the others (rows
+// outside the interior, the cells i = 0 and i = NX+1 and the padding of
+// each row) are the domain's halo, zero in LDS.  Every thread must call
+// it; ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void res_load_ring(T *xo, const T *xg, const ResTile &tl, int RJ, int HX, unsigned HXY,
+                                              int NX, int NY, int NZ, int h)
+{
+    const unsigned nring = (unsigned)(2 * tl.tj + 2 * tl.tk) * (unsigned)HX;
+    for (unsigned e = threadIdx.x; e < nring; e += kResThreads) {
+        const unsigned hr = e / (unsigned)HX, m = e - hr * (unsigned)HX;
+        int jj, kk;
+        if (hr < (unsigned)tl.tj) {
+            jj = (int)hr + 1;
+            kk = 0;
+        } else if (hr < 2u * tl.tj) {
+            jj = (int)hr - tl.tj + 1;
+            kk = tl.tk + 1;
+        } else if (hr < 2u * tl.tj + tl.tk) {
+            jj = 0;
+            kk = (int)hr - 2 * tl.tj + 1;
+        } else {
+            jj = tl.tj + 1;
+            kk = (int)hr - 2 * tl.tj - tl.tk + 1;
+        }
+        const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
+        const int i = 2 * (int)m + ((h + j + k) & 1);
+        if (j >= 1 && j <= NY && k >= 1 && k <= NZ && i >= 1 && i <= NX)
+            xo[(kk * RJ + jj) * HX + (int)m] = sc_load(xg + m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k);
     }
     __syncthreads();
 }
@@ -161,10 +206,47 @@ struct ResSlot {
     bool own, edge;
 };
 
-template <typename T, bool DOT, bool FLUX>
+// Up to two per-thread sums over the 1024-thread workgroup into the
+// workgroup's partials (rbuf layout).  Every thread must call it.
+template <int NS>
+__device__ __forceinline__ void res_partials(const dd (&v)[2], dd *red, const RedBuf &rbuf)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const dd w = row_sum(v[s]);
+        if ((lane & 15) == 15) red[s * 68 + wave * 4 + (lane >> 4)] = w;
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const dd z = row_sum(red[s * 68 + lane]);
+            if ((lane & 15) == 15) red[s * 68 + 64 + (lane >> 4)] = z;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const dd *q = red + s * 68 + 64;
+            const dd t = dd_add_tree(dd_add_tree(q[0], q[1]), dd_add_tree(q[2], q[3]));
+            rbuf.partials[6 * (size_t)blockIdx.x + 2 * s] = t.hi;
+            rbuf.partials[6 * (size_t)blockIdx.x + 2 * s + 1] = t.lo;
+        }
+    }
+}
+
+// MODE: 0 = the preconditioner alone; 1 = and Σ bb·x (CG!'s ρ); 2, 3 =
+// and CalcAX! ax = A x of its result with one or two fused sums,
+// s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (dz_is_bb: dz is the right-hand
+// side bb the launch formed).
+template <typename T, int MODE, bool FLUX>
 __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__restrict__ x, T *__restrict__ bb,
                                                              const T *__restrict__ r, const T *__restrict__ q,
-                                                             RbGeo<T> g, RbRes pl, const KState *st, RedBuf rbuf)
+                                                             RbGeo<T> g, RbRes pl, const KState *st, RedBuf rbuf,
+                                                             T *__restrict__ ax, const T *__restrict__ dy,
+                                                             const T *__restrict__ dz, int dz_is_bb)
 {
     if (halted(st)) return;  // uniform: every workgroup reads the same state
     extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
@@ -178,15 +260,35 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     T *xs1 = xs0 + hsz;
     T *tabs = xs1 + hsz;
     const int ntab = 6 * K * K + K;
-    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem + (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) &
-                                                      ~(size_t)15));
+    T *dzs = tabs + (size_t)pl.TK * ntab;  // ΔZ of the tile's planes
+    dd *red = reinterpret_cast<dd *>(
+        h3d_dyn_smem + (((2 * (size_t)hsz + (size_t)pl.TK * (ntab + 1)) * sizeof(T) + 15) & ~(size_t)15));
 
     // the tables of the tile's planes, and the class bytes of the tile's
     // rows and ring (both halves, the domain's halo rows included: their
     // classes select the boundary conductances) in the x region, laid out
     // as x (byte offset = element offset); x is zeroed once they are used
-    for (int kk = 1; kk <= tl.tk; ++kk)
-        rb_stage_into(g, tl.k0 + kk - 1, tabs + (size_t)(kk - 1) * ntab, tid, kResThreads);
+    // (the tables of all planes in one round of loads: the entries
+    // rb_stage_into forms, per plane, with the same operations)
+    {
+        const int n = 2 * K * K, per = n + K;
+        for (int idx = tid; idx < tl.tk * per; idx += kResThreads) {
+            const int kk = idx / per, a = idx - kk * per;
+            const int k = tl.k0 + kk;
+            T *t = tabs + (size_t)kk * ntab;
+            const T dz_k = g.dZ[k];
+            if (a < n) {
+                const T f = g.tab[a];
+                t[a] = f * g.dy0 * dz_k * g.ddx;
+                t[n + a] = f * g.dx0 * dz_k * g.ddy;
+                t[2 * n + a] = g.ZF[(size_t)(k - 1) * K * K + a];
+            } else if (g.ap) {
+                const int c = a - n;
+                t[3 * n + c] = (T)(g.capc[c] * g.dx0d * g.dy0d * g.dZd[k] * g.ddt);
+            }
+            if (a == 0) dzs[kk] = dz_k;
+        }
+    }
     uint8_t *cs0 = reinterpret_cast<uint8_t *>(h3d_dyn_smem);
     uint8_t *cs1 = cs0 + hsz;
     const size_t HS = g.HS;
@@ -246,6 +348,7 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     // pass counter at the start of this launch (equal in every workgroup)
     const unsigned f0 = bar_load(pl.bar + kBarFlags + kBarFlagStride * blockIdx.x);
 
+
     // per slot and half: validity, diagonal, sink (flux form) and the LDS
     // indices of the six conductances, packed two to a word
     const unsigned zs = (unsigned)(RJ * HX);
@@ -274,7 +377,7 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
             c.zm = ct[e - zs];
             c.zp = ct[e + zs];
             const T *stab = tabs + (size_t)(kk - 1) * ntab;
-            const Cond<T> o = rb_cond(g, stab, c, i, j, k);
+            const Cond<T> o = rb_cond_dz(g, stab, c, i, j, k, [&] { return dzs[kk - 1]; });
             dgl[u][h] = diag(o);
             if (FLUX) snk[u][h] = o.cxp + o.cxm + o.cyp + o.cym + o.czp + o.czm + o.ap0;
             const int b = c.c, n2 = 2 * K * K;
@@ -314,18 +417,96 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
         for (int half = 0; half < 2; ++half) {
             // one colour pass: colour 0 updates half 1, colour 1 half 0
             const int h = 1 - half;
+            const int pass = 2 * sweep + half;
             T *xo = h ? xs1 : xs0;
-            const T *xt = h ? xs0 : xs1;
+            T *xt = h ? xs0 : xs1;
             T *xg = x + (h ? HS : 0);
+            // the tile's inner rows, then (after the ring rows of the half
+            // the previous pass updated have arrived) its edge rows
 #pragma unroll
-            for (int u = 0; u < kResSlots; ++u) {
-                if (!sl[u].own) continue;
-                // the offsets and index words are opaque to the compiler
-                // here, so that it forms the addresses inside the loop
-                // instead of hoisting some hundred of them into registers
-                unsigned e = sl[u].e, base = sl[u].base, w0 = ix[u][h][0], w1 = ix[u][h][1],
-                         w2 = ix[u][h][2];
-                asm volatile("" : "+v"(e), "+v"(base), "+v"(w0), "+v"(w1), "+v"(w2));
+            for (int phase = 0; phase < 2; ++phase) {
+                if (phase == 1 && pass > 0) {
+                    res_wait(pl.bar, tl, f0 + (unsigned)pass);
+                    res_load_ring(xt, (const T *)(x + (h ? 0 : HS)), tl, RJ, HX, HXY, g.NX, g.NY, g.NZ, 1 - h);
+                }
+#pragma unroll
+                for (int u = 0; u < kResSlots; ++u) {
+                    if (!sl[u].own || sl[u].edge != (phase == 1)) continue;
+                    // the offsets and index words are opaque to the
+                    // compiler here, so that it forms the addresses inside
+                    // the loop instead of hoisting some hundred of them
+                    // into registers
+                    unsigned e = sl[u].e, base = sl[u].base, w0 = ix[u][h][0], w1 = ix[u][h][1],
+                             w2 = ix[u][h][2];
+                    asm volatile("" : "+v"(e), "+v"(base), "+v"(w0), "+v"(w1), "+v"(w2));
+                    const unsigned sx = (unsigned)h ^ par[u];
+                    Nb<T> n;
+                    n.xm = xt[e - 1u + sx];
+                    n.xp = xt[e + sx];
+                    n.ym = xt[e - (unsigned)HX];
+                    n.yp = xt[e + (unsigned)HX];
+                    n.zm = xt[e - zs];
+                    n.zp = xt[e + zs];
+                    const T pp = xo[e];
+                    Cond<T> o;
+                    o.xm = tabs[w0 & 0xffffu];
+                    o.xp = tabs[w0 >> 16];
+                    o.ym = tabs[w1 & 0xffffu];
+                    o.yp = tabs[w1 >> 16];
+                    o.zm = tabs[w2 & 0xffffu];
+                    o.zp = tabs[w2 >> 16];
+                    T xn;
+                    if (FLUX) {
+                        const T f = o.xp * (n.xp - pp) + o.xm * (n.xm - pp) + o.yp * (n.yp - pp) +
+                                    o.ym * (n.ym - pp) + o.zp * (n.zp - pp) + o.zm * (n.zm - pp);
+                        xn = pp + ((f - snk[u][h] * pp) - bv[u][h]) / dgl[u][h];
+                    } else {
+                        const T dp = (((rb_offdiag(o, n) - bv[u][h]) / dgl[u][h] - pp)) * T(1);
+                        xn = pp + T(1) * dp;
+                    }
+                    if (valid[u][h]) {
+                        xo[e] = xn;
+                        // HBM receives the rows neighbours read, and in the
+                        // last sweep (final values) every row
+                        if (phase == 1) sc_store(xg + base, xn);
+                        else if (sweep == 4) xg[base] = xn;
+                    }
+                    // one slot's loads in flight at a time: sixteen waves
+                    // per CU hide the LDS latency, and the registers stay
+                    // within 128
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            res_publish(pl.bar, f0 + (unsigned)pass + 1u);
+        }
+    }
+
+    if (MODE == 1) {
+        // Σ bb·x over the tile's interior cells of both halves (x final:
+        // half 0 from the last pass, half 1 from the one before; each
+        // thread reads back its own cells)
+        dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int u = 0; u < kResSlots; ++u) {
+            if (valid[u][0]) v[0] = dd_add(v[0], (double)bv[u][0] * (double)xs0[sl[u].e]);
+            if (valid[u][1]) v[0] = dd_add(v[0], (double)bv[u][1] * (double)xs1[sl[u].e]);
+        }
+        res_partials<1>(v, red, rbuf);
+    }
+    if (MODE >= 2) {
+        // CalcAX! (src/NonUniform.jl:454-511) of the result, read from LDS:
+        // the ring rows of half 0 (updated by the last pass) first
+        res_wait(pl.bar, tl, f0 + 10u);
+        res_load_ring(xs0, (const T *)x, tl, RJ, HX, HXY, g.NX, g.NY, g.NZ, 0);
+        dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int u = 0; u < kResSlots; ++u) {
+            if (!sl[u].own) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                unsigned e = sl[u].e, w0 = ix[u][h][0], w1 = ix[u][h][1], w2 = ix[u][h][2];
+                asm volatile("" : "+v"(e), "+v"(w0), "+v"(w1), "+v"(w2));
+                const T *xo = h ? xs1 : xs0, *xt = h ? xs0 : xs1;
                 const unsigned sx = (unsigned)h ^ par[u];
                 Nb<T> n;
                 n.xm = xt[e - 1u + sx];
@@ -334,7 +515,7 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
                 n.yp = xt[e + (unsigned)HX];
                 n.zm = xt[e - zs];
                 n.zp = xt[e + zs];
-                const T pp = xo[e];
+                const T x0 = xo[e];
                 Cond<T> o;
                 o.xm = tabs[w0 & 0xffffu];
                 o.xp = tabs[w0 >> 16];
@@ -342,80 +523,30 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
                 o.yp = tabs[w1 >> 16];
                 o.zm = tabs[w2 & 0xffffu];
                 o.zp = tabs[w2 >> 16];
-                T xn;
+                // rb_apply with the diagonal (and sink) formed once per launch
+                T av;
                 if (FLUX) {
-                    const T f = o.xp * (n.xp - pp) + o.xm * (n.xm - pp) + o.yp * (n.yp - pp) + o.ym * (n.ym - pp) +
-                                o.zp * (n.zp - pp) + o.zm * (n.zm - pp);
-                    xn = pp + ((f - snk[u][h] * pp) - bv[u][h]) / dgl[u][h];
+                    const T f = o.xp * (n.xp - x0) + o.xm * (n.xm - x0) + o.yp * (n.yp - x0) + o.ym * (n.ym - x0) +
+                                o.zp * (n.zp - x0) + o.zm * (n.zm - x0);
+                    av = f - snk[u][h] * x0;
                 } else {
-                    const T dp = (((rb_offdiag(o, n) - bv[u][h]) / dgl[u][h] - pp)) * T(1);
-                    xn = pp + T(1) * dp;
+                    av = (rb_offdiag(o, n) - dgl[u][h] * x0) * T(1);
                 }
                 if (valid[u][h]) {
-                    xo[e] = xn;
-                    // HBM receives the rows neighbours read, and in the
-                    // last sweep (final values) every row
-                    if (sl[u].edge || sweep == 4) sc_store(xg + base, xn);
+                    const size_t c = (h ? HS : 0) + sl[u].base;
+                    ax[c] = av;
+                    const T yv = dy ? dy[c] : av;
+                    v[0] = dd_add(v[0], (double)av * (double)yv);
+                    if (MODE == 3) {
+                        const T zv = dz_is_bb ? bv[u][h] : dz[c];
+                        v[1] = dd_add(v[1], (double)av * (double)zv);
+                    }
                 }
-                // one slot's loads in flight at a time: sixteen waves per CU
-                // hide the LDS latency, and the registers stay within 128
                 __builtin_amdgcn_sched_barrier(0);
             }
-            if (sweep == 4 && half == 1) break;
-            // the neighbours' edge rows of half h, once they have stored
-            // them (rows outside the interior are the domain halo: zero,
-            // never loaded)
-            res_neighbour_sync(pl.bar, tl, f0 + (unsigned)(2 * sweep + half + 1));
-            const unsigned nh = (unsigned)(2 * tl.tj + 2 * tl.tk);
-            for (unsigned e = tid; e < nh * (unsigned)HX; e += kResThreads) {
-                const unsigned hr = e / (unsigned)HX, m = e - hr * (unsigned)HX;
-                int jj, kk;
-                if (hr < (unsigned)tl.tj) {
-                    jj = (int)hr + 1;
-                    kk = 0;
-                } else if (hr < 2u * tl.tj) {
-                    jj = (int)hr - tl.tj + 1;
-                    kk = tl.tk + 1;
-                } else if (hr < 2u * tl.tj + tl.tk) {
-                    jj = 0;
-                    kk = (int)hr - 2 * tl.tj + 1;
-                } else {
-                    jj = tl.tj + 1;
-                    kk = (int)hr - 2 * tl.tj - tl.tk + 1;
-                }
-                const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
-                if (j >= 1 && j <= g.NY && k >= 1 && k <= g.NZ)
-                    xo[(kk * RJ + jj) * HX + (int)m] =
-                        sc_load(xg + m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k);
-            }
-            __syncthreads();
         }
-    }
-
-    if (DOT) {
-        // Σ bb·x over the tile's interior cells of both halves (x final:
-        // half 0 from the last pass, half 1 from the one before; each
-        // thread reads back its own cells)
-        dd v = {0.0, 0.0};
-#pragma unroll
-        for (int u = 0; u < kResSlots; ++u) {
-            if (valid[u][0]) v = dd_add(v, (double)bv[u][0] * (double)xs0[sl[u].e]);
-            if (valid[u][1]) v = dd_add(v, (double)bv[u][1] * (double)xs1[sl[u].e]);
-        }
-        const int lane = tid & 63, wave = tid >> 6;
-        const dd w = row_sum(v);
-        if ((lane & 15) == 15) red[wave * 4 + (lane >> 4)] = w;
-        __syncthreads();
-        if (wave == 0) {
-            const dd z = row_sum(red[lane]);
-            if ((lane & 15) == 15) red[64 + (lane >> 4)] = z;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            const dd t = dd_add_tree(dd_add_tree(red[64], red[65]), dd_add_tree(red[66], red[67]));
-            rbuf.partials[6 * (size_t)blockIdx.x] = t.hi;
-            rbuf.partials[6 * (size_t)blockIdx.x + 1] = t.lo;
-        }
+        if (MODE == 2) res_partials<1>(v, red, rbuf);
+        else res_partials<2>(v, red, rbuf);
     }
 }
 
@@ -424,9 +555,9 @@ size_t rb_res_lds(int TJ, int TK, int HX, int K)
 {
     const size_t hsz = (size_t)(TJ + 2) * (TK + 2) * HX + 1;
     const size_t ntab = (size_t)6 * K * K + K;
-    size_t n = (2 * hsz + (size_t)TK * ntab) * sizeof(T);
+    size_t n = (2 * hsz + (size_t)TK * (ntab + 1)) * sizeof(T);
     n = (n + 15) & ~(size_t)15;
-    return n + 68 * sizeof(dd);
+    return n + 2 * 68 * sizeof(dd);
 }
 
 // Tile plan: the tiling with the fewest slot rounds per thread (compute per
@@ -466,11 +597,18 @@ RbRes rb_res_plan(const RbGeo<T> &g, int ncu)
     return best;
 }
 
-template <typename T, bool DOT, bool FLUX>
+template <typename T, int MODE, bool FLUX>
 static hipError_t res_attr()
 {
-    return hipFuncSetAttribute((const void *)k_rb_resident<T, DOT, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    return hipFuncSetAttribute((const void *)k_rb_resident<T, MODE, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kResLdsMax);
+}
+
+template <typename T, int MODE>
+static hipError_t res_attr_mode()
+{
+    hipError_t e = res_attr<T, MODE, false>();
+    return e != hipSuccess ? e : res_attr<T, MODE, true>();
 }
 
 // Allow the resident kernels the LDS budget (above the default dynamic
@@ -479,39 +617,61 @@ template <typename T>
 hipError_t rb_res_prepare()
 {
     hipError_t e;
-    if ((e = res_attr<T, false, false>()) != hipSuccess) return e;
-    if ((e = res_attr<T, false, true>()) != hipSuccess) return e;
-    if ((e = res_attr<T, true, false>()) != hipSuccess) return e;
-    return res_attr<T, true, true>();
+    if ((e = res_attr_mode<T, 0>()) != hipSuccess) return e;
+    if ((e = res_attr_mode<T, 1>()) != hipSuccess) return e;
+    if ((e = res_attr_mode<T, 2>()) != hipSuccess) return e;
+    return res_attr_mode<T, 3>();
+}
+
+template <typename T, int MODE>
+static void res_launch(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
+                       const KState *st, const RedBuf &rb, T *ax, const T *dy, const T *dz, int dz_is_bb,
+                       hipStream_t s)
+{
+    const dim3 grid(pl.G), block(kResThreads);
+    if (g.flux)
+        hipLaunchKernelGGL((k_rb_resident<T, MODE, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb, ax,
+                           dy, dz, dz_is_bb);
+    else
+        hipLaunchKernelGGL((k_rb_resident<T, MODE, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb,
+                           ax, dy, dz, dz_is_bb);
 }
 
 template <typename T>
 hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
-                              const KState *st, const DotFin *dot, hipStream_t s)
+                              const KState *st, const DotFin *dot, hipStream_t s, const ResAx<T> *axf)
 {
-    if (pl.G <= 0 || !pl.bar || pl.lds > kResLdsMax) return hipErrorInvalidValue;
-    if (dot && pl.G > dot->rb->nblocks_max) return hipErrorInvalidValue;
-    const dim3 grid(pl.G), block(kResThreads);
+    if (pl.G <= 0 || !pl.bar || pl.lds > kResLdsMax || (dot && axf)) return hipErrorInvalidValue;
+    const RedBuf *rbp = dot ? dot->rb : axf ? axf->rb : nullptr;
+    if (rbp && pl.G > rbp->nblocks_max) return hipErrorInvalidValue;
     const RedBuf none{};
-    const RedBuf &rb = dot ? *dot->rb : none;
+    const RedBuf &rb = rbp ? *rbp : none;
     if (dot) {
-        if (g.flux) hipLaunchKernelGGL((k_rb_resident<T, true, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
-        else hipLaunchKernelGGL((k_rb_resident<T, true, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
+        res_launch<T, 1>(mode, x, bb, r, q, g, pl, st, rb, nullptr, nullptr, nullptr, 0, s);
+    } else if (axf) {
+        const int zb = axf->dz != nullptr && axf->dz == bb;
+        if (axf->dz) res_launch<T, 3>(mode, x, bb, r, q, g, pl, st, rb, axf->ax, axf->dy, axf->dz, zb, s);
+        else res_launch<T, 2>(mode, x, bb, r, q, g, pl, st, rb, axf->ax, axf->dy, nullptr, 0, s);
     } else {
-        if (g.flux) hipLaunchKernelGGL((k_rb_resident<T, false, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
-        else hipLaunchKernelGGL((k_rb_resident<T, false, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb);
+        res_launch<T, 0>(mode, x, bb, r, q, g, pl, st, rb, nullptr, nullptr, nullptr, 0, s);
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !dot) return e;
-    return launch_finalize_sums<T>(1, (unsigned)pl.G, dot->fin, dot->st, dot->hist, *dot->rb, s,
-                                   seq_terms<T>(g, bb, x));
+    if (e != hipSuccess) return e;
+    if (dot)
+        return launch_finalize_sums<T>(1, (unsigned)pl.G, dot->fin, dot->st, dot->hist, *dot->rb, s,
+                                       seq_terms<T>(g, bb, x));
+    if (axf)
+        return launch_finalize_sums<T>(axf->dz ? 2 : 1, (unsigned)pl.G, axf->fin, axf->st, axf->hist, *axf->rb, s,
+                                       seq_terms<T>(g, axf->ax, axf->dy, axf->ax, axf->dz));
+    return hipSuccess;
 }
 
 #define RES_INST(T)                                                                                              \
     template RbRes rb_res_plan<T>(const RbGeo<T> &, int);                                                       \
     template hipError_t rb_res_prepare<T>();                                                                    \
     template hipError_t launch_rb_resident<T>(int, T *, T *, const T *, const T *, const RbGeo<T> &,          \
-                                              const RbRes &, const KState *, const DotFin *, hipStream_t);
+                                              const RbRes &, const KState *, const DotFin *, hipStream_t,     \
+                                              const ResAx<T> *);
 
 RES_INST(double)
 RES_INST(float)
