@@ -75,6 +75,7 @@ struct h3d_ctx {
     int res_wgs = 0;                      // workgroups of the last solve's resident preconditioner (0: none)
     int ncu = 0;                          // compute units of the device
     unsigned *d_resbar[2] = {};           // resident preconditioner: synchronisation words (FP64, FP32)
+    void *d_rescf[2][3] = {};             // and its per-cell coefficients: diagonal, sink, index words
     bool res_prepared[2] = {false, false}; // its kernels' LDS limit raised (FP64, FP32)
     double *rbswap64 = nullptr;           // second buffer of the fused (out-of-place) sweeps
     float *rbswap32 = nullptr;
@@ -774,6 +775,17 @@ int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
                 if (!c->d_resbar[pi]) HIPCHK(c, hipMalloc(&c->d_resbar[pi], kResBarWords * sizeof(unsigned)));
                 HIPCHK(c, hipMemsetAsync(c->d_resbar[pi], 0, kResBarWords * sizeof(unsigned), c->stream));
                 pl.bar = c->d_resbar[pi];
+                // the operator's per-cell coefficients, formed for this solve
+                void **cf = c->d_rescf[pi];
+                const size_t ncell = 2 * c->HS;
+                if (!cf[0]) HIPCHK(c, hipMalloc(&cf[0], ncell * sizeof(T)));
+                if (!cf[1]) HIPCHK(c, hipMalloc(&cf[1], ncell * sizeof(T)));
+                if (!cf[2]) HIPCHK(c, hipMalloc(&cf[2], 3 * ncell * sizeof(unsigned)));
+                pl.dgl = cf[0];
+                pl.snk = cf[1];
+                pl.ix = static_cast<unsigned *>(cf[2]);
+                const ResCoef<T> rc{static_cast<T *>(cf[0]), P->rg.flux ? static_cast<T *>(cf[1]) : nullptr, pl.ix};
+                HIPCHK(c, launch_rb_res_coef<T>(P->rg, rc, c->stream));
                 P->res = pl;
                 P->chunk = 0;  // the resident launch replaces the per-colour and two-colour sweeps
             }
@@ -1169,7 +1181,9 @@ int h3d_destroy(h3d_ctx *c)
         if (c->d_gacc[a]) (void)hipFree(c->d_gacc[a]);
     void *ps[] = {c->d_ZC,  c->d_dZ,   c->d_ap0,  c->d_ZC32,      c->d_dZ32, c->d_ap032, c->d_DZC32, c->d_state,
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
-                  c->d_corr, c->d_best, c->d_prev, c->d_sor_r, c->d_resbar[0], c->d_resbar[1]};
+                  c->d_corr, c->d_best, c->d_prev, c->d_sor_r, c->d_resbar[0], c->d_resbar[1],
+                  c->d_rescf[0][0], c->d_rescf[0][1], c->d_rescf[0][2], c->d_rescf[1][0], c->d_rescf[1][1],
+                  c->d_rescf[1][2]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);

@@ -171,6 +171,16 @@ struct RbRes {
     int G = 0;
     size_t lds = 0;             // dynamic LDS bytes per workgroup
     unsigned *bar = nullptr;    // device synchronisation words (kResBarWords)
+    // per-cell coefficients (launch_rb_res_coef), 2*HS entries each: the
+    // diagonal and the flux form's sink (elements of the solve's type), and
+    // three index words per cell (ix + 2*HS*w, w = 0..2)
+    void *dgl = nullptr, *snk = nullptr;
+    unsigned *ix = nullptr;
+};
+template <typename T>
+struct ResCoef {
+    T *dgl, *snk;
+    unsigned *ix;
 };
 constexpr size_t kResLdsMax = 150 * 1024;
 constexpr int kResMaxWG = 1024;
@@ -407,6 +417,8 @@ hipError_t launch_rb_cg_p(T *p, const T *z, const RbGeo<T> &g, const KState *st,
 template <typename T> RbRes rb_res_plan(const RbGeo<T> &g, int ncu);
 // Raise the kernels' dynamic LDS limit to kResLdsMax (once, outside capture).
 template <typename T> hipError_t rb_res_prepare();
+// The per-cell coefficients of the operator g into cf (once per solve).
+template <typename T> hipError_t launch_rb_res_coef(const RbGeo<T> &g, const ResCoef<T> &cf, hipStream_t s);
 // CalcAX! fused after the resident preconditioner: ax = A x of its result,
 // s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (when dz), finalized with fin.
 template <typename T>

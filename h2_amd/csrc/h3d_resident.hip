@@ -206,6 +206,52 @@ struct ResSlot {
     bool own, edge;
 };
 
+// Per-cell coefficients of the resident launches, formed once per solve
+// (they depend on the operator only): for every interior cell of both
+// halves the diagonal dd = diag(Cond), the sink Σconv + ap0 of the flux
+// form, and the plane-relative indices of its six conductances in the
+// staged tables (rb_stage_into's layout), two 16-bit indices to a word.
+// The same rb_cond the per-colour kernels evaluate, from the same staged
+// tables, so the values are bitwise theirs.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rb_res_coef(RbGeo<T> g, ResCoef<T> cf)
+{
+    const RbThread t = rb_thread(g);
+    const T *stab = rb_stage(g, t.k);
+    if (!t.live) return;
+    const unsigned HX = (unsigned)g.HX, HXY = HX * (unsigned)g.MY, HS = (unsigned)g.HS;
+    const unsigned base = (unsigned)t.m + HX * (unsigned)t.j + HXY * (unsigned)t.k;
+    const int K = g.K, n2 = 2 * K * K;
+    const int h0 = (t.j + t.k) & 1;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int h = h0 ^ a, i = 2 * t.m + a;
+        if (i < 1 || i > g.NX) continue;
+        const unsigned co = (h ? HS : 0u) + base, no = (h ? 0u : HS) + base;
+        const Cls q = rb_cls(g.cls, g.cls, co, no, a, HX, HXY);
+        const Cond<T> o = rb_cond(g, stab, q, i, t.j, t.k);
+        cf.dgl[co] = diag(o);
+        if (cf.snk) cf.snk[co] = o.cxp + o.cxm + o.cyp + o.cym + o.czp + o.czm + o.ap0;
+        const int b = q.c;
+        const unsigned ixm = ((i > 1 ? K : 0) + q.xm) * K + b;
+        const unsigned ixp = ((i < g.NX ? K : 0) + q.xp) * K + b;
+        const unsigned iym = n2 + ((t.j > 1 ? K : 0) + q.ym) * K + b;
+        const unsigned iyp = n2 + ((t.j < g.NY ? K : 0) + q.yp) * K + b;
+        const unsigned izm = 2 * n2 + q.zm * K + b;
+        const unsigned izp = 2 * n2 + K * K + q.zp * K + b;
+        cf.ix[co] = ixm | (ixp << 16);
+        cf.ix[2 * (size_t)HS + co] = iym | (iyp << 16);
+        cf.ix[4 * (size_t)HS + co] = izm | (izp << 16);
+    }
+}
+
+template <typename T>
+hipError_t launch_rb_res_coef(const RbGeo<T> &g, const ResCoef<T> &cf, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rb_res_coef<T>, dim3(g.nb1), dim3(BX, BY), rb_lds(g), s, g, cf);
+    return hipGetLastError();
+}
+
 // Up to two per-thread sums over the 1024-thread workgroup into the
 // workgroup's partials (rbuf layout).  Every thread must call it.
 template <int NS>
@@ -244,8 +290,8 @@ __device__ __forceinline__ void res_partials(const dd (&v)[2], dd *red, const Re
 template <typename T, int MODE, bool FLUX>
 __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__restrict__ x, T *__restrict__ bb,
                                                              const T *__restrict__ r, const T *__restrict__ q,
-                                                             RbGeo<T> g, RbRes pl, const KState *st, RedBuf rbuf,
-                                                             T *__restrict__ ax, const T *__restrict__ dy,
+                                                             RbGeo<T> g, RbRes pl, ResCoef<T> cf, const KState *st,
+                                                             RedBuf rbuf, T *__restrict__ ax, const T *__restrict__ dy,
                                                              const T *__restrict__ dz, int dz_is_bb)
 {
     if (halted(st)) return;  // uniform: every workgroup reads the same state
@@ -260,16 +306,11 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     T *xs1 = xs0 + hsz;
     T *tabs = xs1 + hsz;
     const int ntab = 6 * K * K + K;
-    T *dzs = tabs + (size_t)pl.TK * ntab;  // ΔZ of the tile's planes
-    dd *red = reinterpret_cast<dd *>(
-        h3d_dyn_smem + (((2 * (size_t)hsz + (size_t)pl.TK * (ntab + 1)) * sizeof(T) + 15) & ~(size_t)15));
+    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
+                                     (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
 
-    // the tables of the tile's planes, and the class bytes of the tile's
-    // rows and ring (both halves, the domain's halo rows included: their
-    // classes select the boundary conductances) in the x region, laid out
-    // as x (byte offset = element offset); x is zeroed once they are used
-    // (the tables of all planes in one round of loads: the entries
-    // rb_stage_into forms, per plane, with the same operations)
+    // the tables of the tile's planes in one round of loads: the entries
+    // rb_stage_into forms, per plane, with the same operations
     {
         const int n = 2 * K * K, per = n + K;
         for (int idx = tid; idx < tl.tk * per; idx += kResThreads) {
@@ -286,23 +327,9 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
                 const int c = a - n;
                 t[3 * n + c] = (T)(g.capc[c] * g.dx0d * g.dy0d * g.dZd[k] * g.ddt);
             }
-            if (a == 0) dzs[kk] = dz_k;
         }
     }
-    uint8_t *cs0 = reinterpret_cast<uint8_t *>(h3d_dyn_smem);
-    uint8_t *cs1 = cs0 + hsz;
     const size_t HS = g.HS;
-    {
-        const unsigned rows = (unsigned)((tl.tk + 2) * (tl.tj + 2));
-        for (unsigned e = tid; e < 2 * rows * (unsigned)HX; e += kResThreads) {
-            const unsigned hh = e / (rows * (unsigned)HX), rem = e - hh * rows * (unsigned)HX;
-            const unsigned row = rem / (unsigned)HX, m = rem - row * (unsigned)HX;
-            const unsigned kk = row / (unsigned)(tl.tj + 2), jj = row - kk * (unsigned)(tl.tj + 2);
-            const unsigned j = (unsigned)tl.j0 - 1u + jj, k = (unsigned)tl.k0 - 1u + kk;
-            (hh ? cs1 : cs0)[(kk * (unsigned)RJ + jj) * (unsigned)HX + m] =
-                g.cls[(hh ? HS : 0) + m + (unsigned)HX * j + HXY * k];
-        }
-    }
 
     // slots and the right-hand side (formed as k_rb_first forms it)
     T beta = T(0), omg = T(0), r_alpha = T(0);
@@ -318,6 +345,11 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     ResSlot sl[kResSlots];
     T bv[kResSlots][2];
     unsigned par[kResSlots];  // (j + k) & 1 of the slot
+    // per slot and half: validity, diagonal, sink (flux form) and the LDS
+    // indices of the six conductances, packed two to a word
+    bool valid[kResSlots][2];
+    T dgl[kResSlots][2], snk[kResSlots][2] = {};
+    unsigned ix[kResSlots][2][3];
 #pragma unroll
     for (int u = 0; u < kResSlots; ++u) {
         const unsigned qi = (unsigned)tid + (unsigned)kResThreads * u;
@@ -333,6 +365,8 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
         sl[u].e = (unsigned)((kk * RJ + jj) * HX + m);
         sl[u].base = (unsigned)m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k;
         const unsigned o = sl[u].base;
+        // the tile's plane offset in the staged tables, in both 16-bit halves
+        const unsigned tb = (unsigned)((kk - 1) * ntab) * 0x10001u;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const size_t oh = (h ? HS : 0) + o;
@@ -341,63 +375,22 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
             else if (mode == FIRST_BICG_S) v = r_alpha * q[oh] + r[oh];
             else v = bb[oh];
             bv[u][h] = v;
+            const int i = 2 * m + (int)((unsigned)h ^ par[u]);
+            valid[u][h] = sl[u].own && i >= 1 && i <= g.NX;
+            // cells outside the interior have no coefficients (never stored)
+            dgl[u][h] = valid[u][h] ? cf.dgl[oh] : T(1);
+            if (FLUX) snk[u][h] = valid[u][h] ? cf.snk[oh] : T(0);
+#pragma unroll
+            for (int w = 0; w < 3; ++w) ix[u][h][w] = valid[u][h] ? cf.ix[2 * w * HS + oh] + tb : tb;
         }
     }
-    __syncthreads();  // tables and class tile staged
+    __syncthreads();  // tables staged
 
     // pass counter at the start of this launch (equal in every workgroup)
     const unsigned f0 = bar_load(pl.bar + kBarFlags + kBarFlagStride * blockIdx.x);
 
 
-    // per slot and half: validity, diagonal, sink (flux form) and the LDS
-    // indices of the six conductances, packed two to a word
     const unsigned zs = (unsigned)(RJ * HX);
-    bool valid[kResSlots][2];
-    T dgl[kResSlots][2], snk[kResSlots][2] = {};
-    unsigned ix[kResSlots][2][3];
-#pragma unroll
-    for (int u = 0; u < kResSlots; ++u)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const unsigned e = sl[u].e;
-            const unsigned row = e / (unsigned)HX;
-            const int m = (int)(e - row * (unsigned)HX);
-            const int kk = (int)(row / (unsigned)RJ), jj = (int)(row - (unsigned)kk * (unsigned)RJ);
-            const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
-            const unsigned sx = (unsigned)h ^ par[u];
-            const int i = 2 * m + (int)sx;
-            valid[u][h] = sl[u].own && i >= 1 && i <= g.NX;
-            const uint8_t *co = h ? cs1 : cs0, *ct = h ? cs0 : cs1;
-            Cls c;
-            c.c = co[e];
-            c.xm = ct[e - 1u + sx];
-            c.xp = ct[e + sx];
-            c.ym = ct[e - (unsigned)HX];
-            c.yp = ct[e + (unsigned)HX];
-            c.zm = ct[e - zs];
-            c.zp = ct[e + zs];
-            const T *stab = tabs + (size_t)(kk - 1) * ntab;
-            const Cond<T> o = rb_cond_dz(g, stab, c, i, j, k, [&] { return dzs[kk - 1]; });
-            dgl[u][h] = diag(o);
-            if (FLUX) snk[u][h] = o.cxp + o.cxm + o.cyp + o.cym + o.czp + o.czm + o.ap0;
-            const int b = c.c, n2 = 2 * K * K;
-            const unsigned tb = (unsigned)((kk - 1) * ntab);
-            const unsigned ixm = tb + ((i > 1 ? K : 0) + c.xm) * K + b;
-            const unsigned ixp = tb + ((i < g.NX ? K : 0) + c.xp) * K + b;
-            const unsigned iym = tb + n2 + ((j > 1 ? K : 0) + c.ym) * K + b;
-            const unsigned iyp = tb + n2 + ((j < g.NY ? K : 0) + c.yp) * K + b;
-            const unsigned izm = tb + 2 * n2 + c.zm * K + b;
-            const unsigned izp = tb + 2 * n2 + K * K + c.zp * K + b;
-            ix[u][h][0] = ixm | (ixp << 16);
-            ix[u][h][1] = iym | (iyp << 16);
-            ix[u][h][2] = izm | (izp << 16);
-            // formed here, one slot at a time: left to the compiler, the
-            // diagonals sink towards their use in the pass loop and the
-            // conductances of all eight stay live until then (spilled)
-            asm volatile("" : "+v"(dgl[u][h]), "+v"(snk[u][h]), "+v"(ix[u][h][0]), "+v"(ix[u][h][1]),
-                         "+v"(ix[u][h][2]));
-        }
-    __syncthreads();  // class tile read
     // x = 0 everywhere in LDS (the preconditioner starts from zero; the
     // domain's halo rows and columns stay zero)
     for (unsigned e = tid; e < 2 * hsz; e += kResThreads) xs0[e] = T(0);
@@ -555,7 +548,7 @@ size_t rb_res_lds(int TJ, int TK, int HX, int K)
 {
     const size_t hsz = (size_t)(TJ + 2) * (TK + 2) * HX + 1;
     const size_t ntab = (size_t)6 * K * K + K;
-    size_t n = (2 * hsz + (size_t)TK * (ntab + 1)) * sizeof(T);
+    size_t n = (2 * hsz + (size_t)TK * ntab) * sizeof(T);
     n = (n + 15) & ~(size_t)15;
     return n + 2 * 68 * sizeof(dd);
 }
@@ -625,35 +618,37 @@ hipError_t rb_res_prepare()
 
 template <typename T, int MODE>
 static void res_launch(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
-                       const KState *st, const RedBuf &rb, T *ax, const T *dy, const T *dz, int dz_is_bb,
-                       hipStream_t s)
+                       const ResCoef<T> &cf, const KState *st, const RedBuf &rb, T *ax, const T *dy, const T *dz,
+                       int dz_is_bb, hipStream_t s)
 {
     const dim3 grid(pl.G), block(kResThreads);
     if (g.flux)
-        hipLaunchKernelGGL((k_rb_resident<T, MODE, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb, ax,
-                           dy, dz, dz_is_bb);
-    else
-        hipLaunchKernelGGL((k_rb_resident<T, MODE, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, st, rb,
+        hipLaunchKernelGGL((k_rb_resident<T, MODE, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, cf, st, rb,
                            ax, dy, dz, dz_is_bb);
+    else
+        hipLaunchKernelGGL((k_rb_resident<T, MODE, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, cf, st,
+                           rb, ax, dy, dz, dz_is_bb);
 }
 
 template <typename T>
 hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
                               const KState *st, const DotFin *dot, hipStream_t s, const ResAx<T> *axf)
 {
+    const ResCoef<T> cf{reinterpret_cast<T *>(pl.dgl), reinterpret_cast<T *>(pl.snk), pl.ix};
+    if (!cf.dgl || !cf.ix || (g.flux && !cf.snk)) return hipErrorInvalidValue;
     if (pl.G <= 0 || !pl.bar || pl.lds > kResLdsMax || (dot && axf)) return hipErrorInvalidValue;
     const RedBuf *rbp = dot ? dot->rb : axf ? axf->rb : nullptr;
     if (rbp && pl.G > rbp->nblocks_max) return hipErrorInvalidValue;
     const RedBuf none{};
     const RedBuf &rb = rbp ? *rbp : none;
     if (dot) {
-        res_launch<T, 1>(mode, x, bb, r, q, g, pl, st, rb, nullptr, nullptr, nullptr, 0, s);
+        res_launch<T, 1>(mode, x, bb, r, q, g, pl, cf, st, rb, nullptr, nullptr, nullptr, 0, s);
     } else if (axf) {
         const int zb = axf->dz != nullptr && axf->dz == bb;
-        if (axf->dz) res_launch<T, 3>(mode, x, bb, r, q, g, pl, st, rb, axf->ax, axf->dy, axf->dz, zb, s);
-        else res_launch<T, 2>(mode, x, bb, r, q, g, pl, st, rb, axf->ax, axf->dy, nullptr, 0, s);
+        if (axf->dz) res_launch<T, 3>(mode, x, bb, r, q, g, pl, cf, st, rb, axf->ax, axf->dy, axf->dz, zb, s);
+        else res_launch<T, 2>(mode, x, bb, r, q, g, pl, cf, st, rb, axf->ax, axf->dy, nullptr, 0, s);
     } else {
-        res_launch<T, 0>(mode, x, bb, r, q, g, pl, st, rb, nullptr, nullptr, nullptr, 0, s);
+        res_launch<T, 0>(mode, x, bb, r, q, g, pl, cf, st, rb, nullptr, nullptr, nullptr, 0, s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -669,6 +664,7 @@ hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, con
 #define RES_INST(T)                                                                                              \
     template RbRes rb_res_plan<T>(const RbGeo<T> &, int);                                                       \
     template hipError_t rb_res_prepare<T>();                                                                    \
+    template hipError_t launch_rb_res_coef<T>(const RbGeo<T> &, const ResCoef<T> &, hipStream_t);              \
     template hipError_t launch_rb_resident<T>(int, T *, T *, const T *, const T *, const RbGeo<T> &,          \
                                               const RbRes &, const KState *, const DotFin *, hipStream_t,     \
                                               const ResAx<T> *);
