@@ -76,6 +76,7 @@ struct h3d_ctx {
     int ncu = 0;                          // compute units of the device
     unsigned *d_resbar[2] = {};           // resident preconditioner: synchronisation words (FP64, FP32)
     void *d_rescf[2][3] = {};             // and its per-cell coefficients: diagonal, sink, index words
+    double *d_respart[2] = {};            // and the partials of its in-launch grid sums
     bool res_prepared[2] = {false, false}; // its kernels' LDS limit raised (FP64, FP32)
     double *rbswap64 = nullptr;           // second buffer of the fused (out-of-place) sweeps
     float *rbswap32 = nullptr;
@@ -466,6 +467,18 @@ hipError_t enqueue_bicg_iter(h3d_ctx *c, const KVecs<T> &v, const Path<T> &P, in
     hipStream_t s = c->stream;
     KState *st = c->d_state;
     hipError_t e;
+    if (P.rb && P.res.G > 0 && smoother == H3D_SMOOTHER_GS && !c->rb.seq) {
+        // two resident launches and one finalize: p formed, p_ = M⁻¹p,
+        // q = A p_ and the partials of (q, r0); then alpha, s, s_ = M⁻¹s,
+        // t = A s_, omega, the update of θ and r, and the partials of the
+        // residual and the next ρ, finalized
+        const ResAx<T> aq{v.q, v.r0, nullptr, -1, st, nullptr, &c->rb};
+        if ((e = launch_rb_resident<T>(FIRST_BICG_P, v.p_, v.p, v.r, v.q, P.rg, P.res, st, nullptr, s, &aq)) !=
+            hipSuccess)
+            return e;
+        return launch_rb_resident_bicg_s<T>(v.s_, v.s, v.r, v.q, v.t, v.p_, v.x, v.xbest, v.r, v.r0, P.rg, P.res, st,
+                                            c->d_hist, c->rb, s);
+    }
     if (P.rb && P.res.G > 0 && smoother == H3D_SMOOTHER_GS) {
         // the resident preconditioner with CalcAX! and its sums fused:
         // p formed, p_ = M⁻¹p, q = A p_, alpha
@@ -784,6 +797,8 @@ int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
                 pl.dgl = cf[0];
                 pl.snk = cf[1];
                 pl.ix = static_cast<unsigned *>(cf[2]);
+                if (!c->d_respart[pi]) HIPCHK(c, hipMalloc(&c->d_respart[pi], 4 * kResMaxWG * sizeof(double)));
+                pl.wpart = c->d_respart[pi];
                 const ResCoef<T> rc{static_cast<T *>(cf[0]), P->rg.flux ? static_cast<T *>(cf[1]) : nullptr, pl.ix};
                 HIPCHK(c, launch_rb_res_coef<T>(P->rg, rc, c->stream));
                 P->res = pl;
@@ -1183,7 +1198,7 @@ int h3d_destroy(h3d_ctx *c)
                   c->d_unit, c->d_hist, c->d_cls, c->d_vals, c->d_capc, c->d_tab64, c->d_tab32, c->rb.partials,
                   c->d_corr, c->d_best, c->d_prev, c->d_sor_r, c->d_resbar[0], c->d_resbar[1],
                   c->d_rescf[0][0], c->d_rescf[0][1], c->d_rescf[0][2], c->d_rescf[1][0], c->d_rescf[1][1],
-                  c->d_rescf[1][2]};
+                  c->d_rescf[1][2], c->d_respart[0], c->d_respart[1]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);

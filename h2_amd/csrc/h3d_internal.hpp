@@ -176,17 +176,27 @@ struct RbRes {
     // three index words per cell (ix + 2*HS*w, w = 0..2)
     void *dgl = nullptr, *snk = nullptr;
     unsigned *ix = nullptr;
+    double *wpart = nullptr;    // partials of sums reduced inside a launch (4 per workgroup)
 };
 template <typename T>
 struct ResCoef {
     T *dgl, *snk;
     unsigned *ix;
 };
+// The update fused into PBiCGSTAB!'s second resident launch
+template <typename T>
+struct ResUpd {
+    double *th = nullptr, *xbest = nullptr;  // θ (standard layout) and the guarded solve's best iterate
+    T *r = nullptr;                          // r, written
+    const T *p_ = nullptr, *r0 = nullptr;
+    double *wpart = nullptr;                 // the launch's own partials of (t, t), (t, s)
+    const double *apart = nullptr;           // the first launch's partials of (q, r0)
+};
 constexpr size_t kResLdsMax = 150 * 1024;
 constexpr int kResMaxWG = 1024;
 // synchronisation words: an error word, then a pass counter per workgroup
 // (64 bytes apart); the host zeroes them whenever a solve selects a plan
-constexpr int kResBarWords = 64 + 16 * kResMaxWG;
+constexpr int kResBarWords = 64 + 16 * kResMaxWG + 16 * 10;
 constexpr int kResBarErrWord = 0;
 
 // A grid sum formed inside a launch and its finalize step
@@ -419,6 +429,16 @@ template <typename T> RbRes rb_res_plan(const RbGeo<T> &g, int ncu);
 template <typename T> hipError_t rb_res_prepare();
 // The per-cell coefficients of the operator g into cf (once per solve).
 template <typename T> hipError_t launch_rb_res_coef(const RbGeo<T> &g, const ResCoef<T> &cf, hipStream_t s);
+// PBiCGSTAB!'s second half-iteration as one resident launch (see
+// k_rb_resident, MODE 4) and its finalize (FIN_BICG_RES_RHO): alpha from the
+// partials of (q, r0) the first launch left (launched with fin < 0),
+// s = r - alpha q into sv, s_ = M⁻¹s, t = A s_, omega, θ += alpha p_ +
+// omega s_, r = s - omega t, residual and next ρ.  Not in the reference-
+// order sum mode.
+template <typename T>
+hipError_t launch_rb_resident_bicg_s(T *s_, T *sv, const T *r, const T *q, T *t, const T *p_, double *th,
+                                     double *xbest, T *rw, const T *r0, const RbGeo<T> &g, const RbRes &pl, KState *st,
+                                     double *hist, const RedBuf &rb, hipStream_t s);
 // CalcAX! fused after the resident preconditioner: ax = A x of its result,
 // s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (when dz), finalized with fin.
 template <typename T>

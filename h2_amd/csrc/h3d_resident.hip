@@ -54,6 +54,10 @@ constexpr unsigned kResSpin = 1u << 16;         // polls before a wait gives up
 // one pass counter per workgroup, 64 bytes apart.
 constexpr int kBarErr = kResBarErrWord, kBarFlags = 64;
 constexpr int kBarFlagStride = 16;
+// the grid barrier's words after the pass counters: one arrival counter
+// per XCD group, the counter of the groups, the generation
+constexpr int kBarGrid = kBarFlags + kBarFlagStride * kResMaxWG;
+constexpr int kBarTop = kBarGrid + 16 * 8, kBarGen = kBarGrid + 16 * 9;
 
 __device__ __forceinline__ unsigned bar_load(const unsigned *p)
 {
@@ -94,6 +98,45 @@ __device__ __forceinline__ ResTile res_tile(const RbRes &pl, int NY, int NZ)
     t.nb[2] = b > 0 ? (int)blockIdx.x - pl.ntj : -1;
     t.nb[3] = b + 1 < pl.ntk ? (int)blockIdx.x + pl.ntj : -1;
     return t;
+}
+
+// Grid-wide barrier (every workgroup of the launch): two levels, one
+// counter per XCD group (workgroups are dealt to the eight XCDs round-
+// robin) and one for the groups; the last arriver of each level resets its
+// counter, and the last of all bumps the generation the others poll.
+// Device-scope stores issued before it are visible to every workgroup's
+// device-scope loads after it.  Bounded like every wait here.
+__device__ __forceinline__ void res_grid_sync(unsigned *bar)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x, grp = blockIdx.x & 7u;
+        const unsigned ng = G / 8u + (grp < (G & 7u) ? 1u : 0u);
+        const unsigned ngroups = G < 8u ? G : 8u;
+        // the generation cannot move before this workgroup arrives
+        const unsigned gen0 = bar_load(bar + kBarGen);
+        if (__hip_atomic_fetch_add(bar + kBarGrid + 16 * grp, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            ng - 1u) {
+            bar_store(bar + kBarGrid + 16 * grp, 0u);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if (__hip_atomic_fetch_add(bar + kBarTop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                ngroups - 1u) {
+                bar_store(bar + kBarTop, 0u);
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __hip_atomic_fetch_add(bar + kBarGen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        unsigned n = 0;
+        while (bar_load(bar + kBarGen) == gen0) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++n & 255u) == 0u && (n >= kResSpin || bar_load(bar + kBarErr) != 0u)) {
+                bar_store(bar + kBarErr, 1u);
+                break;
+            }
+        }
+    }
+    __syncthreads();
 }
 
 // Publish that this workgroup has finished pass count `cnt`: its stores
@@ -283,16 +326,58 @@ __device__ __forceinline__ void res_partials(const dd (&v)[2], dd *red, const Re
     }
 }
 
+// The exactly rounded totals (hi + lo, as the finalize kernel forms them)
+// of NS sums given as the double-double partials of G workgroups, in
+// every thread.  Every thread must call it; G <= 1024.
+template <int NS, bool SC>
+__device__ __forceinline__ void res_total(const double *part, int stride, unsigned G, dd *red, double (&tot)[2])
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    if ((unsigned)tid < G) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const double *w = part + (size_t)stride * tid + 2 * q;
+            v[q] = SC ? dd{sc_load(w), sc_load(w + 1)} : dd{w[0], w[1]};
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const dd w = row_sum(v[q]);
+        if ((lane & 15) == 15) red[q * 68 + wave * 4 + (lane >> 4)] = w;
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const dd z = row_sum(red[q * 68 + lane]);
+            if ((lane & 15) == 15) red[q * 68 + 64 + (lane >> 4)] = z;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const dd *r = red + q * 68 + 64;
+        const dd t = dd_add_tree(dd_add_tree(r[0], r[1]), dd_add_tree(r[2], r[3]));
+        tot[q] = t.hi + t.lo;
+    }
+    __syncthreads();  // red is reused
+}
+
 // MODE: 0 = the preconditioner alone; 1 = and Σ bb·x (CG!'s ρ); 2, 3 =
 // and CalcAX! ax = A x of its result with one or two fused sums,
 // s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (dz_is_bb: dz is the right-hand
-// side bb the launch formed).
+// side bb the launch formed); 4 = PBiCGSTAB!'s second half-iteration
+// (mode FIRST_BICG_S) whole: alpha from the previous launch's partials of
+// (q, r0), s = r - alpha q, s_ = M⁻¹s, t = A s_, omega from (t, t) and
+// (t, s) reduced over the grid inside the launch, BICG2!/Triad! on θ and r
+// and the partials of (r, r) and (r, r0) for the finalize launch.
 template <typename T, int MODE, bool FLUX>
 __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__restrict__ x, T *__restrict__ bb,
                                                              const T *__restrict__ r, const T *__restrict__ q,
                                                              RbGeo<T> g, RbRes pl, ResCoef<T> cf, const KState *st,
                                                              RedBuf rbuf, T *__restrict__ ax, const T *__restrict__ dy,
-                                                             const T *__restrict__ dz, int dz_is_bb)
+                                                             const T *__restrict__ dz, int dz_is_bb, ResUpd<T> up)
 {
     if (halted(st)) return;  // uniform: every workgroup reads the same state
     extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
@@ -334,7 +419,15 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     // slots and the right-hand side (formed as k_rb_first forms it)
     T beta = T(0), omg = T(0), r_alpha = T(0);
     bool first_itr = false;
-    if (mode == FIRST_BICG_P) {
+    double alpha = 0.0;
+    if (MODE == 4) {
+        // FIN_BICG_ALPHA on the previous launch's partials of (q, r0), in
+        // every workgroup: alpha = rho / (q, r0)
+        double tq[2];
+        res_total<1, false>(up.apart, 6, gridDim.x, red, tq);
+        alpha = st->rho / tq[0];
+        r_alpha = (T)(-alpha);
+    } else if (mode == FIRST_BICG_P) {
         first_itr = (st->itr == 1);
         beta = (T)st->beta;
         omg = (T)st->omega;
@@ -492,6 +585,7 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
         res_wait(pl.bar, tl, f0 + 10u);
         res_load_ring(xs0, (const T *)x, tl, RJ, HX, HXY, g.NX, g.NY, g.NZ, 0);
         dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+        T tav[kResSlots][2];  // mode 4: t kept for the update
 #pragma unroll
         for (int u = 0; u < kResSlots; ++u) {
             if (!sl[u].own) continue;
@@ -525,12 +619,13 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
                 } else {
                     av = (rb_offdiag(o, n) - dgl[u][h] * x0) * T(1);
                 }
+                tav[u][h] = av;
                 if (valid[u][h]) {
                     const size_t c = (h ? HS : 0) + sl[u].base;
                     ax[c] = av;
                     const T yv = dy ? dy[c] : av;
                     v[0] = dd_add(v[0], (double)av * (double)yv);
-                    if (MODE == 3) {
+                    if (MODE >= 3) {
                         const T zv = dz_is_bb ? bv[u][h] : dz[c];
                         v[1] = dd_add(v[1], (double)av * (double)zv);
                     }
@@ -538,8 +633,82 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if (MODE == 2) res_partials<1>(v, red, rbuf);
-        else res_partials<2>(v, red, rbuf);
+        if (MODE == 2) {
+            res_partials<1>(v, red, rbuf);
+        } else if (MODE == 3) {
+            res_partials<2>(v, red, rbuf);
+        } else {
+            // omega: the grid's (t, t) and (t, s) in every workgroup
+            const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const dd w = row_sum(v[q]);
+                if ((lane & 15) == 15) red[q * 68 + wave * 4 + (lane >> 4)] = w;
+            }
+            __syncthreads();
+            if (wave == 0) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const dd z = row_sum(red[q * 68 + lane]);
+                    if ((lane & 15) == 15) red[q * 68 + 64 + (lane >> 4)] = z;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const dd *rq = red + q * 68 + 64;
+                    const dd t = dd_add_tree(dd_add_tree(rq[0], rq[1]), dd_add_tree(rq[2], rq[3]));
+                    sc_store(up.wpart + 4 * (size_t)blockIdx.x + 2 * q, t.hi);
+                    sc_store(up.wpart + 4 * (size_t)blockIdx.x + 2 * q + 1, t.lo);
+                }
+            }
+            __syncthreads();
+            res_grid_sync(pl.bar);
+            double tw[2];
+            res_total<2, true>(up.wpart, 4, gridDim.x, red, tw);
+            // FIN_BICG_OMEGA (NonUniform.jl:320-326)
+            const bool breakdown = fabs(tw[0]) < kFloatMin;
+            const double omega = breakdown ? 0.0 : tw[1] / tw[0];
+            // every workgroup has read the state by now: publish the scalars
+            if (tid == 0) {
+                KState *sw = const_cast<KState *>(st);
+                sw->alpha = alpha;
+                if (breakdown) sw->status = ST_BREAKDOWN;
+                else sw->omega = omega;
+            }
+            if (!breakdown) {
+                // BICG2!(θ, p_, s_, alpha, omega); Triad!(r, t, s, -omega);
+                // Σ r², Σ r·r0 (NonUniform.jl:328-332, 289), as
+                // k_rb_bicg_update forms them
+                const T r_omega = (T)(-omega);
+                const bool snap = up.xbest != nullptr && st->improved != 0;
+                const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
+                dd w[2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+                for (int u = 0; u < kResSlots; ++u) {
+                    const unsigned b0 = sl[u].base;
+                    const unsigned k = b0 / HXY, jr = b0 - k * HXY, j = jr / (unsigned)HX, m = jr - j * (unsigned)HX;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (!valid[u][h]) continue;
+                        const size_t c = (h ? HS : 0) + b0;
+                        const size_t cs = (size_t)(2 * m + ((unsigned)h ^ par[u])) + sy * j + sz * k;
+                        const T sh = (h ? xs1 : xs0)[sl[u].e];
+                        const double xo = up.th[cs];
+                        const double xn = xo + (alpha * (double)up.p_[c] + omega * (double)sh);
+                        const T rv = r_omega * tav[u][h] + bv[u][h];
+                        const T r0v = up.r0[c];
+                        w[0] = dd_add(w[0], (double)rv * (double)rv);
+                        w[1] = dd_add(w[1], (double)rv * (double)r0v);
+                        if (snap) up.xbest[cs] = xo;
+                        up.th[cs] = xn;
+                        up.r[c] = rv;
+                    }
+                }
+                res_partials<2>(w, red, rbuf);
+            }
+        }
     }
 }
 
@@ -613,21 +782,22 @@ hipError_t rb_res_prepare()
     if ((e = res_attr_mode<T, 0>()) != hipSuccess) return e;
     if ((e = res_attr_mode<T, 1>()) != hipSuccess) return e;
     if ((e = res_attr_mode<T, 2>()) != hipSuccess) return e;
-    return res_attr_mode<T, 3>();
+    if ((e = res_attr_mode<T, 3>()) != hipSuccess) return e;
+    return res_attr_mode<T, 4>();
 }
 
 template <typename T, int MODE>
 static void res_launch(int mode, T *x, T *bb, const T *r, const T *q, const RbGeo<T> &g, const RbRes &pl,
                        const ResCoef<T> &cf, const KState *st, const RedBuf &rb, T *ax, const T *dy, const T *dz,
-                       int dz_is_bb, hipStream_t s)
+                       int dz_is_bb, hipStream_t s, const ResUpd<T> &up = ResUpd<T>{})
 {
     const dim3 grid(pl.G), block(kResThreads);
     if (g.flux)
         hipLaunchKernelGGL((k_rb_resident<T, MODE, true>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, cf, st, rb,
-                           ax, dy, dz, dz_is_bb);
+                           ax, dy, dz, dz_is_bb, up);
     else
         hipLaunchKernelGGL((k_rb_resident<T, MODE, false>), grid, block, pl.lds, s, mode, x, bb, r, q, g, pl, cf, st,
-                           rb, ax, dy, dz, dz_is_bb);
+                           rb, ax, dy, dz, dz_is_bb, up);
 }
 
 template <typename T>
@@ -655,11 +825,36 @@ hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, con
     if (dot)
         return launch_finalize_sums<T>(1, (unsigned)pl.G, dot->fin, dot->st, dot->hist, *dot->rb, s,
                                        seq_terms<T>(g, bb, x));
-    if (axf)
+    if (axf && axf->fin >= 0)
         return launch_finalize_sums<T>(axf->dz ? 2 : 1, (unsigned)pl.G, axf->fin, axf->st, axf->hist, *axf->rb, s,
                                        seq_terms<T>(g, axf->ax, axf->dy, axf->ax, axf->dz));
     return hipSuccess;
 }
+
+template <typename T>
+hipError_t launch_rb_resident_bicg_s(T *s_, T *sv, const T *r, const T *q, T *t, const T *p_, double *th,
+                                     double *xbest, T *rw, const T *r0, const RbGeo<T> &g, const RbRes &pl, KState *st,
+                                     double *hist, const RedBuf &rb, hipStream_t s)
+{
+    if (pl.G <= 0 || !pl.bar || !pl.wpart || pl.lds > kResLdsMax || pl.G > rb.nblocks_max || rb.seq)
+        return hipErrorInvalidValue;
+    const ResCoef<T> cf{reinterpret_cast<T *>(pl.dgl), reinterpret_cast<T *>(pl.snk), pl.ix};
+    if (!cf.dgl || !cf.ix || (g.flux && !cf.snk)) return hipErrorInvalidValue;
+    ResUpd<T> up;
+    up.th = th;
+    up.xbest = xbest;
+    up.r = rw;
+    up.p_ = p_;
+    up.r0 = r0;
+    up.wpart = pl.wpart;
+    up.apart = rb.partials;
+    res_launch<T, 4>(FIRST_BICG_S, s_, sv, r, q, g, pl, cf, st, rb, t, nullptr, sv, 1, s, up);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_finalize_sums<T>(2, (unsigned)pl.G, FIN_BICG_RES_RHO, st, hist, rb, s,
+                                   seq_terms<T>(g, rw, nullptr, rw, r0));
+}
+
 
 #define RES_INST(T)                                                                                              \
     template RbRes rb_res_plan<T>(const RbGeo<T> &, int);                                                       \
@@ -667,7 +862,10 @@ hipError_t launch_rb_resident(int mode, T *x, T *bb, const T *r, const T *q, con
     template hipError_t launch_rb_res_coef<T>(const RbGeo<T> &, const ResCoef<T> &, hipStream_t);              \
     template hipError_t launch_rb_resident<T>(int, T *, T *, const T *, const T *, const RbGeo<T> &,          \
                                               const RbRes &, const KState *, const DotFin *, hipStream_t,     \
-                                              const ResAx<T> *);
+                                              const ResAx<T> *);                                              \
+    template hipError_t launch_rb_resident_bicg_s<T>(T *, T *, const T *, const T *, T *, const T *, double *,   \
+                                                     double *, T *, const T *, const RbGeo<T> &, const RbRes &,  \
+                                                     KState *, double *, const RedBuf &, hipStream_t);
 
 RES_INST(double)
 RES_INST(float)
