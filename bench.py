@@ -19,11 +19,16 @@ Printed JSON line (one, on stdout):
              fixture of the oracle's sequential run; max_abs_err_K is the
              largest |θ| difference (K, whole field) of the default solve
              (exactly rounded sums) from it
-  roofline   dominant kernel (one red-black GS colour, k_rb_sweep):
-             algorithmic bytes per launch / mean launch time (HIP events
-             around 200 launches on the context stream); traffic = HBM
-             bytes per launch from the committed rocprofv3 FETCH_SIZE and
-             WRITE_SIZE figures (profiles/traffic.json)
+  roofline   dominant kernel: on grids the resident preconditioner serves
+             (240 x 240 x 30) PBiCGSTAB!'s first launch of an iteration,
+             k_rb_resident<double, 2> (BiCG1!, GS(5), CalcAX!, (q, r0)
+             partials; 232 B per interior cell in the reference's
+             per-operation units), else one red-black GS colour
+             (k_rb_sweep, 16 B per cell); algorithmic bytes per launch /
+             mean launch time (HIP events around 200 launches on the
+             context stream); traffic = HBM bytes per launch from the
+             committed rocprofv3 FETCH_SIZE and WRITE_SIZE figures
+             (profiles/traffic.json)
   cpu_baseline  the oracle (OpenMP C restatement of the reference
              algorithm) on a bounded sample of the same workload
   fp32_refined  the same case solved to a true relative residual of 1e-8
@@ -45,6 +50,9 @@ Profiling workloads (no JSON line; run under rocprofv3):
              k_rb_sweep<double>)
   --profile solve [--nx 240|480]   the headline solve, once, for kernel
              traces
+  --profile resident   the modelA 240 x 240 x 30 case, a 3-iteration solve,
+             the copy calibration, then 20 launches of PBiCGSTAB!'s first
+             resident launch (k_rb_resident<double, 2>)
   --profile kernels [--nx 240|480]  the modelA case, a 3-iteration solve,
              then 20 launches each of one full FP64 sweep, FP64 CalcAX!,
              FP64 CalcAX! with its dot, the FP32 sweep and FP32 CalcAX!
@@ -67,6 +75,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_ITER = {"fp64": 560.0, "fp32": 280.0}
 FUSED_BYTES_PER_CELL_ITER = 480.0  # the same op sequence without the two fills
 SWEEP_BYTES_PER_CELL = 32.0  # x, λ, rhs read + x written, FP64, one full red-black sweep
+# the first resident launch of a PBiCGSTAB! iteration in the reference's
+# per-operation units (FP64): BiCG1! 32 + five full sweeps 160 + CalcAX! 24
+# + the (q, r0) dot 16 bytes per interior cell
+RESIDENT_K1_BYTES_PER_CELL = 32.0 + 5 * SWEEP_BYTES_PER_CELL + 24.0 + 16.0
 
 
 def parse():
@@ -84,7 +96,7 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG! solve")
     ap.add_argument("--no-parity", action="store_true", help="skip the reference-order parity solve")
     ap.add_argument("--cpu-iters", type=int, default=200)
-    ap.add_argument("--profile", choices=("sweep", "solve", "transient", "kernels"), default=None,
+    ap.add_argument("--profile", choices=("sweep", "solve", "transient", "kernels", "resident"), default=None,
                     help="run a profiling workload instead of the benchmark")
     ap.add_argument("--solver", choices=("pbicgstab", "cg"), default="pbicgstab")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
@@ -279,7 +291,7 @@ def profile_workload(args):
     import h2_amd
     from h2_amd import _lib as L
     from h2_amd.heat3ds import WorkBuffers, load_model
-    if args.profile in ("sweep", "solve", "kernels"):
+    if args.profile in ("sweep", "solve", "kernels", "resident"):
         nx = args.nx
         nz = 30 * max(1, nx // 240)
         m = h2_amd.build_model(nx, nx, nz)
@@ -290,6 +302,12 @@ def profile_workload(args):
             wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 3, L.SMOOTHER_GS, True)
             for which in (0, 1, 2, 3, 4):
                 print("kernel %d ms %.4f" % (which, wk.time_kernel(which, 20, 10000.0, True)), flush=True)
+        elif args.profile == "resident":
+            wk.calc_rhs(10000.0, True)
+            wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 3, L.SMOOTHER_GS, True)
+            for _ in range(3):
+                wk.copy("pcg_p", "theta")  # calibration: a plain copy of H*8 bytes
+            print("resident first launch ms", wk.time_kernel(9, 20, 10000.0, True), flush=True)
         elif args.profile == "sweep":
             wk.calc_rhs(10000.0, True)
             wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 3, L.SMOOTHER_GS, True)
@@ -402,21 +420,32 @@ def main():
         log("parity: sequential sums %d iterations, %.1f ms, max |dθ| %.3e K"
             % (parity["itr"], parity["ms"], parity["max_abs_err_K"]))
 
-    # dominant kernel: one red-black colour sweep (k_rb_sweep), HIP events on the context stream
-    sweep_ms = wk.time_kernel(0, 200, dt, True)  # both colours
-    half_ms = sweep_ms / 2.0
-    per_launch_bytes = SWEEP_BYTES_PER_CELL / 2.0 * cells
-    achieved = per_launch_bytes / (half_ms * 1e-3) / 1e9
-    traffic = None
+    # dominant kernel, HIP events on the context stream
+    resident = wk.resident_workgroups()
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    tfig = {}
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("k_rb_sweep_%dx%dx%d" % (nx, nx, nz), {}).get("bytes_per_launch")
-    roofline = {"bound": "hbm", "kernel": "k_rb_sweep<double> (one red-black colour, colour-split layout)",
-                "achieved": round(achieved, 1),
+            tfig = json.load(f)
+    if resident > 0:
+        launch_ms = min(wk.time_kernel(9, 200, dt, True) for _ in range(3))
+        per_launch_bytes = RESIDENT_K1_BYTES_PER_CELL * cells
+        name = ("k_rb_resident<double, 2> (PBiCGSTAB! first launch of an iteration: BiCG1!, GS(5) with the "
+                "iterate in LDS, CalcAX!, (q, r0) partials; %d co-resident workgroups)" % resident)
+        traffic = tfig.get("k_rb_resident_k1_%dx%dx%d" % (nx, nx, nz), {}).get("bytes_per_launch")
+        sweep_ms = wk.time_kernel(0, 200, dt, True)  # the per-colour launches, for reference
+    else:
+        sweep_ms = wk.time_kernel(0, 200, dt, True)  # both colours
+        launch_ms = sweep_ms / 2.0
+        per_launch_bytes = SWEEP_BYTES_PER_CELL / 2.0 * cells
+        name = "k_rb_sweep<double> (one red-black colour, colour-split layout)"
+        traffic = tfig.get("k_rb_sweep_%dx%dx%d" % (nx, nx, nz), {}).get("bytes_per_launch")
+    achieved = per_launch_bytes / (launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "launch_ms": round(half_ms, 5),
-                "algorithmic_bytes_per_launch": per_launch_bytes}
+                "traffic": traffic, "launch_ms": round(launch_ms, 5),
+                "algorithmic_bytes_per_launch": per_launch_bytes,
+                "per_colour_sweep_launch_ms": round(sweep_ms / 2.0, 5)}
 
     refined = None
     if not args.no_refined:

@@ -1580,7 +1580,7 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
 {
     if (!c) return H3D_ERR_ARG;
     if (!ms || reps < 1) return fail(c, H3D_ERR_ARG, "bad arguments");
-    if (which < 0 || which > 8) return fail(c, H3D_ERR_ARG, "unknown kernel id");
+    if (which < 0 || which > 10) return fail(c, H3D_ERR_ARG, "unknown kernel id");
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
@@ -1592,9 +1592,10 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
     double *x = w[1], *b = w[0], *q = w[4], *r0 = w[3];
     Path<float> P32;
     float *w32[8] = {};
-    const bool fp32 = which == 3 || which == 4 || which == 6 || which == 8;
+    const bool fp32 = which == 3 || which == 4 || which == 6 || which == 8 || which == 10;
     if (which == 5 && P.chunk < 1) return fail(c, H3D_ERR_ARG, "no two-colour sweep for this operator");
-    if (which == 7 && P.res.G < 1) return fail(c, H3D_ERR_ARG, "no resident preconditioner for this grid");
+    if ((which == 7 || which == 9) && P.res.G < 1)
+        return fail(c, H3D_ERR_ARG, "no resident preconditioner for this grid");
     if (fp32) {
         Geo<float> g32{};
         if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;
@@ -1603,7 +1604,8 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
         HIPCHK(c, launch_convert<float, double>(w32[1], x, P.n, c->stream));
         HIPCHK(c, launch_convert<float, double>(w32[0], b, P.n, c->stream));
         if (which == 6 && P32.chunk < 1) return fail(c, H3D_ERR_ARG, "no two-colour sweep for this operator");
-        if (which == 8 && P32.res.G < 1) return fail(c, H3D_ERR_ARG, "no resident preconditioner for this grid");
+        if ((which == 8 || which == 10) && P32.res.G < 1)
+            return fail(c, H3D_ERR_ARG, "no resident preconditioner for this grid");
     }
     auto once = [&]() -> hipError_t {
         // one full sweep as one two-colour launch, x -> swap (the result is
@@ -1616,6 +1618,18 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
         if (which == 8)
             return launch_rb_resident<float>(FIRST_COPY, w32[1], w32[0], nullptr, nullptr, P32.rg, P32.res,
                                              c->d_unit, nullptr, c->stream);
+        // PBiCGSTAB!'s first launch of an iteration: BiCG1!, the
+        // preconditioner, CalcAX! and the partials of (q, r0)
+        if (which == 9) {
+            const ResAx<double> a{q, r0, nullptr, -1, c->d_unit, nullptr, &c->rb};
+            return launch_rb_resident<double>(FIRST_BICG_P, x, b, w[2], q, P.rg, P.res, c->d_unit, nullptr,
+                                              c->stream, &a);
+        }
+        if (which == 10) {
+            const ResAx<float> a{w32[4], w32[3], nullptr, -1, c->d_unit, nullptr, &c->rb};
+            return launch_rb_resident<float>(FIRST_BICG_P, w32[1], w32[0], w32[2], w32[4], P32.rg, P32.res,
+                                             c->d_unit, nullptr, c->stream, &a);
+        }
         if (which == 6)
             return launch_rb_sweep2<float>(w32[1], P32.swap, w32[0], P32.rg, 1, P32.chunk, c->d_unit, c->stream);
         if (which == 3) {

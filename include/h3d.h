@@ -300,7 +300,9 @@ int h3d_sor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, 
  * 5 and 6 = one full sweep as one two-colour launch, FP64 and FP32
  * (H3D_OPT_SWEEP_CHUNK; H3D_ERR_ARG when the operator has none); 7 and 8 =
  * the whole preconditioner (five sweeps from zero) as one resident launch,
- * FP64 and FP32 (H3D_OPT_RESIDENT; H3D_ERR_ARG when the grid does not fit). */
+ * FP64 and FP32 (H3D_OPT_RESIDENT; H3D_ERR_ARG when the grid does not fit);
+ * 9 and 10 = PBiCGSTAB!'s first resident launch of an iteration (BiCG1!,
+ * the preconditioner, CalcAX! and its (q, r0) partials), FP64 and FP32. */
 int h3d_time_kernel(h3d_ctx *ctx, int which, int reps, double dt, int is_steady, double *ms);
 
 /* Fdot1/Fdot2 on caller-owned device arrays without a context and without
