@@ -20,13 +20,14 @@ Printed JSON line (one, on stdout):
              largest |θ| difference (K, whole field) of the default solve
              (exactly rounded sums) from it
   roofline   dominant kernel: on grids the resident preconditioner serves
-             (240 x 240 x 30) PBiCGSTAB!'s first launch of an iteration,
-             k_rb_resident<double, 2> (BiCG1!, GS(5), CalcAX!, (q, r0)
-             partials; 232 B per interior cell in the reference's
-             per-operation units), else one red-black GS colour
-             (k_rb_sweep, 16 B per cell); algorithmic bytes per launch /
-             mean launch time (HIP events around 200 launches on the
-             context stream); traffic = HBM bytes per launch from the
+             (240 x 240 x 30) the one-launch PBiCGSTAB! loop,
+             k_rb_bicg_solve<double> (every iteration of the solve in one
+             launch; 544 B per interior cell and iteration in the
+             reference's per-operation units), timed as one launch of 200
+             iterations (HIP events on the context stream, h3d_time_kernel
+             id 11), per iteration; else one red-black GS colour
+             (k_rb_sweep, 16 B per cell, HIP events around 200 launches);
+             traffic = HBM bytes per iteration (per launch) from the
              committed rocprofv3 FETCH_SIZE and WRITE_SIZE figures
              (profiles/traffic.json)
   cpu_baseline  the oracle (OpenMP C restatement of the reference
@@ -51,8 +52,10 @@ Profiling workloads (no JSON line; run under rocprofv3):
   --profile solve [--nx 240|480]   the headline solve, once, for kernel
              traces
   --profile resident   the modelA 240 x 240 x 30 case, a 3-iteration solve,
-             the copy calibration, then 20 launches of PBiCGSTAB!'s first
-             resident launch (k_rb_resident<double, 2>)
+             the copy calibration, 20 launches of PBiCGSTAB!'s first
+             resident launch (k_rb_resident<double, 2>, three-launch
+             iteration), then the one-launch loop (k_rb_bicg_solve<double>)
+             for 2 and for 200 iterations
   --profile kernels [--nx 240|480]  the modelA case, a 3-iteration solve,
              then 20 launches each of one full FP64 sweep, FP64 CalcAX!,
              FP64 CalcAX! with its dot, the FP32 sweep and FP32 CalcAX!
@@ -79,6 +82,11 @@ SWEEP_BYTES_PER_CELL = 32.0  # x, λ, rhs read + x written, FP64, one full red-b
 # per-operation units (FP64): BiCG1! 32 + five full sweeps 160 + CalcAX! 24
 # + the (q, r0) dot 16 bytes per interior cell
 RESIDENT_K1_BYTES_PER_CELL = 32.0 + 5 * SWEEP_BYTES_PER_CELL + 24.0 + 16.0
+# the rest of the iteration: Triad! 24 + five sweeps 160 + CalcAX! 24 + the
+# (t, t) and (t, s) dots 8 + 16 + BICG2! 32 + Triad! 24 + the (r, r) and
+# (r, r0) dots 8 + 16; one whole iteration = 544 bytes per interior cell
+RESIDENT_K2_BYTES_PER_CELL = 24.0 + 5 * SWEEP_BYTES_PER_CELL + 24.0 + 8.0 + 16.0 + 32.0 + 24.0 + 8.0 + 16.0
+RESIDENT_ITER_BYTES_PER_CELL = RESIDENT_K1_BYTES_PER_CELL + RESIDENT_K2_BYTES_PER_CELL
 
 
 def parse():
@@ -308,6 +316,7 @@ def profile_workload(args):
             for _ in range(3):
                 wk.copy("pcg_p", "theta")  # calibration: a plain copy of H*8 bytes
             print("resident first launch ms", wk.time_kernel(9, 20, 10000.0, True), flush=True)
+            print("one-launch loop ms per iteration", wk.time_kernel(11, 200, 10000.0, True), flush=True)
         elif args.profile == "sweep":
             wk.calc_rhs(10000.0, True)
             wk.solve(L.SOLVER_PBICGSTAB, 10000.0, 1e-30, 3, L.SMOOTHER_GS, True)
@@ -427,13 +436,17 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tfig = json.load(f)
+    extra = {}
     if resident > 0:
-        launch_ms = min(wk.time_kernel(9, 200, dt, True) for _ in range(3))
-        per_launch_bytes = RESIDENT_K1_BYTES_PER_CELL * cells
-        name = ("k_rb_resident<double, 2> (PBiCGSTAB! first launch of an iteration: BiCG1!, GS(5) with the "
-                "iterate in LDS, CalcAX!, (q, r0) partials; %d co-resident workgroups)" % resident)
-        traffic = tfig.get("k_rb_resident_k1_%dx%dx%d" % (nx, nx, nz), {}).get("bytes_per_launch")
+        # ms per iteration of one launch running 200 iterations
+        launch_ms = min(wk.time_kernel(11, 200, dt, True) for _ in range(3))
+        per_launch_bytes = RESIDENT_ITER_BYTES_PER_CELL * cells
+        name = ("k_rb_bicg_solve<double> (PBiCGSTAB!'s whole loop in one launch: per iteration BiCG1!, GS(5) "
+                "with the iterate in LDS, CalcAX!, Triad!, GS(5), CalcAX!, BICG2!, Triad!, five dots through "
+                "in-launch grid barriers; %d co-resident workgroups; per iteration)" % resident)
+        traffic = tfig.get("k_rb_bicg_solve_%dx%dx%d" % (nx, nx, nz), {}).get("bytes_per_iteration")
         sweep_ms = wk.time_kernel(0, 200, dt, True)  # the per-colour launches, for reference
+        extra["resident_k1_launch_ms"] = round(min(wk.time_kernel(9, 200, dt, True) for _ in range(3)), 5)
     else:
         sweep_ms = wk.time_kernel(0, 200, dt, True)  # both colours
         launch_ms = sweep_ms / 2.0
@@ -446,6 +459,7 @@ def main():
                 "traffic": traffic, "launch_ms": round(launch_ms, 5),
                 "algorithmic_bytes_per_launch": per_launch_bytes,
                 "per_colour_sweep_launch_ms": round(sweep_ms / 2.0, 5)}
+    roofline.update(extra)
 
     refined = None
     if not args.no_refined:

@@ -71,7 +71,7 @@ struct h3d_ctx {
     int layout = H3D_LAYOUT_AUTO;
     int op_form = H3D_FORM_REFERENCE;     // H3D_OPT_OPERATOR_FORM of the FP64 operator
     int sweep_chunk = -1;                 // H3D_OPT_SWEEP_CHUNK
-    int resident = 1;                     // H3D_OPT_RESIDENT
+    int resident = 2;                     // H3D_OPT_RESIDENT
     int res_wgs = 0;                      // workgroups of the last solve's resident preconditioner (0: none)
     int ncu = 0;                          // compute units of the device
     unsigned *d_resbar[2] = {};           // resident preconditioner: synchronisation words (FP64, FP32)
@@ -580,6 +580,19 @@ int iteration_graph(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P,
     return H3D_OK;
 }
 
+template <typename T>
+int res_finish(h3d_ctx *c, const Path<T> &P, int smoother);
+int res_check(h3d_ctx *c, const RbRes &pl);
+
+// PBiCGSTAB! runs every iteration of a solve in one resident launch
+// (H3D_OPT_RESIDENT 2) when the resident preconditioner applies and the
+// grid sums are the exactly rounded ones.
+template <typename T>
+bool one_launch_bicg(const h3d_ctx *c, const Path<T> &P, int smoother)
+{
+    return c->resident == 2 && P.rb && P.res.G > 0 && smoother == H3D_SMOOTHER_GS && !c->rb.seq;
+}
+
 // Run a Krylov solve whose vectors are in v, starting from the current θ.
 template <typename T>
 int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double tol, int maxItr, int smoother,
@@ -603,6 +616,17 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
         HIPCHK(c, p_precond_from_zero<T>(P, FIRST_COPY, v.s, v.r, nullptr, nullptr, smoother, c->d_state, s));
         HIPCHK(c, p_copy_interior<T>(P, v.p, v.s, c->d_state, s));
         HIPCHK(c, p_dot<T>(P, v.r, v.s, nullptr, FIN_CG_RHO0, c->d_state, nullptr, c->rb, s));
+    }
+    if (bicg && one_launch_bicg<T>(c, P, smoother)) {
+        // every iteration in one launch of the resident grid
+        if (maxItr >= 1) {
+            const ResBicg<T> rv{v.p, v.p_, v.r, v.q, v.s_, v.t, v.r0, v.x, v.xbest};
+            HIPCHK(c, launch_rb_bicg_solve<T>(rv, P.rg, P.res, c->d_state, c->d_hist, s));
+        }
+        HIPCHK(c, hipMemcpyAsync(&c->h_state[1], c->d_state, sizeof(KState), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        *fin_state = c->h_state[1];
+        return res_finish(c, P, smoother);
     }
     hipGraphExec_t exec = nullptr;
     if ((rc = iteration_graph<T>(c, solver, v, P, smoother, &exec)) != H3D_OK) return rc;
@@ -632,19 +656,30 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
     }
     HIPCHK(c, hipStreamSynchronize(s));
     *fin_state = c->h_state[1 + ((nbatch - 1) & 1)];
-    c->res_wgs = smoother == H3D_SMOOTHER_GS ? P.res.G : 0;
-    if (P.res.G > 0 && smoother == H3D_SMOOTHER_GS) {
-        // a timed-out wait of the resident preconditioner leaves its error
-        // word set (and the iterates wrong): report it, reset the words
-        unsigned err = 0;
-        HIPCHK(c, hipMemcpy(&err, P.res.bar + kResBarErrWord, sizeof(err), hipMemcpyDeviceToHost));
-        if (err != 0) {
-            HIPCHK(c, hipMemset(P.res.bar, 0, kResBarWords * sizeof(unsigned)));
-            return fail(c, H3D_ERR_HIP, "resident preconditioner: a wait for a neighbouring tile timed out "
-                                        "(workgroups not co-resident); set H3D_OPT_RESIDENT to 0");
-        }
+    return res_finish(c, P, smoother);
+}
+
+// A timed-out wait of the resident launches leaves their error word set
+// (and the iterates wrong): report it, reset the words.
+int res_check(h3d_ctx *c, const RbRes &pl)
+{
+    unsigned err = 0;
+    HIPCHK(c, hipMemcpy(&err, pl.bar + kResBarErrWord, sizeof(err), hipMemcpyDeviceToHost));
+    if (err != 0) {
+        HIPCHK(c, hipMemset(pl.bar, 0, kResBarWords * sizeof(unsigned)));
+        return fail(c, H3D_ERR_HIP, "resident preconditioner: a wait for a neighbouring tile timed out "
+                                    "(workgroups not co-resident); set H3D_OPT_RESIDENT to 0");
     }
     return H3D_OK;
+}
+
+// After a solve: record whether the resident preconditioner ran, and check
+// its error word.
+template <typename T>
+int res_finish(h3d_ctx *c, const Path<T> &P, int smoother)
+{
+    c->res_wgs = smoother == H3D_SMOOTHER_GS ? P.res.G : 0;
+    return P.res.G > 0 && smoother == H3D_SMOOTHER_GS ? res_check(c, P.res) : H3D_OK;
 }
 
 // Colour-split set-up for the current λ classes and grid: the class field
@@ -797,7 +832,7 @@ int make_path(h3d_ctx *c, const Geo<T> &g, Path<T> *P)
                 pl.dgl = cf[0];
                 pl.snk = cf[1];
                 pl.ix = static_cast<unsigned *>(cf[2]);
-                if (!c->d_respart[pi]) HIPCHK(c, hipMalloc(&c->d_respart[pi], 4 * kResMaxWG * sizeof(double)));
+                if (!c->d_respart[pi]) HIPCHK(c, hipMalloc(&c->d_respart[pi], 12 * kResMaxWG * sizeof(double)));
                 pl.wpart = c->d_respart[pi];
                 const ResCoef<T> rc{static_cast<T *>(cf[0]), P->rg.flux ? static_cast<T *>(cf[1]) : nullptr, pl.ix};
                 HIPCHK(c, launch_rb_res_coef<T>(P->rg, rc, c->stream));
@@ -1240,7 +1275,7 @@ int h3d_set_option(h3d_ctx *c, int option, int value)
         return H3D_OK;
     }
     if (option == H3D_OPT_RESIDENT) {
-        if (value != 0 && value != 1) return fail(c, H3D_ERR_ARG, "bad resident setting");
+        if (value < 0 || value > 2) return fail(c, H3D_ERR_ARG, "bad resident setting");
         c->resident = value;
         c->grid_version++;  // captured iterations carry the preconditioner's launches
         return H3D_OK;
@@ -1580,7 +1615,7 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
 {
     if (!c) return H3D_ERR_ARG;
     if (!ms || reps < 1) return fail(c, H3D_ERR_ARG, "bad arguments");
-    if (which < 0 || which > 10) return fail(c, H3D_ERR_ARG, "unknown kernel id");
+    if (which < 0 || which > 11) return fail(c, H3D_ERR_ARG, "unknown kernel id");
     Geo<double> g;
     int rc = make_geo64(c, dt, is_steady, &g);
     if (rc != H3D_OK) return rc;
@@ -1594,8 +1629,44 @@ int h3d_time_kernel(h3d_ctx *c, int which, int reps, double dt, int is_steady, d
     float *w32[8] = {};
     const bool fp32 = which == 3 || which == 4 || which == 6 || which == 8 || which == 10;
     if (which == 5 && P.chunk < 1) return fail(c, H3D_ERR_ARG, "no two-colour sweep for this operator");
-    if ((which == 7 || which == 9) && P.res.G < 1)
+    if ((which == 7 || which == 9 || which == 11) && P.res.G < 1)
         return fail(c, H3D_ERR_ARG, "no resident preconditioner for this grid");
+    if (which == 11) {
+        // PBiCGSTAB!'s one-launch loop for `reps` iterations (tolerance 0)
+        // from CalcRK! at θ = 0 on the right-hand side b, with a scratch θ
+        // (the unused s buffer) so that the solution field is untouched;
+        // ms per iteration run (fewer than reps if the solve breaks down)
+        if ((rc = ensure_hist(c, reps)) != H3D_OK) return rc;
+        double *th = w[5];
+        const ResBicg<double> rv{w[0], w[1], w[2], w[4], w[6], w[7], w[3], th, nullptr};
+        auto run = [&](int n, float *t) -> int {
+            int r2;
+            if ((r2 = reset_state(c, 0.0, n)) != H3D_OK) return r2;
+            HIPCHK(c, launch_fill<double>(th, 0.0, P.n, c->stream));
+            for (int a = 0; a < 8; ++a)
+                if (a != 5) HIPCHK(c, launch_fill<double>(w[a], 0.0, P.n, c->stream));
+            HIPCHK(c, p_calc_rk<double>(P, w[2], w[3], th, c->f[H3D_F_B], FIN_RES0_BICG, c->d_state, nullptr, c->rb,
+                                        c->stream));
+            HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            HIPCHK(c, launch_rb_bicg_solve<double>(rv, P.rg, P.res, c->d_state, c->d_hist, c->stream));
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev1));
+            HIPCHK(c, hipEventElapsedTime(t, c->ev0, c->ev1));
+            HIPCHK(c, hipMemcpy(&c->h_state[0], c->d_state, sizeof(KState), hipMemcpyDeviceToHost));
+            return H3D_OK;
+        };
+        float t = 0.f;
+        if ((rc = run(std::min(reps, 2), &t)) != H3D_OK) return rc;  // warm-up
+        if ((rc = run(reps, &t)) != H3D_OK) return rc;
+        const int done = c->h_state[0].itr;
+        if (done < 1)
+            return fail(c, H3D_ERR_ARG, "the timed solve stopped before its first iteration (status " +
+                                            std::to_string(c->h_state[0].status) +
+                                            ", res0 " + std::to_string(c->h_state[0].res0) + ")");
+        if ((rc = res_check(c, P.res)) != H3D_OK) return rc;
+        *ms = (double)t / done;
+        return H3D_OK;
+    }
     if (fp32) {
         Geo<float> g32{};
         if ((rc = make_geo32(c, is_steady, &g32)) != H3D_OK) return rc;

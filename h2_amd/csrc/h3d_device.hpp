@@ -180,4 +180,153 @@ __device__ __forceinline__ T diag(const Cond<T> &o)
 
 
 
+// ---------------------------------------------------------------------------
+// scalar steps (one thread, after the sums are known)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void fin_res0(double s0, KState *st)  // CalcRK! result; NonUniform.jl:263-272
+{
+    const double r0 = sqrt(s0);
+    st->res0 = r0;
+    st->itr = 0;
+    if (r0 == 0.0) st->status = ST_CONVERGED;
+    else if (!isfinite(r0)) st->status = ST_NAN;
+}
+
+__device__ __forceinline__ void fin_bicg_rho(double rho, KState *st)  // NonUniform.jl:287-300
+{
+    st->itr += 1;
+    st->rho = rho;
+    if (fabs(rho) < kFloatMin) {
+        st->status = ST_BREAKDOWN;
+        return;
+    }
+    if (st->itr > 1) st->beta = st->rho / st->rho_old * st->alpha / st->omega;
+}
+
+// Divergence guard of a guarded solve (no reference counterpart): returns
+// false after stopping the solve.
+__device__ __forceinline__ bool guard_step(double res, KState *st)
+{
+    st->improved = 0;
+    if (!isfinite(res) || res > kDivergeFactor * st->res_min || st->since_best >= kStallIters) {
+        st->status = ST_DIVERGED;
+        return false;
+    }
+    if (res < st->res_min) {
+        st->res_min = res;
+        st->improved = 1;
+        st->since_best = 0;
+    } else {
+        st->since_best += 1;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void fin_bicg_res(double s0, KState *st, double *hist)  // NonUniform.jl:331-346
+{
+    double res = sqrt(s0);
+    res /= st->res0;
+    st->res = res;
+    if (hist) hist[st->itr - 1] = res;
+    if (st->guard && !guard_step(res, st)) return;
+    if (!isfinite(res)) st->status = ST_NAN;
+    else if (res < st->tol) st->status = ST_CONVERGED;
+    else {
+        st->rho_old = st->rho;
+        if (st->itr >= st->maxItr) st->status = ST_MAXITER;
+    }
+}
+
+__device__ __forceinline__ void finalize(int fin, const double *s, KState *st, double *hist)
+{
+    switch (fin) {
+    case FIN_STORE:
+        st->out[0] = s[0];
+        st->out[1] = s[1];
+        st->out[2] = s[2];
+        break;
+    case FIN_RES0:
+        fin_res0(s[0], st);
+        break;
+    case FIN_RES0_BICG:  // res0, then ρ of iteration 1 = (r, r0) with r0 == r: the same sum
+        fin_res0(s[0], st);
+        if (st->status == ST_RUNNING) fin_bicg_rho(s[0], st);
+        break;
+    case FIN_BICG_RHO:
+        fin_bicg_rho(s[0], st);
+        break;
+    case FIN_BICG_ALPHA:  // NonUniform.jl:309
+        st->alpha = st->rho / s[0];
+        break;
+    case FIN_BICG_OMEGA:  // NonUniform.jl:320-326
+        if (fabs(s[0]) < kFloatMin) {
+            st->status = ST_BREAKDOWN;
+            break;
+        }
+        st->omega = s[1] / s[0];
+        break;
+    case FIN_BICG_RES:
+        fin_bicg_res(s[0], st, hist);
+        break;
+    case FIN_BICG_RES_RHO:  // this iteration's residual, then the next iteration's ρ
+        fin_bicg_res(s[0], st, hist);
+        if (st->status == ST_RUNNING) fin_bicg_rho(s[1], st);
+        break;
+    case FIN_CG_RHO0:  // NonUniform.jl:155
+        st->rho_old = s[0];
+        break;
+    case FIN_CG_ALPHA:  // NonUniform.jl:160-176
+        st->itr += 1;
+        if (fabs(s[0]) < kFloatMin) {
+            st->status = ST_BREAKDOWN;
+            break;
+        }
+        st->alpha = st->rho_old / s[0];
+        break;
+    case FIN_CG_RES: {  // NonUniform.jl:184-197
+        double res = sqrt(s[0]);
+        res /= st->res0;
+        st->res = res;
+        if (hist) hist[st->itr - 1] = res;
+        if (st->guard && !guard_step(res, st)) break;
+        if (!isfinite(res)) st->status = ST_NAN;
+        else if (res < st->tol) st->status = ST_CONVERGED;
+        break;
+    }
+    case FIN_SOR_RES0: {  // solveSOR!, NonUniform.jl:1035-1038
+        double r0 = sqrt(s[0]);
+        if (r0 == 0.0) r0 = 1.0;
+        st->res0 = r0;
+        st->itr = 0;
+        if (!isfinite(r0)) st->status = ST_NAN;
+        break;
+    }
+    case FIN_SOR_RES: {  // rbsor!'s sqrt((0 + res_c0) + res_c1) / res0, NonUniform.jl:950-956, 1042-1049
+        st->itr += 1;
+        double res = sqrt(st->out[0] + s[0]);
+        res /= st->res0;
+        st->res = res;
+        if (hist) hist[st->itr - 1] = res;
+        if (!isfinite(res)) st->status = ST_NAN;
+        else if (res < st->tol) st->status = ST_CONVERGED;
+        else if (st->itr >= st->maxItr) st->status = ST_MAXITER;
+        break;
+    }
+    case FIN_CG_BETA: {  // NonUniform.jl:205-222
+        const double rho_new = s[0];
+        if (fabs(rho_new) < kFloatMin) {
+            st->status = ST_BREAKDOWN;
+            break;
+        }
+        st->beta = rho_new / st->rho_old;
+        st->rho_old = rho_new;
+        if (st->itr >= st->maxItr) st->status = ST_MAXITER;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+
 }  // namespace h3d

@@ -176,7 +176,7 @@ struct RbRes {
     // three index words per cell (ix + 2*HS*w, w = 0..2)
     void *dgl = nullptr, *snk = nullptr;
     unsigned *ix = nullptr;
-    double *wpart = nullptr;    // partials of sums reduced inside a launch (4 per workgroup)
+    double *wpart = nullptr;    // partials of sums reduced inside a launch (3 x 4 per workgroup)
 };
 template <typename T>
 struct ResCoef {
@@ -191,6 +191,15 @@ struct ResUpd {
     const T *p_ = nullptr, *r0 = nullptr;
     double *wpart = nullptr;                 // the launch's own partials of (t, t), (t, s)
     const double *apart = nullptr;           // the first launch's partials of (q, r0)
+};
+// The vectors of the one-launch PBiCGSTAB! loop (launch_rb_bicg_solve):
+// the Krylov vectors in the colour-split layout, θ (and the guarded solve's
+// best iterate) in the standard one.
+template <typename T>
+struct ResBicg {
+    T *p, *p_, *r, *q, *s_, *t;
+    const T *r0;
+    double *th, *xbest;
 };
 constexpr size_t kResLdsMax = 150 * 1024;
 constexpr int kResMaxWG = 1024;
@@ -439,6 +448,14 @@ template <typename T>
 hipError_t launch_rb_resident_bicg_s(T *s_, T *sv, const T *r, const T *q, T *t, const T *p_, double *th,
                                      double *xbest, T *rw, const T *r0, const RbGeo<T> &g, const RbRes &pl, KState *st,
                                      double *hist, const RedBuf &rb, hipStream_t s);
+// Every PBiCGSTAB! iteration of a solve (after CalcRK! and the first ρ,
+// FIN_RES0_BICG) in one launch of the resident grid, until the state leaves
+// ST_RUNNING (converged, breakdown, NaN, maxItr, guard); the state and the
+// residual history are written as the per-iteration launches write them.
+// Not in the reference-order sum mode.
+template <typename T>
+hipError_t launch_rb_bicg_solve(const ResBicg<T> &v, const RbGeo<T> &g, const RbRes &pl, KState *st, double *hist,
+                                hipStream_t s);
 // CalcAX! fused after the resident preconditioner: ax = A x of its result,
 // s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (when dz), finalized with fin.
 template <typename T>

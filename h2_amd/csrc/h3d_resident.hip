@@ -78,6 +78,14 @@ __device__ __forceinline__ T sc_load(const T *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// x, opaque to the compiler: an address formed from it is formed where it
+// is used, not hoisted out of the loops around the use into registers.
+__device__ __forceinline__ unsigned opq(unsigned x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // The tile of workgroup bid = a + ntj*b: first row (j0, k0), extents
 // tj x tk, and its up to four neighbours in (j, k) (-1: none).
 struct ResTile {
@@ -155,7 +163,9 @@ __device__ __forceinline__ void res_wait(unsigned *bar, const ResTile &tl, unsig
 {
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        const int nb = lane < 4 ? tl.nb[lane] : -1;
+        // selects instead of an indexed read, which would put the tile in
+        // scratch memory
+        const int nb = lane == 0 ? tl.nb[0] : lane == 1 ? tl.nb[1] : lane == 2 ? tl.nb[2] : lane == 3 ? tl.nb[3] : -1;
         const unsigned *f = bar + kBarFlags + kBarFlagStride * (nb < 0 ? 0 : nb);
         unsigned n = 0;
         while (true) {
@@ -182,7 +192,7 @@ __device__ __forceinline__ void res_load_ring(T *xo, const T *xg, const ResTile 
                                               int NX, int NY, int NZ, int h)
 {
     const unsigned nring = (unsigned)(2 * tl.tj + 2 * tl.tk) * (unsigned)HX;
-    for (unsigned e = threadIdx.x; e < nring; e += kResThreads) {
+    for (unsigned e = opq(threadIdx.x); e < nring; e += kResThreads) {
         const unsigned hr = e / (unsigned)HX, m = e - hr * (unsigned)HX;
         int jj, kk;
         if (hr < (unsigned)tl.tj) {
@@ -364,6 +374,302 @@ __device__ __forceinline__ void res_total(const double *part, int stride, unsign
     __syncthreads();  // red is reused
 }
 
+// What a tile's passes use besides its per-slot registers.
+template <typename T>
+struct ResView {
+    T *xs0, *xs1;      // the tile's two colour halves in LDS (rows and ring)
+    const T *tabs;     // staged conductance tables of the tile's planes
+    unsigned *bar;
+    ResTile tl;
+    int RJ, HX, NX, NY, NZ;
+    unsigned HXY, zs;  // zs: the LDS offset of the next plane
+    size_t HS;
+};
+
+// The per-slot registers of a tile's cells (see k_rb_resident).
+template <typename T>
+struct ResRegs {
+    ResSlot sl[kResSlots];
+    unsigned par[kResSlots];  // (j + k) & 1 of the slot
+    bool valid[kResSlots][2];
+    T dgl[kResSlots][2], snk[kResSlots][2];
+    unsigned ix[kResSlots][2][3];
+};
+
+// Preconditioner! :gs, the five red-black sweeps from x = 0 (LDS zeroed
+// by the caller) on the right-hand side bv, into LDS and into x in HBM;
+// publishes pass counts f0+1 .. f0+10.  Every thread must call it.
+template <typename T, bool FLUX>
+__device__ __forceinline__ void res_gs5(const ResView<T> &w, T *x, const ResRegs<T> &rg,
+                                        const T (&bv)[kResSlots][2], unsigned f0)
+{
+    const unsigned HX = (unsigned)w.HX, zs = w.zs;
+#pragma unroll 1
+    for (int sweep = 0; sweep < 5; ++sweep) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            // one colour pass: colour 0 updates half 1, colour 1 half 0
+            const int h = 1 - half;
+            const int pass = 2 * sweep + half;
+            T *xo = h ? w.xs1 : w.xs0;
+            T *xt = h ? w.xs0 : w.xs1;
+            T *xg = x + (h ? w.HS : 0);
+            // the tile's inner rows, then (after the ring rows of the half
+            // the previous pass updated have arrived) its edge rows
+#pragma unroll
+            for (int phase = 0; phase < 2; ++phase) {
+                if (phase == 1 && pass > 0) {
+                    res_wait(w.bar, w.tl, f0 + (unsigned)pass);
+                    res_load_ring(xt, (const T *)(x + (h ? 0 : w.HS)), w.tl, w.RJ, w.HX, w.HXY, w.NX, w.NY, w.NZ,
+                                  1 - h);
+                }
+#pragma unroll
+                for (int u = 0; u < kResSlots; ++u) {
+                    if (!rg.sl[u].own || rg.sl[u].edge != (phase == 1)) continue;
+                    // the offsets and index words are opaque to the
+                    // compiler here, so that it forms the addresses inside
+                    // the loop instead of hoisting some hundred of them
+                    // into registers
+                    unsigned e = rg.sl[u].e, base = rg.sl[u].base, w0 = rg.ix[u][h][0], w1 = rg.ix[u][h][1],
+                             w2 = rg.ix[u][h][2];
+                    asm volatile("" : "+v"(e), "+v"(base), "+v"(w0), "+v"(w1), "+v"(w2));
+                    const unsigned sx = (unsigned)h ^ rg.par[u];
+                    Nb<T> n;
+                    n.xm = xt[e - 1u + sx];
+                    n.xp = xt[e + sx];
+                    n.ym = xt[e - HX];
+                    n.yp = xt[e + HX];
+                    n.zm = xt[e - zs];
+                    n.zp = xt[e + zs];
+                    const T pp = xo[e];
+                    Cond<T> o;
+                    o.xm = w.tabs[w0 & 0xffffu];
+                    o.xp = w.tabs[w0 >> 16];
+                    o.ym = w.tabs[w1 & 0xffffu];
+                    o.yp = w.tabs[w1 >> 16];
+                    o.zm = w.tabs[w2 & 0xffffu];
+                    o.zp = w.tabs[w2 >> 16];
+                    T xn;
+                    if (FLUX) {
+                        const T f = o.xp * (n.xp - pp) + o.xm * (n.xm - pp) + o.yp * (n.yp - pp) +
+                                    o.ym * (n.ym - pp) + o.zp * (n.zp - pp) + o.zm * (n.zm - pp);
+                        xn = pp + ((f - rg.snk[u][h] * pp) - bv[u][h]) / rg.dgl[u][h];
+                    } else {
+                        const T dp = (((rb_offdiag(o, n) - bv[u][h]) / rg.dgl[u][h] - pp)) * T(1);
+                        xn = pp + T(1) * dp;
+                    }
+                    if (rg.valid[u][h]) {
+                        xo[e] = xn;
+                        // HBM receives the rows neighbours read, and in the
+                        // last sweep (final values) every row
+                        if (phase == 1) sc_store(xg + base, xn);
+                        else if (sweep == 4) xg[base] = xn;
+                    }
+                    // one slot's loads in flight at a time: sixteen waves
+                    // per CU hide the LDS latency, and the registers stay
+                    // within 128
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            res_publish(w.bar, f0 + (unsigned)pass + 1u);
+        }
+    }
+}
+
+// CalcAX! (src/NonUniform.jl:454-511) of x, read from LDS once the ring rows
+// of half 0 (updated by the last pass, published as count want) have
+// arrived: ax = A x on the tile's cells, tav = the same values for the
+// caller, v[0] += Σ ax·(dy ? dy : ax), v[1] += Σ ax·(dz_bb ? bv : dz) when
+// NS = 2.  Every thread must call it.
+template <typename T, bool FLUX, int NS>
+__device__ __forceinline__ void res_calc_ax(const ResView<T> &w, const T *x, const ResRegs<T> &rg,
+                                            const T (&bv)[kResSlots][2], unsigned want, T *ax, const T *dy,
+                                            const T *dz, bool dz_bb, dd (&v)[2], T (&tav)[kResSlots][2])
+{
+    const unsigned HX = (unsigned)w.HX, zs = w.zs;
+    res_wait(w.bar, w.tl, want);
+    res_load_ring(w.xs0, x, w.tl, w.RJ, w.HX, w.HXY, w.NX, w.NY, w.NZ, 0);
+#pragma unroll
+    for (int u = 0; u < kResSlots; ++u) {
+        if (!rg.sl[u].own) continue;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            unsigned e = rg.sl[u].e, base = rg.sl[u].base, w0 = rg.ix[u][h][0], w1 = rg.ix[u][h][1],
+                     w2 = rg.ix[u][h][2];
+            asm volatile("" : "+v"(e), "+v"(base), "+v"(w0), "+v"(w1), "+v"(w2));
+            const T *xo = h ? w.xs1 : w.xs0, *xt = h ? w.xs0 : w.xs1;
+            const unsigned sx = (unsigned)h ^ rg.par[u];
+            Nb<T> n;
+            n.xm = xt[e - 1u + sx];
+            n.xp = xt[e + sx];
+            n.ym = xt[e - HX];
+            n.yp = xt[e + HX];
+            n.zm = xt[e - zs];
+            n.zp = xt[e + zs];
+            const T x0 = xo[e];
+            Cond<T> o;
+            o.xm = w.tabs[w0 & 0xffffu];
+            o.xp = w.tabs[w0 >> 16];
+            o.ym = w.tabs[w1 & 0xffffu];
+            o.yp = w.tabs[w1 >> 16];
+            o.zm = w.tabs[w2 & 0xffffu];
+            o.zp = w.tabs[w2 >> 16];
+            // rb_apply with the diagonal (and sink) formed once per solve
+            T av;
+            if (FLUX) {
+                const T f = o.xp * (n.xp - x0) + o.xm * (n.xm - x0) + o.yp * (n.yp - x0) + o.ym * (n.ym - x0) +
+                            o.zp * (n.zp - x0) + o.zm * (n.zm - x0);
+                av = f - rg.snk[u][h] * x0;
+            } else {
+                av = (rb_offdiag(o, n) - rg.dgl[u][h] * x0) * T(1);
+            }
+            tav[u][h] = av;
+            if (rg.valid[u][h]) {
+                const size_t c = (h ? w.HS : 0) + base;
+                ax[c] = av;
+                const T yv = dy ? dy[c] : av;
+                v[0] = dd_add(v[0], (double)av * (double)yv);
+                if (NS >= 2) {
+                    const T zv = dz_bb ? bv[u][h] : dz[c];
+                    v[1] = dd_add(v[1], (double)av * (double)zv);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// The tables of the tile's planes in one round of loads (the entries
+// rb_stage_into forms, per plane, with the same operations), and the
+// tile's slots with their per-cell constants (cf, formed once per solve).
+// Ends without a barrier.
+template <typename T, bool FLUX>
+__device__ __forceinline__ void res_setup(const RbGeo<T> &g, const ResCoef<T> &cf, const ResView<T> &w, int ntab,
+                                          T *tabs, ResRegs<T> &rg)
+{
+    const int tid = threadIdx.x, K = g.K;
+    const ResTile &tl = w.tl;
+    {
+        const int n = 2 * K * K, per = n + K;
+        for (int idx = tid; idx < tl.tk * per; idx += kResThreads) {
+            const int kk = idx / per, a = idx - kk * per;
+            const int k = tl.k0 + kk;
+            T *t = tabs + (size_t)kk * ntab;
+            const T dz_k = g.dZ[k];
+            if (a < n) {
+                const T f = g.tab[a];
+                t[a] = f * g.dy0 * dz_k * g.ddx;
+                t[n + a] = f * g.dx0 * dz_k * g.ddy;
+                t[2 * n + a] = g.ZF[(size_t)(k - 1) * K * K + a];
+            } else if (g.ap) {
+                const int c = a - n;
+                t[3 * n + c] = (T)(g.capc[c] * g.dx0d * g.dy0d * g.dZd[k] * g.ddt);
+            }
+        }
+    }
+    const unsigned HX = (unsigned)w.HX;
+    const unsigned nown = (unsigned)(tl.tj * tl.tk) * HX;
+#pragma unroll
+    for (int u = 0; u < kResSlots; ++u) {
+        const unsigned qi = (unsigned)tid + (unsigned)kResThreads * u;
+        rg.sl[u].own = qi < nown;
+        const unsigned qq = rg.sl[u].own ? qi : 0u;
+        const int row = (int)(qq / HX);
+        const int m = (int)(qq - (unsigned)row * HX);
+        int jj, kk;
+        res_row(row, tl.tj, tl.tk, jj, kk);
+        const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
+        rg.sl[u].edge = row < tl.tj * tl.tk - max(tl.tj - 2, 0) * max(tl.tk - 2, 0);
+        rg.par[u] = (unsigned)((j + k) & 1);
+        rg.sl[u].e = (unsigned)((kk * w.RJ + jj) * w.HX + m);
+        rg.sl[u].base = (unsigned)m + HX * (unsigned)j + w.HXY * (unsigned)k;
+        // the tile's plane offset in the staged tables, in both 16-bit halves
+        const unsigned tb = (unsigned)((kk - 1) * ntab) * 0x10001u;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const size_t oh = (h ? w.HS : 0) + rg.sl[u].base;
+            const int i = 2 * m + (int)((unsigned)h ^ rg.par[u]);
+            rg.valid[u][h] = rg.sl[u].own && i >= 1 && i <= g.NX;
+            // cells outside the interior have no coefficients (never stored)
+            rg.dgl[u][h] = rg.valid[u][h] ? cf.dgl[oh] : T(1);
+            rg.snk[u][h] = FLUX && rg.valid[u][h] ? cf.snk[oh] : T(0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) rg.ix[u][h][q] = rg.valid[u][h] ? cf.ix[2 * q * w.HS + oh] + tb : tb;
+        }
+    }
+}
+
+// Up to two per-thread sums over the workgroup into its device-scope
+// partials wp[4 * blockIdx.x + 2s + {0, 1}] (hi, lo), for a grid
+// reduction inside the launch.  Every thread must call it.
+template <int NS>
+__device__ __forceinline__ void res_wpartials(const dd (&v)[2], dd *red, double *wp)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const dd w = row_sum(v[q]);
+        if ((lane & 15) == 15) red[q * 68 + wave * 4 + (lane >> 4)] = w;
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const dd z = row_sum(red[q * 68 + lane]);
+            if ((lane & 15) == 15) red[q * 68 + 64 + (lane >> 4)] = z;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const dd *rq = red + q * 68 + 64;
+            const dd t = dd_add_tree(dd_add_tree(rq[0], rq[1]), dd_add_tree(rq[2], rq[3]));
+            sc_store(wp + 4 * (size_t)blockIdx.x + 2 * q, t.hi);
+            sc_store(wp + 4 * (size_t)blockIdx.x + 2 * q + 1, t.lo);
+        }
+    }
+    __syncthreads();
+}
+
+// BICG2!(θ, p_, s_, alpha, omega); Triad!(r, t, s, -omega); Σ r², Σ r·r0
+// (src/NonUniform.jl:328-332, 289) on the tile's cells, as k_rb_bicg_update
+// forms them: s_ from LDS, s = bv, t = tav; θ in the standard layout; with
+// xbest the iterate is copied there before it is overwritten.
+template <typename T>
+__device__ __forceinline__ void res_bicg_update(const RbGeo<T> &g, const ResView<T> &w, const ResRegs<T> &rg,
+                                                const T (&bv)[kResSlots][2], const T (&tav)[kResSlots][2],
+                                                double *th, double *xbest, T *r, const T *p_, const T *r0,
+                                                double alpha, double omega, dd (&v)[2])
+{
+    const T r_omega = (T)(-omega);
+    const unsigned HX = (unsigned)w.HX, HXY = w.HXY;
+    const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
+    v[0] = dd{0.0, 0.0};
+    v[1] = dd{0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < kResSlots; ++u) {
+        const unsigned b0 = opq(rg.sl[u].base);
+        const unsigned k = b0 / HXY, jr = b0 - k * HXY, j = jr / HX, m = jr - j * HX;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (!rg.valid[u][h]) continue;
+            const size_t c = (h ? w.HS : 0) + b0;
+            const size_t cs = (size_t)(2 * m + ((unsigned)h ^ rg.par[u])) + sy * j + sz * k;
+            const T sh = (h ? w.xs1 : w.xs0)[opq(rg.sl[u].e)];
+            const double xo = th[cs];
+            const double xn = xo + (alpha * (double)p_[c] + omega * (double)sh);
+            const T rv = r_omega * tav[u][h] + bv[u][h];
+            const T r0v = r0[c];
+            v[0] = dd_add(v[0], (double)rv * (double)rv);
+            v[1] = dd_add(v[1], (double)rv * (double)r0v);
+            if (xbest) xbest[cs] = xo;
+            th[cs] = xn;
+            r[c] = rv;
+        }
+    }
+}
+
 // MODE: 0 = the preconditioner alone; 1 = and Σ bb·x (CG!'s ρ); 2, 3 =
 // and CalcAX! ax = A x of its result with one or two fused sums,
 // s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (dz_is_bb: dz is the right-hand
@@ -383,40 +689,34 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
     const int tid = threadIdx.x;
     const int HX = g.HX, K = g.K;
-    const unsigned HXY = (unsigned)HX * (unsigned)g.MY;
-    const ResTile tl = res_tile(pl, g.NY, g.NZ);
     const int RJ = pl.TJ + 2;
     const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);  // one LDS half (+1 pad)
-    T *xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
-    T *xs1 = xs0 + hsz;
-    T *tabs = xs1 + hsz;
     const int ntab = 6 * K * K + K;
+    ResView<T> w;
+    w.xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
+    w.xs1 = w.xs0 + hsz;
+    w.tabs = w.xs1 + hsz;
+    w.bar = pl.bar;
+    w.tl = res_tile(pl, g.NY, g.NZ);
+    w.RJ = RJ;
+    w.HX = HX;
+    w.NX = g.NX;
+    w.NY = g.NY;
+    w.NZ = g.NZ;
+    w.HXY = (unsigned)HX * (unsigned)g.MY;
+    w.zs = (unsigned)(RJ * HX);
+    w.HS = g.HS;
+    T *tabs = w.xs1 + hsz;
     dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
                                      (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
-
-    // the tables of the tile's planes in one round of loads: the entries
-    // rb_stage_into forms, per plane, with the same operations
-    {
-        const int n = 2 * K * K, per = n + K;
-        for (int idx = tid; idx < tl.tk * per; idx += kResThreads) {
-            const int kk = idx / per, a = idx - kk * per;
-            const int k = tl.k0 + kk;
-            T *t = tabs + (size_t)kk * ntab;
-            const T dz_k = g.dZ[k];
-            if (a < n) {
-                const T f = g.tab[a];
-                t[a] = f * g.dy0 * dz_k * g.ddx;
-                t[n + a] = f * g.dx0 * dz_k * g.ddy;
-                t[2 * n + a] = g.ZF[(size_t)(k - 1) * K * K + a];
-            } else if (g.ap) {
-                const int c = a - n;
-                t[3 * n + c] = (T)(g.capc[c] * g.dx0d * g.dy0d * g.dZd[k] * g.ddt);
-            }
-        }
-    }
     const size_t HS = g.HS;
+    const unsigned HXY = w.HXY;
 
-    // slots and the right-hand side (formed as k_rb_first forms it)
+    // tables, slots and per-cell constants
+    ResRegs<T> rg;
+    res_setup<T, FLUX>(g, cf, w, ntab, tabs, rg);
+
+    // the right-hand side (formed as k_rb_first forms it)
     T beta = T(0), omg = T(0), r_alpha = T(0);
     bool first_itr = false;
     double alpha = 0.0;
@@ -434,138 +734,37 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     } else if (mode == FIRST_BICG_S) {
         r_alpha = (T)(-st->alpha);
     }
-    const unsigned nown = (unsigned)(tl.tj * tl.tk * HX);
-    ResSlot sl[kResSlots];
     T bv[kResSlots][2];
-    unsigned par[kResSlots];  // (j + k) & 1 of the slot
-    // per slot and half: validity, diagonal, sink (flux form) and the LDS
-    // indices of the six conductances, packed two to a word
-    bool valid[kResSlots][2];
-    T dgl[kResSlots][2], snk[kResSlots][2] = {};
-    unsigned ix[kResSlots][2][3];
 #pragma unroll
-    for (int u = 0; u < kResSlots; ++u) {
-        const unsigned qi = (unsigned)tid + (unsigned)kResThreads * u;
-        sl[u].own = qi < nown;
-        const unsigned qq = sl[u].own ? qi : 0u;
-        const int row = (int)(qq / (unsigned)HX);
-        const int m = (int)(qq - (unsigned)row * (unsigned)HX);
-        int jj, kk;
-        res_row(row, tl.tj, tl.tk, jj, kk);
-        const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
-        sl[u].edge = row < tl.tj * tl.tk - max(tl.tj - 2, 0) * max(tl.tk - 2, 0);
-        par[u] = (unsigned)((j + k) & 1);
-        sl[u].e = (unsigned)((kk * RJ + jj) * HX + m);
-        sl[u].base = (unsigned)m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k;
-        const unsigned o = sl[u].base;
-        // the tile's plane offset in the staged tables, in both 16-bit halves
-        const unsigned tb = (unsigned)((kk - 1) * ntab) * 0x10001u;
+    for (int u = 0; u < kResSlots; ++u)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const size_t oh = (h ? HS : 0) + o;
+            const size_t oh = (h ? HS : 0) + rg.sl[u].base;
             T v;
             if (mode == FIRST_BICG_P) v = first_itr ? r[oh] : r[oh] + beta * (bb[oh] - omg * q[oh]);
             else if (mode == FIRST_BICG_S) v = r_alpha * q[oh] + r[oh];
             else v = bb[oh];
             bv[u][h] = v;
-            const int i = 2 * m + (int)((unsigned)h ^ par[u]);
-            valid[u][h] = sl[u].own && i >= 1 && i <= g.NX;
-            // cells outside the interior have no coefficients (never stored)
-            dgl[u][h] = valid[u][h] ? cf.dgl[oh] : T(1);
-            if (FLUX) snk[u][h] = valid[u][h] ? cf.snk[oh] : T(0);
-#pragma unroll
-            for (int w = 0; w < 3; ++w) ix[u][h][w] = valid[u][h] ? cf.ix[2 * w * HS + oh] + tb : tb;
         }
-    }
     __syncthreads();  // tables staged
 
     // pass counter at the start of this launch (equal in every workgroup)
     const unsigned f0 = bar_load(pl.bar + kBarFlags + kBarFlagStride * blockIdx.x);
 
-
-    const unsigned zs = (unsigned)(RJ * HX);
     // x = 0 everywhere in LDS (the preconditioner starts from zero; the
     // domain's halo rows and columns stay zero)
-    for (unsigned e = tid; e < 2 * hsz; e += kResThreads) xs0[e] = T(0);
+    for (unsigned e = tid; e < 2 * hsz; e += kResThreads) w.xs0[e] = T(0);
     // the bb stores of modes that form it (interior cells only)
     if (mode != FIRST_COPY) {
 #pragma unroll
         for (int u = 0; u < kResSlots; ++u)
 #pragma unroll
             for (int h = 0; h < 2; ++h)
-                if (valid[u][h]) bb[(h ? HS : 0) + sl[u].base] = bv[u][h];
+                if (rg.valid[u][h]) bb[(h ? HS : 0) + rg.sl[u].base] = bv[u][h];
     }
     __syncthreads();  // x zeroed
 
-#pragma unroll 1
-    for (int sweep = 0; sweep < 5; ++sweep) {
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            // one colour pass: colour 0 updates half 1, colour 1 half 0
-            const int h = 1 - half;
-            const int pass = 2 * sweep + half;
-            T *xo = h ? xs1 : xs0;
-            T *xt = h ? xs0 : xs1;
-            T *xg = x + (h ? HS : 0);
-            // the tile's inner rows, then (after the ring rows of the half
-            // the previous pass updated have arrived) its edge rows
-#pragma unroll
-            for (int phase = 0; phase < 2; ++phase) {
-                if (phase == 1 && pass > 0) {
-                    res_wait(pl.bar, tl, f0 + (unsigned)pass);
-                    res_load_ring(xt, (const T *)(x + (h ? 0 : HS)), tl, RJ, HX, HXY, g.NX, g.NY, g.NZ, 1 - h);
-                }
-#pragma unroll
-                for (int u = 0; u < kResSlots; ++u) {
-                    if (!sl[u].own || sl[u].edge != (phase == 1)) continue;
-                    // the offsets and index words are opaque to the
-                    // compiler here, so that it forms the addresses inside
-                    // the loop instead of hoisting some hundred of them
-                    // into registers
-                    unsigned e = sl[u].e, base = sl[u].base, w0 = ix[u][h][0], w1 = ix[u][h][1],
-                             w2 = ix[u][h][2];
-                    asm volatile("" : "+v"(e), "+v"(base), "+v"(w0), "+v"(w1), "+v"(w2));
-                    const unsigned sx = (unsigned)h ^ par[u];
-                    Nb<T> n;
-                    n.xm = xt[e - 1u + sx];
-                    n.xp = xt[e + sx];
-                    n.ym = xt[e - (unsigned)HX];
-                    n.yp = xt[e + (unsigned)HX];
-                    n.zm = xt[e - zs];
-                    n.zp = xt[e + zs];
-                    const T pp = xo[e];
-                    Cond<T> o;
-                    o.xm = tabs[w0 & 0xffffu];
-                    o.xp = tabs[w0 >> 16];
-                    o.ym = tabs[w1 & 0xffffu];
-                    o.yp = tabs[w1 >> 16];
-                    o.zm = tabs[w2 & 0xffffu];
-                    o.zp = tabs[w2 >> 16];
-                    T xn;
-                    if (FLUX) {
-                        const T f = o.xp * (n.xp - pp) + o.xm * (n.xm - pp) + o.yp * (n.yp - pp) +
-                                    o.ym * (n.ym - pp) + o.zp * (n.zp - pp) + o.zm * (n.zm - pp);
-                        xn = pp + ((f - snk[u][h] * pp) - bv[u][h]) / dgl[u][h];
-                    } else {
-                        const T dp = (((rb_offdiag(o, n) - bv[u][h]) / dgl[u][h] - pp)) * T(1);
-                        xn = pp + T(1) * dp;
-                    }
-                    if (valid[u][h]) {
-                        xo[e] = xn;
-                        // HBM receives the rows neighbours read, and in the
-                        // last sweep (final values) every row
-                        if (phase == 1) sc_store(xg + base, xn);
-                        else if (sweep == 4) xg[base] = xn;
-                    }
-                    // one slot's loads in flight at a time: sixteen waves
-                    // per CU hide the LDS latency, and the registers stay
-                    // within 128
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-            res_publish(pl.bar, f0 + (unsigned)pass + 1u);
-        }
-    }
+    res_gs5<T, FLUX>(w, x, rg, bv, f0);
 
     if (MODE == 1) {
         // Σ bb·x over the tile's interior cells of both halves (x final:
@@ -574,96 +773,22 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
         dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
         for (int u = 0; u < kResSlots; ++u) {
-            if (valid[u][0]) v[0] = dd_add(v[0], (double)bv[u][0] * (double)xs0[sl[u].e]);
-            if (valid[u][1]) v[0] = dd_add(v[0], (double)bv[u][1] * (double)xs1[sl[u].e]);
+            if (rg.valid[u][0]) v[0] = dd_add(v[0], (double)bv[u][0] * (double)w.xs0[rg.sl[u].e]);
+            if (rg.valid[u][1]) v[0] = dd_add(v[0], (double)bv[u][1] * (double)w.xs1[rg.sl[u].e]);
         }
         res_partials<1>(v, red, rbuf);
     }
     if (MODE >= 2) {
-        // CalcAX! (src/NonUniform.jl:454-511) of the result, read from LDS:
-        // the ring rows of half 0 (updated by the last pass) first
-        res_wait(pl.bar, tl, f0 + 10u);
-        res_load_ring(xs0, (const T *)x, tl, RJ, HX, HXY, g.NX, g.NY, g.NZ, 0);
         dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
         T tav[kResSlots][2];  // mode 4: t kept for the update
-#pragma unroll
-        for (int u = 0; u < kResSlots; ++u) {
-            if (!sl[u].own) continue;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                unsigned e = sl[u].e, w0 = ix[u][h][0], w1 = ix[u][h][1], w2 = ix[u][h][2];
-                asm volatile("" : "+v"(e), "+v"(w0), "+v"(w1), "+v"(w2));
-                const T *xo = h ? xs1 : xs0, *xt = h ? xs0 : xs1;
-                const unsigned sx = (unsigned)h ^ par[u];
-                Nb<T> n;
-                n.xm = xt[e - 1u + sx];
-                n.xp = xt[e + sx];
-                n.ym = xt[e - (unsigned)HX];
-                n.yp = xt[e + (unsigned)HX];
-                n.zm = xt[e - zs];
-                n.zp = xt[e + zs];
-                const T x0 = xo[e];
-                Cond<T> o;
-                o.xm = tabs[w0 & 0xffffu];
-                o.xp = tabs[w0 >> 16];
-                o.ym = tabs[w1 & 0xffffu];
-                o.yp = tabs[w1 >> 16];
-                o.zm = tabs[w2 & 0xffffu];
-                o.zp = tabs[w2 >> 16];
-                // rb_apply with the diagonal (and sink) formed once per launch
-                T av;
-                if (FLUX) {
-                    const T f = o.xp * (n.xp - x0) + o.xm * (n.xm - x0) + o.yp * (n.yp - x0) + o.ym * (n.ym - x0) +
-                                o.zp * (n.zp - x0) + o.zm * (n.zm - x0);
-                    av = f - snk[u][h] * x0;
-                } else {
-                    av = (rb_offdiag(o, n) - dgl[u][h] * x0) * T(1);
-                }
-                tav[u][h] = av;
-                if (valid[u][h]) {
-                    const size_t c = (h ? HS : 0) + sl[u].base;
-                    ax[c] = av;
-                    const T yv = dy ? dy[c] : av;
-                    v[0] = dd_add(v[0], (double)av * (double)yv);
-                    if (MODE >= 3) {
-                        const T zv = dz_is_bb ? bv[u][h] : dz[c];
-                        v[1] = dd_add(v[1], (double)av * (double)zv);
-                    }
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-        }
+        res_calc_ax<T, FLUX, (MODE >= 3 ? 2 : 1)>(w, x, rg, bv, f0 + 10u, ax, dy, dz, dz_is_bb != 0, v, tav);
         if (MODE == 2) {
             res_partials<1>(v, red, rbuf);
         } else if (MODE == 3) {
             res_partials<2>(v, red, rbuf);
         } else {
             // omega: the grid's (t, t) and (t, s) in every workgroup
-            const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const dd w = row_sum(v[q]);
-                if ((lane & 15) == 15) red[q * 68 + wave * 4 + (lane >> 4)] = w;
-            }
-            __syncthreads();
-            if (wave == 0) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const dd z = row_sum(red[q * 68 + lane]);
-                    if ((lane & 15) == 15) red[q * 68 + 64 + (lane >> 4)] = z;
-                }
-            }
-            __syncthreads();
-            if (tid == 0) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const dd *rq = red + q * 68 + 64;
-                    const dd t = dd_add_tree(dd_add_tree(rq[0], rq[1]), dd_add_tree(rq[2], rq[3]));
-                    sc_store(up.wpart + 4 * (size_t)blockIdx.x + 2 * q, t.hi);
-                    sc_store(up.wpart + 4 * (size_t)blockIdx.x + 2 * q + 1, t.lo);
-                }
-            }
-            __syncthreads();
+            res_wpartials<2>(v, red, up.wpart);
             res_grid_sync(pl.bar);
             double tw[2];
             res_total<2, true>(up.wpart, 4, gridDim.x, red, tw);
@@ -678,38 +803,155 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
                 else sw->omega = omega;
             }
             if (!breakdown) {
-                // BICG2!(θ, p_, s_, alpha, omega); Triad!(r, t, s, -omega);
-                // Σ r², Σ r·r0 (NonUniform.jl:328-332, 289), as
-                // k_rb_bicg_update forms them
-                const T r_omega = (T)(-omega);
-                const bool snap = up.xbest != nullptr && st->improved != 0;
-                const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
-                dd w[2] = {{0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-                for (int u = 0; u < kResSlots; ++u) {
-                    const unsigned b0 = sl[u].base;
-                    const unsigned k = b0 / HXY, jr = b0 - k * HXY, j = jr / (unsigned)HX, m = jr - j * (unsigned)HX;
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        if (!valid[u][h]) continue;
-                        const size_t c = (h ? HS : 0) + b0;
-                        const size_t cs = (size_t)(2 * m + ((unsigned)h ^ par[u])) + sy * j + sz * k;
-                        const T sh = (h ? xs1 : xs0)[sl[u].e];
-                        const double xo = up.th[cs];
-                        const double xn = xo + (alpha * (double)up.p_[c] + omega * (double)sh);
-                        const T rv = r_omega * tav[u][h] + bv[u][h];
-                        const T r0v = up.r0[c];
-                        w[0] = dd_add(w[0], (double)rv * (double)rv);
-                        w[1] = dd_add(w[1], (double)rv * (double)r0v);
-                        if (snap) up.xbest[cs] = xo;
-                        up.th[cs] = xn;
-                        up.r[c] = rv;
-                    }
-                }
-                res_partials<2>(w, red, rbuf);
+                dd wv[2];
+                res_bicg_update<T>(g, w, rg, bv, tav, up.th, up.xbest != nullptr && st->improved != 0 ? up.xbest
+                                                                                                        : nullptr,
+                                   up.r, up.p_, up.r0, alpha, omega, wv);
+                res_partials<2>(wv, red, rbuf);
             }
         }
     }
+    (void)HXY;
+}
+
+// The whole PBiCGSTAB! loop (src/NonUniform.jl:283-349, the iterations
+// after CalcRK! and the first ρ) with the :gs preconditioner in ONE launch
+// of the resident grid.  Each iteration is the two halves the launches
+// above run (k_rb_resident modes FIRST_BICG_P with CalcAX! and 4), with
+// the separate launches' boundaries replaced by in-launch grid barriers:
+//   p = r (first iteration) or BiCG1! p = r + beta (p - omega q);
+//   p_ = M⁻¹p; q = A p_; (q, r0) -> barrier -> alpha = rho / (q, r0);
+//   s = r - alpha q; s_ = M⁻¹s; t = A s_; (t, t), (t, s) -> barrier ->
+//   omega; BICG2!/Triad! on θ and r; (r, r), (r, r0) -> barrier ->
+//   FIN_BICG_RES_RHO (residual, history, convergence, next rho and beta).
+// The scalar steps run redundantly in every workgroup on the same exactly
+// rounded totals, so every workgroup holds the same solver state (in LDS);
+// workgroup 0 writes the history and, at the end, the state.  The tables,
+// slots and per-cell constants are set up once per solve instead of twice
+// per iteration.  s lives in registers (it is never stored), q is reused
+// from CalcAX!'s registers.  The three partial buffers of the in-launch
+// sums (4 doubles per workgroup each) are each reused once per iteration:
+// every workgroup has read one before any can pass the next barrier.
+template <typename T, bool FLUX>
+__global__ void __launch_bounds__(kResThreads) k_rb_bicg_solve(RbGeo<T> g, RbRes pl, ResCoef<T> cf, KState *st,
+                                                               double *hist, ResBicg<T> v)
+{
+    __shared__ KState ks;
+    const int tid = threadIdx.x;
+    if (tid == 0) ks = *st;
+    __syncthreads();
+    if (ks.status != ST_RUNNING) return;  // uniform
+    extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
+    const int HX = g.HX, K = g.K;
+    const int RJ = pl.TJ + 2;
+    const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);
+    const int ntab = 6 * K * K + K;
+    ResView<T> w;
+    w.xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
+    w.xs1 = w.xs0 + hsz;
+    w.tabs = w.xs1 + hsz;
+    w.bar = pl.bar;
+    w.tl = res_tile(pl, g.NY, g.NZ);
+    w.RJ = RJ;
+    w.HX = HX;
+    w.NX = g.NX;
+    w.NY = g.NY;
+    w.NZ = g.NZ;
+    w.HXY = (unsigned)HX * (unsigned)g.MY;
+    w.zs = (unsigned)(RJ * HX);
+    w.HS = g.HS;
+    T *tabs = w.xs1 + hsz;
+    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
+                                     (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
+    const size_t HS = g.HS;
+    ResRegs<T> rg;
+    res_setup<T, FLUX>(g, cf, w, ntab, tabs, rg);
+    __syncthreads();  // tables staged
+    double *wq = pl.wpart, *wt = pl.wpart + 4 * kResMaxWG, *wr = pl.wpart + 8 * kResMaxWG;
+    unsigned fc = bar_load(pl.bar + kBarFlags + kBarFlagStride * blockIdx.x);
+
+#pragma unroll 1
+    for (;;) {
+        // ---- p formed; p_ = M⁻¹p; q = A p_; Σ q·r0
+        T bv[kResSlots][2];
+        {
+            const bool first = ks.itr == 1;
+            const T beta = (T)ks.beta, omg = (T)ks.omega;
+#pragma unroll
+            for (int u = 0; u < kResSlots; ++u)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const size_t oh = (h ? HS : 0) + opq(rg.sl[u].base);
+                    bv[u][h] = first ? v.r[oh] : v.r[oh] + beta * (v.p[oh] - omg * v.q[oh]);
+                }
+        }
+        for (unsigned e = tid; e < 2 * hsz; e += kResThreads) w.xs0[e] = T(0);
+#pragma unroll
+        for (int u = 0; u < kResSlots; ++u)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (rg.valid[u][h]) v.p[(h ? HS : 0) + opq(rg.sl[u].base)] = bv[u][h];
+        __syncthreads();
+        res_gs5<T, FLUX>(w, v.p_, rg, bv, fc);
+        dd sv[2] = {{0.0, 0.0}, {0.0, 0.0}};
+        T tav[kResSlots][2] = {};
+        res_calc_ax<T, FLUX, 1>(w, v.p_, rg, bv, fc + 10u, v.q, v.r0, nullptr, false, sv, tav);
+        fc += 10u;
+        res_wpartials<1>(sv, red, wq);
+        res_grid_sync(pl.bar);
+        double tq[2];
+        res_total<1, true>(wq, 4, gridDim.x, red, tq);
+        const double alpha = ks.rho / tq[0];  // FIN_BICG_ALPHA (NonUniform.jl:309)
+
+        // ---- s = r - alpha q; s_ = M⁻¹s; t = A s_; Σ t·t, Σ t·s
+        {
+            const T r_alpha = (T)(-alpha);
+#pragma unroll
+            for (int u = 0; u < kResSlots; ++u)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) bv[u][h] = r_alpha * tav[u][h] + v.r[(h ? HS : 0) + opq(rg.sl[u].base)];
+        }
+        for (unsigned e = tid; e < 2 * hsz; e += kResThreads) w.xs0[e] = T(0);
+        __syncthreads();
+        res_gs5<T, FLUX>(w, v.s_, rg, bv, fc);
+        sv[0] = dd{0.0, 0.0};
+        res_calc_ax<T, FLUX, 2>(w, v.s_, rg, bv, fc + 10u, v.t, nullptr, nullptr, true, sv, tav);
+        fc += 10u;
+        res_wpartials<2>(sv, red, wt);
+        res_grid_sync(pl.bar);
+        double tw[2];
+        res_total<2, true>(wt, 4, gridDim.x, red, tw);
+        // FIN_BICG_OMEGA (NonUniform.jl:320-326)
+        if (fabs(tw[0]) < kFloatMin) {
+            if (tid == 0) {
+                ks.alpha = alpha;
+                ks.status = ST_BREAKDOWN;
+            }
+            break;
+        }
+        const double omega = tw[1] / tw[0];
+
+        // ---- θ += alpha p_ + omega s_; r = s - omega t; Σ r·r, Σ r·r0
+        dd wv[2];
+        res_bicg_update<T>(g, w, rg, bv, tav, v.th, v.xbest != nullptr && ks.improved != 0 ? v.xbest : nullptr, v.r,
+                           v.p_, v.r0, alpha, omega, wv);
+        res_wpartials<2>(wv, red, wr);
+        res_grid_sync(pl.bar);
+        double tr[2];
+        res_total<2, true>(wr, 4, gridDim.x, red, tr);
+        if (tid == 0) {
+            ks.alpha = alpha;
+            ks.omega = omega;
+            finalize(FIN_BICG_RES_RHO, tr, &ks, blockIdx.x == 0 ? hist : nullptr);
+        }
+        __syncthreads();
+        if (ks.status != ST_RUNNING) break;
+        // a timed-out wait leaves the iterates wrong: stop early (the host
+        // reports it)
+        if (bar_load(pl.bar + kBarErr) != 0u) break;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid == 0) *st = ks;
 }
 
 template <typename T>
@@ -779,6 +1021,12 @@ template <typename T>
 hipError_t rb_res_prepare()
 {
     hipError_t e;
+    if ((e = hipFuncSetAttribute((const void *)k_rb_bicg_solve<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kResLdsMax)) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute((const void *)k_rb_bicg_solve<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kResLdsMax)) != hipSuccess)
+        return e;
     if ((e = res_attr_mode<T, 0>()) != hipSuccess) return e;
     if ((e = res_attr_mode<T, 1>()) != hipSuccess) return e;
     if ((e = res_attr_mode<T, 2>()) != hipSuccess) return e;
@@ -856,6 +1104,23 @@ hipError_t launch_rb_resident_bicg_s(T *s_, T *sv, const T *r, const T *q, T *t,
 }
 
 
+template <typename T>
+hipError_t launch_rb_bicg_solve(const ResBicg<T> &v, const RbGeo<T> &g, const RbRes &pl, KState *st, double *hist,
+                                hipStream_t s)
+{
+    if (pl.G <= 0 || !pl.bar || !pl.wpart || pl.lds > kResLdsMax) return hipErrorInvalidValue;
+    const ResCoef<T> cf{reinterpret_cast<T *>(pl.dgl), reinterpret_cast<T *>(pl.snk), pl.ix};
+    if (!cf.dgl || !cf.ix || (g.flux && !cf.snk)) return hipErrorInvalidValue;
+    if (!v.p || !v.p_ || !v.r || !v.q || !v.s_ || !v.t || !v.r0 || !v.th) return hipErrorInvalidValue;
+    const dim3 grid(pl.G), block(kResThreads);
+    if (g.flux)
+        hipLaunchKernelGGL((k_rb_bicg_solve<T, true>), grid, block, pl.lds, s, g, pl, cf, st, hist, v);
+    else
+        hipLaunchKernelGGL((k_rb_bicg_solve<T, false>), grid, block, pl.lds, s, g, pl, cf, st, hist, v);
+    return hipGetLastError();
+}
+
+
 #define RES_INST(T)                                                                                              \
     template RbRes rb_res_plan<T>(const RbGeo<T> &, int);                                                       \
     template hipError_t rb_res_prepare<T>();                                                                    \
@@ -865,7 +1130,9 @@ hipError_t launch_rb_resident_bicg_s(T *s_, T *sv, const T *r, const T *q, T *t,
                                               const ResAx<T> *);                                              \
     template hipError_t launch_rb_resident_bicg_s<T>(T *, T *, const T *, const T *, T *, const T *, double *,   \
                                                      double *, T *, const T *, const RbGeo<T> &, const RbRes &,  \
-                                                     KState *, double *, const RedBuf &, hipStream_t);
+                                                     KState *, double *, const RedBuf &, hipStream_t);      \
+    template hipError_t launch_rb_bicg_solve<T>(const ResBicg<T> &, const RbGeo<T> &, const RbRes &, KState *,   \
+                                                double *, hipStream_t);
 
 RES_INST(double)
 RES_INST(float)

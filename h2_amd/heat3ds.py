@@ -275,12 +275,14 @@ class WorkBuffers:
         planes, -1 = chosen from the grid (default).  Bitwise identical."""
         _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_SWEEP_CHUNK, int(chunk)))
 
-    def set_resident(self, on):
-        """1 (default): on grids whose iterate fits the CUs' LDS the Gauss-
-        Seidel preconditioner runs as one launch of co-resident workgroups
-        (when the sweep chunk is left at -1); 0: one launch per colour pass.
-        Bitwise identical."""
-        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_RESIDENT, int(on)))
+    def set_resident(self, mode):
+        """On grids whose iterate fits the CUs' LDS (when the sweep chunk is
+        left at -1): 2 (default) = PBiCGSTAB runs every iteration of a solve
+        in one launch of co-resident workgroups; 1 = the Gauss-Seidel
+        preconditioner (with its neighbours in the iteration) runs as one
+        such launch, three launches per PBiCGSTAB iteration; 0 = one launch
+        per colour pass.  Bitwise identical."""
+        _check(self.ctx, self.lib.h3d_set_option(self.ctx, L.OPT_RESIDENT, int(mode)))
 
     def resident_workgroups(self):
         """Workgroups of the last solve's resident preconditioner (0: none)."""

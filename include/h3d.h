@@ -98,10 +98,14 @@ extern "C" {
                                   runs its last sweep per colour either way (that pass also forms
                                   rho = (r, z)); bitwise identical */
 #define H3D_OPT_RESIDENT 6     /* Gauss-Seidel preconditioner of the colour-split layout on grids whose
-                                  iterate fits the CUs' LDS (240x240x30 in FP64 and below): 1 = all five
-                                  sweeps in one launch of co-resident workgroups that keep the iterate
-                                  in LDS (default, used when H3D_OPT_SWEEP_CHUNK is -1 and the grid
-                                  fits), 0 = one launch per colour pass; bitwise identical */
+                                  iterate fits the CUs' LDS (240x240x30 in FP64 and below), used when
+                                  H3D_OPT_SWEEP_CHUNK is -1 and the grid fits: 2 (default) = as 1, and
+                                  h3d_pbicgstab runs every iteration of a solve in ONE launch of the
+                                  resident grid (in-launch grid barriers for its sums; not with
+                                  H3D_SUM_SEQUENTIAL); 1 = all five sweeps in one launch of co-resident
+                                  workgroups that keep the iterate in LDS (three launches per
+                                  PBiCGSTAB iteration); 0 = one launch per colour pass; bitwise
+                                  identical */
 #define H3D_OPT_RESIDENT_WGS 7 /* read-only: workgroups of the last solve's resident preconditioner
                                   launches (0 when it did not use one) */
 #define H3D_OPT_SUM_ORDER 5    /* order of the Krylov solvers' grid sums (dots, residual norms) */
@@ -302,7 +306,10 @@ int h3d_sor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, 
  * the whole preconditioner (five sweeps from zero) as one resident launch,
  * FP64 and FP32 (H3D_OPT_RESIDENT; H3D_ERR_ARG when the grid does not fit);
  * 9 and 10 = PBiCGSTAB!'s first resident launch of an iteration (BiCG1!,
- * the preconditioner, CalcAX! and its (q, r0) partials), FP64 and FP32. */
+ * the preconditioner, CalcAX! and its (q, r0) partials), FP64 and FP32;
+ * 11 = PBiCGSTAB!'s one-launch loop (FP64) run for `reps` iterations with
+ * tolerance 0 from θ = 0 on the current b (a scratch θ: the fields are left
+ * as they are), *ms = the launch's time per iteration run. */
 int h3d_time_kernel(h3d_ctx *ctx, int which, int reps, double dt, int is_steady, double *ms);
 
 /* Fdot1/Fdot2 on caller-owned device arrays without a context and without

@@ -1,10 +1,11 @@
 """The resident Gauss-Seidel preconditioner (all five red-black sweeps of
 Preconditioner!, src/NonUniform.jl:531-559, in one launch of co-resident
-workgroups holding the iterate in LDS) against the per-colour launches:
-whole solves must be bitwise identical (iteration count, residual history,
-θ) for both solvers, steady and transient, FP64 and the FP32 refinement,
-on grids with even and odd row lengths and tiles cut unevenly, and at the
-240x240x30 headline case."""
+workgroups holding the iterate in LDS; mode 1) and PBiCGSTAB!'s whole loop
+(src/NonUniform.jl:283-349) in one such launch (mode 2) against the
+per-colour launches (mode 0): whole solves must be bitwise identical
+(iteration count, residual history, θ) for both solvers, steady and
+transient, FP64 and the FP32 refinement, on grids with even and odd row
+lengths and tiles cut unevenly, and at the 240x240x30 headline case."""
 import numpy as np
 import pytest
 
@@ -35,21 +36,37 @@ def run(m, resident, solver, steady, precision, tol, max_itr=3000):
 def test_resident_solves_bitwise(shape, solver, steady, precision):
     print("case resident", shape, "solver", solver, "steady", steady, "precision", precision, flush=True)
     m = h2_amd.build_model(*shape)
-    a = run(m, 1, solver, steady, precision, 1e-8)
     b = run(m, 0, solver, steady, precision, 1e-8)
-    assert a[3] > 0, "the resident preconditioner must have run"
     assert b[3] == 0
+    modes = (1, 2) if solver == L.SOLVER_PBICGSTAB else (1,)
+    for mode in modes:
+        a = run(m, mode, solver, steady, precision, 1e-8)
+        assert a[3] > 0, "the resident preconditioner must have run"
+        assert a[0] == b[0], mode
+        assert np.array_equal(a[1], b[1]), mode
+        assert np.array_equal(a[2], b[2]), mode
+
+
+@pytest.mark.parametrize("max_itr", [1, 2, 7])
+def test_resident_one_launch_max_itr(max_itr):
+    """The one-launch loop stops at maxItr with the state, history and θ the
+    per-iteration launches leave."""
+    m = h2_amd.build_model(40, 40, 30)
+    a = run(m, 2, L.SOLVER_PBICGSTAB, True, L.FP64, 1e-12, max_itr=max_itr)
+    b = run(m, 0, L.SOLVER_PBICGSTAB, True, L.FP64, 1e-12, max_itr=max_itr)
     assert a[0] == b[0]
+    assert len(a[1]) == max_itr
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])
 
 
-@pytest.mark.parametrize("solver,tol", [(L.SOLVER_PBICGSTAB, 1e-4), (L.SOLVER_CG, 1e-6)])
-def test_resident_headline_bitwise(solver, tol):
+@pytest.mark.parametrize("solver,tol,mode", [(L.SOLVER_PBICGSTAB, 1e-4, 1), (L.SOLVER_PBICGSTAB, 1e-4, 2),
+                                             (L.SOLVER_CG, 1e-6, 1)])
+def test_resident_headline_bitwise(solver, tol, mode):
     """modelA 240x240x30, steady, PBiCGSTAB+GS to 1e-4 and CG+GS to 1e-6."""
-    print("case resident 240x240x30 solver", solver, "tol", tol, flush=True)
+    print("case resident 240x240x30 solver", solver, "tol", tol, "mode", mode, flush=True)
     m = h2_amd.build_model(240, 240, 30)
-    a = run(m, 1, solver, True, L.FP64, tol)
+    a = run(m, mode, solver, True, L.FP64, tol)
     b = run(m, 0, solver, True, L.FP64, tol)
     assert a[3] > 0
     assert a[0] == b[0]
@@ -58,11 +75,14 @@ def test_resident_headline_bitwise(solver, tol):
 
 
 def test_resident_kernel_timing_entry():
-    """h3d_time_kernel ids 7 and 8 run the resident launch on its own."""
+    """h3d_time_kernel ids 7 to 11 run the resident launches on their own;
+    id 11 (the one-launch loop) leaves θ as it was."""
     m = h2_amd.build_model(40, 40, 30)
     with WorkBuffers(*m.shape) as wk:
         load_model(wk, m)
         wk.calc_rhs(10000.0, True)
-        for which in (7, 8):
+        th0 = wk.download("theta")
+        for which in (7, 8, 9, 10, 11):
             ms = wk.time_kernel(which, 5)
             assert 0.0 < ms < 50.0
+        assert np.array_equal(wk.download("theta"), th0)
