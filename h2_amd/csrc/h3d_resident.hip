@@ -181,18 +181,19 @@ __device__ __forceinline__ void res_wait(unsigned *bar, const ResTile &tl, unsig
     __syncthreads();
 }
 
-// The ring rows of half h (ring row hr = 0..2tj+2tk-1: planes k0-1 and
-// k0+tk, then rows j0-1 and j0+tj of each plane) from x in HBM (xg: that
-// half) into LDS xo.  Only interior cells are read: the others (rows
-// outside the interior, the cells i = 0 and i = NX+1 and the padding of
-// each row) are the domain's halo, zero in LDS.  Every thread must call
-// it; ends with a barrier.
-template <typename T>
-__device__ __forceinline__ void res_load_ring(T *xo, const T *xg, const ResTile &tl, int RJ, int HX, unsigned HXY,
-                                              int NX, int NY, int NZ, int h)
+// The tile's ring (ring row hr = 0..2tj+2tk-1: planes k0-1 and k0+tk, then
+// rows j0-1 and j0+tj of each plane, HX cells each) as a table in LDS, one
+// entry per ring cell: the cell's offset in an HBM half, and its offset in
+// an LDS half with bit 16 + h set when the cell is an interior cell in half
+// h.  Only interior cells are ever read: the others (rows outside the
+// interior, the cells i = 0 and i = NX+1 and the padding of each row) are
+// the domain's halo, zero in LDS.  Formed once per launch, so that a ring
+// load is one round of independent loads with no index arithmetic.
+__device__ __forceinline__ void res_ring_table(uint2 *rt, const ResTile &tl, int RJ, int HX, unsigned HXY, int NX,
+                                               int NY, int NZ)
 {
     const unsigned nring = (unsigned)(2 * tl.tj + 2 * tl.tk) * (unsigned)HX;
-    for (unsigned e = opq(threadIdx.x); e < nring; e += kResThreads) {
+    for (unsigned e = threadIdx.x; e < nring; e += kResThreads) {
         const unsigned hr = e / (unsigned)HX, m = e - hr * (unsigned)HX;
         int jj, kk;
         if (hr < (unsigned)tl.tj) {
@@ -209,9 +210,43 @@ __device__ __forceinline__ void res_load_ring(T *xo, const T *xg, const ResTile 
             kk = (int)hr - 2 * tl.tj - tl.tk + 1;
         }
         const int j = tl.j0 + jj - 1, k = tl.k0 + kk - 1;
-        const int i = 2 * (int)m + ((h + j + k) & 1);
-        if (j >= 1 && j <= NY && k >= 1 && k <= NZ && i >= 1 && i <= NX)
-            xo[(kk * RJ + jj) * HX + (int)m] = sc_load(xg + m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k);
+        unsigned go = 0, bits = 0;
+        if (j >= 1 && j <= NY && k >= 1 && k <= NZ) {
+            go = m + (unsigned)HX * (unsigned)j + HXY * (unsigned)k;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = 2 * (int)m + ((h + j + k) & 1);
+                if (i >= 1 && i <= NX) bits |= 1u << (16 + h);
+            }
+        }
+        rt[e] = make_uint2(go, (unsigned)((kk * RJ + jj) * HX + (int)m) | bits);
+    }
+}
+
+constexpr int kRingBatch = 4;  // ring loads a thread has in flight at once
+
+// The ring cells of half h from x in HBM (xg: that half) into LDS xo, by the
+// table of res_ring_table.  Every thread must call it; ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void res_load_ring(T *xo, const T *xg, const uint2 *rt, unsigned nring, int h)
+{
+    const unsigned sh = 16u + (unsigned)h;
+#pragma unroll 1
+    for (unsigned e0 = 0; e0 < nring; e0 += kRingBatch * kResThreads) {
+        T v[kRingBatch];
+        unsigned lo[kRingBatch];
+        bool ok[kRingBatch];
+#pragma unroll
+        for (int q = 0; q < kRingBatch; ++q) {
+            const unsigned e = e0 + threadIdx.x + (unsigned)(q * kResThreads);
+            const uint2 w = e < nring ? rt[e] : make_uint2(0u, 0u);
+            ok[q] = ((w.y >> sh) & 1u) != 0u;
+            lo[q] = w.y & 0xffffu;
+            v[q] = ok[q] ? sc_load(xg + w.x) : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kRingBatch; ++q)
+            if (ok[q]) xo[lo[q]] = v[q];
     }
     __syncthreads();
 }
@@ -384,6 +419,8 @@ struct ResView {
     int RJ, HX, NX, NY, NZ;
     unsigned HXY, zs;  // zs: the LDS offset of the next plane
     size_t HS;
+    const uint2 *rt;   // the ring table (res_ring_table)
+    unsigned nring;    // its entries
 };
 
 // The per-slot registers of a tile's cells (see k_rb_resident).
@@ -420,8 +457,7 @@ __device__ __forceinline__ void res_gs5(const ResView<T> &w, T *x, const ResRegs
             for (int phase = 0; phase < 2; ++phase) {
                 if (phase == 1 && pass > 0) {
                     res_wait(w.bar, w.tl, f0 + (unsigned)pass);
-                    res_load_ring(xt, (const T *)(x + (h ? 0 : w.HS)), w.tl, w.RJ, w.HX, w.HXY, w.NX, w.NY, w.NZ,
-                                  1 - h);
+                    res_load_ring(xt, (const T *)(x + (h ? 0 : w.HS)), w.rt, w.nring, 1 - h);
                 }
 #pragma unroll
                 for (int u = 0; u < kResSlots; ++u) {
@@ -488,7 +524,7 @@ __device__ __forceinline__ void res_calc_ax(const ResView<T> &w, const T *x, con
 {
     const unsigned HX = (unsigned)w.HX, zs = w.zs;
     res_wait(w.bar, w.tl, want);
-    res_load_ring(w.xs0, x, w.tl, w.RJ, w.HX, w.HXY, w.NX, w.NY, w.NZ, 0);
+    res_load_ring(w.xs0, x, w.rt, w.nring, 0);
 #pragma unroll
     for (int u = 0; u < kResSlots; ++u) {
         if (!rg.sl[u].own) continue;
@@ -545,10 +581,11 @@ __device__ __forceinline__ void res_calc_ax(const ResView<T> &w, const T *x, con
 // Ends without a barrier.
 template <typename T, bool FLUX>
 __device__ __forceinline__ void res_setup(const RbGeo<T> &g, const ResCoef<T> &cf, const ResView<T> &w, int ntab,
-                                          T *tabs, ResRegs<T> &rg)
+                                          T *tabs, uint2 *rt, ResRegs<T> &rg)
 {
     const int tid = threadIdx.x, K = g.K;
     const ResTile &tl = w.tl;
+    res_ring_table(rt, tl, w.RJ, w.HX, w.HXY, g.NX, g.NY, g.NZ);
     {
         const int n = 2 * K * K, per = n + K;
         for (int idx = tid; idx < tl.tk * per; idx += kResThreads) {
@@ -709,12 +746,15 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     T *tabs = w.xs1 + hsz;
     dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
                                      (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
+    uint2 *rt = reinterpret_cast<uint2 *>(red + 2 * 68);
+    w.rt = rt;
+    w.nring = (unsigned)(2 * w.tl.tj + 2 * w.tl.tk) * (unsigned)HX;
     const size_t HS = g.HS;
     const unsigned HXY = w.HXY;
 
     // tables, slots and per-cell constants
     ResRegs<T> rg;
-    res_setup<T, FLUX>(g, cf, w, ntab, tabs, rg);
+    res_setup<T, FLUX>(g, cf, w, ntab, tabs, rt, rg);
 
     // the right-hand side (formed as k_rb_first forms it)
     T beta = T(0), omg = T(0), r_alpha = T(0);
@@ -863,9 +903,12 @@ __global__ void __launch_bounds__(kResThreads) k_rb_bicg_solve(RbGeo<T> g, RbRes
     T *tabs = w.xs1 + hsz;
     dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
                                      (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
+    uint2 *rt = reinterpret_cast<uint2 *>(red + 2 * 68);
+    w.rt = rt;
+    w.nring = (unsigned)(2 * w.tl.tj + 2 * w.tl.tk) * (unsigned)HX;
     const size_t HS = g.HS;
     ResRegs<T> rg;
-    res_setup<T, FLUX>(g, cf, w, ntab, tabs, rg);
+    res_setup<T, FLUX>(g, cf, w, ntab, tabs, rt, rg);
     __syncthreads();  // tables staged
     double *wq = pl.wpart, *wt = pl.wpart + 4 * kResMaxWG, *wr = pl.wpart + 8 * kResMaxWG;
     unsigned fc = bar_load(pl.bar + kBarFlags + kBarFlagStride * blockIdx.x);
@@ -961,7 +1004,8 @@ size_t rb_res_lds(int TJ, int TK, int HX, int K)
     const size_t ntab = (size_t)6 * K * K + K;
     size_t n = (2 * hsz + (size_t)TK * ntab) * sizeof(T);
     n = (n + 15) & ~(size_t)15;
-    return n + 2 * 68 * sizeof(dd);
+    // the sums' scratch, then the ring table
+    return n + 2 * 68 * sizeof(dd) + (size_t)(2 * TJ + 2 * TK) * HX * sizeof(uint2);
 }
 
 // Tile plan: the tiling with the fewest slot rounds per thread (compute per
@@ -981,7 +1025,9 @@ RbRes rb_res_plan(const RbGeo<T> &g, int ncu)
             if (rows * g.HX > (long)kResThreads * kResSlots) break;
             const int ntj = (g.NY + TJ - 1) / TJ, ntk = (g.NZ + TK - 1) / TK;
             const long G = (long)ntj * ntk;
-            if (G > ncu || G > kResMaxWG || TK * ntab >= 65536) continue;
+            // 16-bit table indices and ring-table LDS offsets
+            const long hsz = (long)(TJ + 2) * (TK + 2) * g.HX + 1;
+            if (G > ncu || G > kResMaxWG || TK * ntab >= 65536 || hsz >= 65536) continue;
             const size_t lds = rb_res_lds<T>(TJ, TK, g.HX, g.K);
             if (lds > kResLdsMax) continue;
             const long rounds = (rows * g.HX + kResThreads - 1) / kResThreads;
