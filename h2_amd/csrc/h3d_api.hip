@@ -584,11 +584,11 @@ template <typename T>
 int res_finish(h3d_ctx *c, const Path<T> &P, int smoother);
 int res_check(h3d_ctx *c, const RbRes &pl);
 
-// PBiCGSTAB! runs every iteration of a solve in one resident launch
+// The Krylov solvers run every iteration of a solve in one resident launch
 // (H3D_OPT_RESIDENT 2) when the resident preconditioner applies and the
 // grid sums are the exactly rounded ones.
 template <typename T>
-bool one_launch_bicg(const h3d_ctx *c, const Path<T> &P, int smoother)
+bool one_launch(const h3d_ctx *c, const Path<T> &P, int smoother)
 {
     return c->resident == 2 && P.rb && P.res.G > 0 && smoother == H3D_SMOOTHER_GS && !c->rb.seq;
 }
@@ -617,11 +617,14 @@ int krylov(h3d_ctx *c, int solver, const KVecs<T> &v, const Path<T> &P, double t
         HIPCHK(c, p_copy_interior<T>(P, v.p, v.s, c->d_state, s));
         HIPCHK(c, p_dot<T>(P, v.r, v.s, nullptr, FIN_CG_RHO0, c->d_state, nullptr, c->rb, s));
     }
-    if (bicg && one_launch_bicg<T>(c, P, smoother)) {
+    if (one_launch<T>(c, P, smoother)) {
         // every iteration in one launch of the resident grid
-        if (maxItr >= 1) {
+        if (maxItr >= 1 && bicg) {
             const ResBicg<T> rv{v.p, v.p_, v.r, v.q, v.s_, v.t, v.r0, v.x, v.xbest};
             HIPCHK(c, launch_rb_bicg_solve<T>(rv, P.rg, P.res, c->d_state, c->d_hist, s));
+        } else if (maxItr >= 1) {
+            const ResCg<T> rv{v.p, v.r, v.q, v.s, v.x, v.xbest};
+            HIPCHK(c, launch_rb_cg_solve<T>(rv, P.rg, P.res, c->d_state, c->d_hist, s));
         }
         HIPCHK(c, hipMemcpyAsync(&c->h_state[1], c->d_state, sizeof(KState), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));

@@ -201,6 +201,14 @@ struct ResBicg {
     const T *r0;
     double *th, *xbest;
 };
+// The vectors of the one-launch CG! loop (launch_rb_cg_solve): p, r, q
+// and z = M⁻¹r in the colour-split layout, θ (and the guarded solve's best
+// iterate) in the standard one.
+template <typename T>
+struct ResCg {
+    T *p, *r, *q, *z;
+    double *th, *xbest;
+};
 constexpr size_t kResLdsMax = 150 * 1024;
 constexpr int kResMaxWG = 1024;
 // synchronisation words: an error word, then a pass counter per workgroup
@@ -456,6 +464,14 @@ hipError_t launch_rb_resident_bicg_s(T *s_, T *sv, const T *r, const T *q, T *t,
 template <typename T>
 hipError_t launch_rb_bicg_solve(const ResBicg<T> &v, const RbGeo<T> &g, const RbRes &pl, KState *st, double *hist,
                                 hipStream_t s);
+// Every CG! iteration of a solve (after CalcRK!, z = M⁻¹r, p = z and
+// ρ_old = (r, z), FIN_CG_RHO0) in one launch of the resident grid, until
+// the state leaves ST_RUNNING; the state and the residual history are
+// written as the per-iteration launches write them.  Not in the
+// reference-order sum mode.
+template <typename T>
+hipError_t launch_rb_cg_solve(const ResCg<T> &v, const RbGeo<T> &g, const RbRes &pl, KState *st, double *hist,
+                              hipStream_t s);
 // CalcAX! fused after the resident preconditioner: ax = A x of its result,
 // s0 = Σ ax·(dy ? dy : ax), s1 = Σ ax·dz (when dz), finalized with fin.
 template <typename T>

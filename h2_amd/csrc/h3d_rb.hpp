@@ -170,12 +170,13 @@ __device__ __forceinline__ Cond<T> rb_cond_dz(const RbGeo<T> &g, const T *stab, 
     const bool ixm = i > 1, ixp = i < g.NX, iym = j > 1, iyp = j < g.NY;
     // stab: XT, YT (plane k) and the two z faces, staged by rb_stage
     Cond<T> o;
-    o.xm = stab[((ixm ? K : 0) + q.xm) * K + b];
-    o.xp = stab[((ixp ? K : 0) + q.xp) * K + b];
-    o.ym = stab[n2 + ((iym ? K : 0) + q.ym) * K + b];
-    o.yp = stab[n2 + ((iyp ? K : 0) + q.yp) * K + b];
-    o.zm = stab[2 * n2 + q.zm * K + b];
-    o.zp = stab[2 * n2 + K * K + q.zp * K + b];
+    // indices below 2^24: full-rate 24-bit multiplies
+    o.xm = stab[__umul24((ixm ? K : 0) + q.xm, K) + b];
+    o.xp = stab[__umul24((ixp ? K : 0) + q.xp, K) + b];
+    o.ym = stab[n2 + __umul24((iym ? K : 0) + q.ym, K) + b];
+    o.yp = stab[n2 + __umul24((iyp ? K : 0) + q.yp, K) + b];
+    o.zm = stab[2 * n2 + __umul24(q.zm, K) + b];
+    o.zp = stab[2 * n2 + K * K + __umul24(q.zp, K) + b];
     // convective terms: exact zeros away from the boundary (skipped there)
     o.cxm = o.cxp = o.cym = o.cyp = o.czm = o.czp = T(0);
     if (!(ixm && ixp && iym && iyp && k > 1 && k < g.NZ)) {

@@ -512,19 +512,16 @@ __device__ __forceinline__ void res_gs5(const ResView<T> &w, T *x, const ResRegs
     }
 }
 
-// CalcAX! (src/NonUniform.jl:454-511) of x, read from LDS once the ring rows
-// of half 0 (updated by the last pass, published as count want) have
-// arrived: ax = A x on the tile's cells, tav = the same values for the
-// caller, v[0] += Σ ax·(dy ? dy : ax), v[1] += Σ ax·(dz_bb ? bv : dz) when
-// NS = 2.  Every thread must call it.
-template <typename T, bool FLUX, int NS>
-__device__ __forceinline__ void res_calc_ax(const ResView<T> &w, const T *x, const ResRegs<T> &rg,
-                                            const T (&bv)[kResSlots][2], unsigned want, T *ax, const T *dy,
-                                            const T *dz, bool dz_bb, dd (&v)[2], T (&tav)[kResSlots][2])
+// CalcAX! (src/NonUniform.jl:454-511) of x, read from LDS (both halves and
+// their rings in place): ax = A x on the tile's cells, tav = the same values
+// for the caller, v[0] += Σ ax·y with y = x (DYX), dy, or ax itself,
+// v[1] += Σ ax·(dz_bb ? bv : dz) when NS = 2.  Every thread must call it.
+template <typename T, bool FLUX, int NS, bool DYX = false>
+__device__ __forceinline__ void res_calc_ax_core(const ResView<T> &w, const ResRegs<T> &rg,
+                                                 const T (&bv)[kResSlots][2], T *ax, const T *dy, const T *dz,
+                                                 bool dz_bb, dd (&v)[2], T (&tav)[kResSlots][2])
 {
     const unsigned HX = (unsigned)w.HX, zs = w.zs;
-    res_wait(w.bar, w.tl, want);
-    res_load_ring(w.xs0, x, w.rt, w.nring, 0);
 #pragma unroll
     for (int u = 0; u < kResSlots; ++u) {
         if (!rg.sl[u].own) continue;
@@ -563,7 +560,7 @@ __device__ __forceinline__ void res_calc_ax(const ResView<T> &w, const T *x, con
             if (rg.valid[u][h]) {
                 const size_t c = (h ? w.HS : 0) + base;
                 ax[c] = av;
-                const T yv = dy ? dy[c] : av;
+                const T yv = DYX ? x0 : dy ? dy[c] : av;
                 v[0] = dd_add(v[0], (double)av * (double)yv);
                 if (NS >= 2) {
                     const T zv = dz_bb ? bv[u][h] : dz[c];
@@ -573,6 +570,18 @@ __device__ __forceinline__ void res_calc_ax(const ResView<T> &w, const T *x, con
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+}
+
+// res_calc_ax_core after the ring rows of half 0 (updated by the last
+// pass, published as count want) have arrived.
+template <typename T, bool FLUX, int NS>
+__device__ __forceinline__ void res_calc_ax(const ResView<T> &w, const T *x, const ResRegs<T> &rg,
+                                            const T (&bv)[kResSlots][2], unsigned want, T *ax, const T *dy,
+                                            const T *dz, bool dz_bb, dd (&v)[2], T (&tav)[kResSlots][2])
+{
+    res_wait(w.bar, w.tl, want);
+    res_load_ring(w.xs0, x, w.rt, w.nring, 0);
+    res_calc_ax_core<T, FLUX, NS>(w, rg, bv, ax, dy, dz, dz_bb, v, tav);
 }
 
 // The tables of the tile's planes in one round of loads (the entries
@@ -637,10 +646,10 @@ __device__ __forceinline__ void res_setup(const RbGeo<T> &g, const ResCoef<T> &c
 }
 
 // Up to two per-thread sums over the workgroup into its device-scope
-// partials wp[4 * blockIdx.x + 2s + {0, 1}] (hi, lo), for a grid
+// partials wp[4 * blockIdx.x + 2(s0 + s) + {0, 1}] (hi, lo), for a grid
 // reduction inside the launch.  Every thread must call it.
 template <int NS>
-__device__ __forceinline__ void res_wpartials(const dd (&v)[2], dd *red, double *wp)
+__device__ __forceinline__ void res_wpartials(const dd (&v)[2], dd *red, double *wp, int s0 = 0)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 #pragma unroll
@@ -662,8 +671,8 @@ __device__ __forceinline__ void res_wpartials(const dd (&v)[2], dd *red, double 
         for (int q = 0; q < NS; ++q) {
             const dd *rq = red + q * 68 + 64;
             const dd t = dd_add_tree(dd_add_tree(rq[0], rq[1]), dd_add_tree(rq[2], rq[3]));
-            sc_store(wp + 4 * (size_t)blockIdx.x + 2 * q, t.hi);
-            sc_store(wp + 4 * (size_t)blockIdx.x + 2 * q + 1, t.lo);
+            sc_store(wp + 4 * (size_t)blockIdx.x + 2 * (s0 + q), t.hi);
+            sc_store(wp + 4 * (size_t)blockIdx.x + 2 * (s0 + q) + 1, t.lo);
         }
     }
     __syncthreads();
@@ -997,6 +1006,169 @@ __global__ void __launch_bounds__(kResThreads) k_rb_bicg_solve(RbGeo<T> g, RbRes
     if (blockIdx.x == 0 && tid == 0) *st = ks;
 }
 
+// The whole CG! loop (src/NonUniform.jl:160-223, the iterations after
+// CalcRK!, the first preconditioner application, p = z and ρ_old) with the
+// :gs preconditioner in ONE launch of the resident grid.  An iteration:
+//   q = A p from LDS (p's ring rows exchanged once), Σ p·q -> barrier ->
+//   FIN_CG_ALPHA; θ += α p, r -= α q, Σ r²; z = M⁻¹ r (ten passes in LDS),
+//   Σ r·z; both sums -> one barrier -> FIN_CG_RES, then FIN_CG_BETA while
+//   the solve runs; p = z + β p, stored and published for the next CalcAX!.
+// Sharing one barrier between Σ r² and Σ r·z runs the preconditioner of
+// the iteration that converges as well; its z is workspace nothing reads,
+// and the scalar steps run in the reference's order, so θ, r, the history
+// and the state are those of the per-iteration launches.  As in
+// k_rb_bicg_solve the scalar steps run in every workgroup on the same
+// exactly rounded totals; workgroup 0 writes the history and the state.
+template <typename T, bool FLUX>
+__global__ void __launch_bounds__(kResThreads) k_rb_cg_solve(RbGeo<T> g, RbRes pl, ResCoef<T> cf, KState *st,
+                                                             double *hist, ResCg<T> v)
+{
+    __shared__ KState ks;
+    const int tid = threadIdx.x;
+    if (tid == 0) ks = *st;
+    __syncthreads();
+    if (ks.status != ST_RUNNING) return;  // uniform
+    extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
+    const int HX = g.HX, K = g.K;
+    const int RJ = pl.TJ + 2;
+    const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);
+    const int ntab = 6 * K * K + K;
+    ResView<T> w;
+    w.xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
+    w.xs1 = w.xs0 + hsz;
+    w.tabs = w.xs1 + hsz;
+    w.bar = pl.bar;
+    w.tl = res_tile(pl, g.NY, g.NZ);
+    w.RJ = RJ;
+    w.HX = HX;
+    w.NX = g.NX;
+    w.NY = g.NY;
+    w.NZ = g.NZ;
+    w.HXY = (unsigned)HX * (unsigned)g.MY;
+    w.zs = (unsigned)(RJ * HX);
+    w.HS = g.HS;
+    T *tabs = w.xs1 + hsz;
+    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
+                                     (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
+    uint2 *rt = reinterpret_cast<uint2 *>(red + 2 * 68);
+    w.rt = rt;
+    w.nring = (unsigned)(2 * w.tl.tj + 2 * w.tl.tk) * (unsigned)HX;
+    const size_t HS = g.HS;
+    ResRegs<T> rg;
+    res_setup<T, FLUX>(g, cf, w, ntab, tabs, rt, rg);
+    // p's cells of the tile into LDS, whose other entries (the domain's
+    // halo and the ring) start at zero
+    for (unsigned e = tid; e < 2 * hsz; e += kResThreads) w.xs0[e] = T(0);
+    __syncthreads();  // tables staged, LDS zeroed
+#pragma unroll
+    for (int u = 0; u < kResSlots; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (rg.valid[u][h]) (h ? w.xs1 : w.xs0)[rg.sl[u].e] = v.p[(h ? HS : 0) + opq(rg.sl[u].base)];
+    double *wq = pl.wpart, *wr = pl.wpart + 4 * kResMaxWG;
+    unsigned fc = bar_load(pl.bar + kBarFlags + kBarFlagStride * blockIdx.x);
+    const unsigned HXY = w.HXY;
+    const size_t sy = (size_t)g.MX, sz = (size_t)g.MX * (size_t)g.MY;
+
+#pragma unroll 1
+    for (;;) {
+        // ---- q = A p; Σ p·q (the neighbours' p published as count fc)
+        res_wait(pl.bar, w.tl, fc);
+        res_load_ring(w.xs0, (const T *)v.p, w.rt, w.nring, 0);
+        res_load_ring(w.xs1, (const T *)(v.p + HS), w.rt, w.nring, 1);
+        dd sv[2] = {{0.0, 0.0}, {0.0, 0.0}};
+        T tav[kResSlots][2] = {};
+        // (the right-hand side argument is read only with two sums)
+        res_calc_ax_core<T, FLUX, 1, true>(w, rg, tav, v.q, nullptr, nullptr, false, sv, tav);
+        res_wpartials<1>(sv, red, wq);
+        res_grid_sync(pl.bar);
+        double tq[2];
+        res_total<1, true>(wq, 4, gridDim.x, red, tq);
+        if (tid == 0) finalize(FIN_CG_ALPHA, tq, &ks, nullptr);  // NonUniform.jl:160-176
+        __syncthreads();
+        if (ks.status != ST_RUNNING) break;
+
+        // ---- θ += α p; r -= α q; Σ r² (as k_rb_cg_update forms them)
+        T bv[kResSlots][2];  // the new r: the preconditioner's right-hand side
+        {
+            const double alpha_d = ks.alpha;
+            const T alpha = (T)ks.alpha;
+            double *xbest = v.xbest != nullptr && ks.improved != 0 ? v.xbest : nullptr;
+            dd rr[2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+            for (int u = 0; u < kResSlots; ++u) {
+                const unsigned b0 = opq(rg.sl[u].base);
+                const unsigned k = b0 / HXY, jr = b0 - k * HXY, j = jr / (unsigned)HX, m = jr - j * (unsigned)HX;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const size_t c = (h ? HS : 0) + b0;
+                    const T rv = v.r[c] - alpha * tav[u][h];
+                    bv[u][h] = rv;
+                    if (!rg.valid[u][h]) continue;
+                    const size_t cs = (size_t)(2 * m + ((unsigned)h ^ rg.par[u])) + sy * j + sz * k;
+                    const T pv = (h ? w.xs1 : w.xs0)[opq(rg.sl[u].e)];
+                    const double xo = v.th[cs];
+                    rr[0] = dd_add(rr[0], (double)rv * (double)rv);
+                    if (xbest) xbest[cs] = xo;
+                    v.th[cs] = xo + alpha_d * (double)pv;
+                    v.r[c] = rv;
+                }
+            }
+            res_wpartials<1>(rr, red, wr, 0);
+        }
+
+        // ---- z = M⁻¹ r from z = 0; Σ r·z
+        for (unsigned e = tid; e < 2 * hsz; e += kResThreads) w.xs0[e] = T(0);
+        __syncthreads();
+        res_gs5<T, FLUX>(w, v.z, rg, bv, fc);
+        fc += 10u;
+        {
+            // each thread reads back its own cells (half 0 from the last
+            // pass, half 1 from the one before)
+            dd rz[2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+            for (int u = 0; u < kResSlots; ++u) {
+                if (rg.valid[u][0]) rz[0] = dd_add(rz[0], (double)bv[u][0] * (double)w.xs0[rg.sl[u].e]);
+                if (rg.valid[u][1]) rz[0] = dd_add(rz[0], (double)bv[u][1] * (double)w.xs1[rg.sl[u].e]);
+            }
+            res_wpartials<1>(rz, red, wr, 1);
+        }
+        res_grid_sync(pl.bar);
+        double tr[2];
+        res_total<2, true>(wr, 4, gridDim.x, red, tr);
+        if (tid == 0) {
+            finalize(FIN_CG_RES, &tr[0], &ks, blockIdx.x == 0 ? hist : nullptr);  // NonUniform.jl:184-197
+            if (ks.status == ST_RUNNING) finalize(FIN_CG_BETA, &tr[1], &ks, nullptr);  // :205-222
+        }
+        __syncthreads();
+        if (ks.status != ST_RUNNING) break;
+        if (bar_load(pl.bar + kBarErr) != 0u) break;  // a timed-out wait (the host reports it)
+
+        // ---- p = z + β p (NonUniform.jl:218-220): into LDS and HBM, the
+        // rows on the tile's edge with device-scope stores, then published
+        {
+            const T beta = (T)ks.beta;
+#pragma unroll
+            for (int u = 0; u < kResSlots; ++u)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!rg.valid[u][h]) continue;
+                    const unsigned e = opq(rg.sl[u].e);
+                    const size_t c = (h ? HS : 0) + opq(rg.sl[u].base);
+                    T *xh = h ? w.xs1 : w.xs0;
+                    const T pn = xh[e] + beta * v.p[c];
+                    xh[e] = pn;
+                    if (rg.sl[u].edge) sc_store(v.p + c, pn);
+                    else v.p[c] = pn;
+                }
+        }
+        res_publish(pl.bar, fc + 1u);
+        fc += 1u;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid == 0) *st = ks;
+}
+
 template <typename T>
 size_t rb_res_lds(int TJ, int TK, int HX, int K)
 {
@@ -1071,6 +1243,12 @@ hipError_t rb_res_prepare()
                                  (int)kResLdsMax)) != hipSuccess)
         return e;
     if ((e = hipFuncSetAttribute((const void *)k_rb_bicg_solve<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kResLdsMax)) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute((const void *)k_rb_cg_solve<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kResLdsMax)) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute((const void *)k_rb_cg_solve<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)kResLdsMax)) != hipSuccess)
         return e;
     if ((e = res_attr_mode<T, 0>()) != hipSuccess) return e;
@@ -1167,6 +1345,22 @@ hipError_t launch_rb_bicg_solve(const ResBicg<T> &v, const RbGeo<T> &g, const Rb
 }
 
 
+template <typename T>
+hipError_t launch_rb_cg_solve(const ResCg<T> &v, const RbGeo<T> &g, const RbRes &pl, KState *st, double *hist,
+                              hipStream_t s)
+{
+    if (pl.G <= 0 || !pl.bar || !pl.wpart || pl.lds > kResLdsMax) return hipErrorInvalidValue;
+    const ResCoef<T> cf{reinterpret_cast<T *>(pl.dgl), reinterpret_cast<T *>(pl.snk), pl.ix};
+    if (!cf.dgl || !cf.ix || (g.flux && !cf.snk)) return hipErrorInvalidValue;
+    if (!v.p || !v.r || !v.q || !v.z || !v.th) return hipErrorInvalidValue;
+    const dim3 grid(pl.G), block(kResThreads);
+    if (g.flux)
+        hipLaunchKernelGGL((k_rb_cg_solve<T, true>), grid, block, pl.lds, s, g, pl, cf, st, hist, v);
+    else
+        hipLaunchKernelGGL((k_rb_cg_solve<T, false>), grid, block, pl.lds, s, g, pl, cf, st, hist, v);
+    return hipGetLastError();
+}
+
 #define RES_INST(T)                                                                                              \
     template RbRes rb_res_plan<T>(const RbGeo<T> &, int);                                                       \
     template hipError_t rb_res_prepare<T>();                                                                    \
@@ -1178,7 +1372,9 @@ hipError_t launch_rb_bicg_solve(const ResBicg<T> &v, const RbGeo<T> &g, const Rb
                                                      double *, T *, const T *, const RbGeo<T> &, const RbRes &,  \
                                                      KState *, double *, const RedBuf &, hipStream_t);      \
     template hipError_t launch_rb_bicg_solve<T>(const ResBicg<T> &, const RbGeo<T> &, const RbRes &, KState *,   \
-                                                double *, hipStream_t);
+                                                double *, hipStream_t);                                         \
+    template hipError_t launch_rb_cg_solve<T>(const ResCg<T> &, const RbGeo<T> &, const RbRes &, KState *,       \
+                                              double *, hipStream_t);
 
 RES_INST(double)
 RES_INST(float)

@@ -277,8 +277,8 @@ class WorkBuffers:
 
     def set_resident(self, mode):
         """On grids whose iterate fits the CUs' LDS (when the sweep chunk is
-        left at -1): 2 (default) = PBiCGSTAB runs every iteration of a solve
-        in one launch of co-resident workgroups; 1 = the Gauss-Seidel
+        left at -1): 2 (default) = PBiCGSTAB and CG run every iteration of a
+        solve in one launch of co-resident workgroups; 1 = the Gauss-Seidel
         preconditioner (with its neighbours in the iteration) runs as one
         such launch, three launches per PBiCGSTAB iteration; 0 = one launch
         per colour pass.  Bitwise identical."""

@@ -100,11 +100,11 @@ extern "C" {
 #define H3D_OPT_RESIDENT 6     /* Gauss-Seidel preconditioner of the colour-split layout on grids whose
                                   iterate fits the CUs' LDS (240x240x30 in FP64 and below), used when
                                   H3D_OPT_SWEEP_CHUNK is -1 and the grid fits: 2 (default) = as 1, and
-                                  h3d_pbicgstab runs every iteration of a solve in ONE launch of the
-                                  resident grid (in-launch grid barriers for its sums; not with
-                                  H3D_SUM_SEQUENTIAL); 1 = all five sweeps in one launch of co-resident
-                                  workgroups that keep the iterate in LDS (three launches per
-                                  PBiCGSTAB iteration); 0 = one launch per colour pass; bitwise
+                                  h3d_pbicgstab and h3d_cg run every iteration of a solve in ONE launch
+                                  of the resident grid (in-launch grid barriers for their sums; not
+                                  with H3D_SUM_SEQUENTIAL); 1 = all five sweeps in one launch of
+                                  co-resident workgroups that keep the iterate in LDS (three launches
+                                  per PBiCGSTAB iteration); 0 = one launch per colour pass; bitwise
                                   identical */
 #define H3D_OPT_RESIDENT_WGS 7 /* read-only: workgroups of the last solve's resident preconditioner
                                   launches (0 when it did not use one) */

@@ -1,7 +1,8 @@
 """The resident Gauss-Seidel preconditioner (all five red-black sweeps of
 Preconditioner!, src/NonUniform.jl:531-559, in one launch of co-resident
-workgroups holding the iterate in LDS; mode 1) and PBiCGSTAB!'s whole loop
-(src/NonUniform.jl:283-349) in one such launch (mode 2) against the
+workgroups holding the iterate in LDS; mode 1) and the whole loops of
+PBiCGSTAB! and CG! (src/NonUniform.jl:283-349, 160-223) in one such launch
+(mode 2) against the
 per-colour launches (mode 0): whole solves must be bitwise identical
 (iteration count, residual history, θ) for both solvers, steady and
 transient, FP64 and the FP32 refinement, on grids with even and odd row
@@ -38,8 +39,7 @@ def test_resident_solves_bitwise(shape, solver, steady, precision):
     m = h2_amd.build_model(*shape)
     b = run(m, 0, solver, steady, precision, 1e-8)
     assert b[3] == 0
-    modes = (1, 2) if solver == L.SOLVER_PBICGSTAB else (1,)
-    for mode in modes:
+    for mode in (1, 2):
         a = run(m, mode, solver, steady, precision, 1e-8)
         assert a[3] > 0, "the resident preconditioner must have run"
         assert a[0] == b[0], mode
@@ -47,13 +47,14 @@ def test_resident_solves_bitwise(shape, solver, steady, precision):
         assert np.array_equal(a[2], b[2]), mode
 
 
+@pytest.mark.parametrize("solver", [L.SOLVER_PBICGSTAB, L.SOLVER_CG])
 @pytest.mark.parametrize("max_itr", [1, 2, 7])
-def test_resident_one_launch_max_itr(max_itr):
+def test_resident_one_launch_max_itr(solver, max_itr):
     """The one-launch loop stops at maxItr with the state, history and θ the
     per-iteration launches leave."""
     m = h2_amd.build_model(40, 40, 30)
-    a = run(m, 2, L.SOLVER_PBICGSTAB, True, L.FP64, 1e-12, max_itr=max_itr)
-    b = run(m, 0, L.SOLVER_PBICGSTAB, True, L.FP64, 1e-12, max_itr=max_itr)
+    a = run(m, 2, solver, True, L.FP64, 1e-12, max_itr=max_itr)
+    b = run(m, 0, solver, True, L.FP64, 1e-12, max_itr=max_itr)
     assert a[0] == b[0]
     assert len(a[1]) == max_itr
     assert np.array_equal(a[1], b[1])
@@ -61,7 +62,7 @@ def test_resident_one_launch_max_itr(max_itr):
 
 
 @pytest.mark.parametrize("solver,tol,mode", [(L.SOLVER_PBICGSTAB, 1e-4, 1), (L.SOLVER_PBICGSTAB, 1e-4, 2),
-                                             (L.SOLVER_CG, 1e-6, 1)])
+                                             (L.SOLVER_CG, 1e-6, 1), (L.SOLVER_CG, 1e-6, 2)])
 def test_resident_headline_bitwise(solver, tol, mode):
     """modelA 240x240x30, steady, PBiCGSTAB+GS to 1e-4 and CG+GS to 1e-6."""
     print("case resident 240x240x30 solver", solver, "tol", tol, "mode", mode, flush=True)
