@@ -423,6 +423,43 @@ struct ResView {
     unsigned nring;    // its entries
 };
 
+// The tile's view of the launch's dynamic LDS (rb_res_lds gives its size):
+// both colour halves of its rows and ring, the staged tables of its planes,
+// the sums' scratch, then the ring table.
+template <typename T>
+__device__ __forceinline__ ResView<T> res_view(const RbGeo<T> &g, const RbRes &pl, unsigned char *smem)
+{
+    const int HX = g.HX, RJ = pl.TJ + 2;
+    const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);  // one LDS half (+1 pad)
+    const int ntab = 6 * g.K * g.K + g.K;
+    ResView<T> w;
+    w.xs0 = reinterpret_cast<T *>(smem);
+    w.xs1 = w.xs0 + hsz;
+    w.tabs = w.xs1 + hsz;
+    w.bar = pl.bar;
+    w.tl = res_tile(pl, g.NY, g.NZ);
+    w.RJ = RJ;
+    w.HX = HX;
+    w.NX = g.NX;
+    w.NY = g.NY;
+    w.NZ = g.NZ;
+    w.HXY = (unsigned)HX * (unsigned)g.MY;
+    w.zs = (unsigned)(RJ * HX);
+    w.HS = g.HS;
+    const size_t tend = ((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15;
+    w.rt = reinterpret_cast<const uint2 *>(smem + tend + 2 * 68 * sizeof(dd));
+    w.nring = (unsigned)(2 * w.tl.tj + 2 * w.tl.tk) * (unsigned)HX;
+    return w;
+}
+
+// The sums' scratch of the view (2 x 68 double-double words, before the
+// ring table).
+template <typename T>
+__device__ __forceinline__ dd *res_red(const ResView<T> &w)
+{
+    return reinterpret_cast<dd *>(const_cast<uint2 *>(w.rt)) - 2 * 68;
+}
+
 // The per-slot registers of a tile's cells (see k_rb_resident).
 template <typename T>
 struct ResRegs {
@@ -734,32 +771,13 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
     if (halted(st)) return;  // uniform: every workgroup reads the same state
     extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
     const int tid = threadIdx.x;
-    const int HX = g.HX, K = g.K;
-    const int RJ = pl.TJ + 2;
-    const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);  // one LDS half (+1 pad)
-    const int ntab = 6 * K * K + K;
-    ResView<T> w;
-    w.xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
-    w.xs1 = w.xs0 + hsz;
-    w.tabs = w.xs1 + hsz;
-    w.bar = pl.bar;
-    w.tl = res_tile(pl, g.NY, g.NZ);
-    w.RJ = RJ;
-    w.HX = HX;
-    w.NX = g.NX;
-    w.NY = g.NY;
-    w.NZ = g.NZ;
-    w.HXY = (unsigned)HX * (unsigned)g.MY;
-    w.zs = (unsigned)(RJ * HX);
-    w.HS = g.HS;
-    T *tabs = w.xs1 + hsz;
-    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
-                                     (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
-    uint2 *rt = reinterpret_cast<uint2 *>(red + 2 * 68);
-    w.rt = rt;
-    w.nring = (unsigned)(2 * w.tl.tj + 2 * w.tl.tk) * (unsigned)HX;
+    const int ntab = 6 * g.K * g.K + g.K;
+    const ResView<T> w = res_view(g, pl, h3d_dyn_smem);
+    const unsigned hsz = (unsigned)(w.xs1 - w.xs0);  // one LDS half
+    T *tabs = const_cast<T *>(w.tabs);
+    dd *red = res_red(w);
+    uint2 *rt = const_cast<uint2 *>(w.rt);
     const size_t HS = g.HS;
-    const unsigned HXY = w.HXY;
 
     // tables, slots and per-cell constants
     ResRegs<T> rg;
@@ -860,7 +878,6 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
             }
         }
     }
-    (void)HXY;
 }
 
 // The whole PBiCGSTAB! loop (src/NonUniform.jl:283-349, the iterations
@@ -891,30 +908,12 @@ __global__ void __launch_bounds__(kResThreads) k_rb_bicg_solve(RbGeo<T> g, RbRes
     __syncthreads();
     if (ks.status != ST_RUNNING) return;  // uniform
     extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
-    const int HX = g.HX, K = g.K;
-    const int RJ = pl.TJ + 2;
-    const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);
-    const int ntab = 6 * K * K + K;
-    ResView<T> w;
-    w.xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
-    w.xs1 = w.xs0 + hsz;
-    w.tabs = w.xs1 + hsz;
-    w.bar = pl.bar;
-    w.tl = res_tile(pl, g.NY, g.NZ);
-    w.RJ = RJ;
-    w.HX = HX;
-    w.NX = g.NX;
-    w.NY = g.NY;
-    w.NZ = g.NZ;
-    w.HXY = (unsigned)HX * (unsigned)g.MY;
-    w.zs = (unsigned)(RJ * HX);
-    w.HS = g.HS;
-    T *tabs = w.xs1 + hsz;
-    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
-                                     (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
-    uint2 *rt = reinterpret_cast<uint2 *>(red + 2 * 68);
-    w.rt = rt;
-    w.nring = (unsigned)(2 * w.tl.tj + 2 * w.tl.tk) * (unsigned)HX;
+    const int ntab = 6 * g.K * g.K + g.K;
+    const ResView<T> w = res_view(g, pl, h3d_dyn_smem);
+    const unsigned hsz = (unsigned)(w.xs1 - w.xs0);  // one LDS half
+    T *tabs = const_cast<T *>(w.tabs);
+    dd *red = res_red(w);
+    uint2 *rt = const_cast<uint2 *>(w.rt);
     const size_t HS = g.HS;
     ResRegs<T> rg;
     res_setup<T, FLUX>(g, cf, w, ntab, tabs, rt, rg);
@@ -1029,30 +1028,12 @@ __global__ void __launch_bounds__(kResThreads) k_rb_cg_solve(RbGeo<T> g, RbRes p
     __syncthreads();
     if (ks.status != ST_RUNNING) return;  // uniform
     extern __shared__ __align__(16) unsigned char h3d_dyn_smem[];
-    const int HX = g.HX, K = g.K;
-    const int RJ = pl.TJ + 2;
-    const unsigned hsz = (unsigned)(RJ * (pl.TK + 2) * HX + 1);
-    const int ntab = 6 * K * K + K;
-    ResView<T> w;
-    w.xs0 = reinterpret_cast<T *>(h3d_dyn_smem);
-    w.xs1 = w.xs0 + hsz;
-    w.tabs = w.xs1 + hsz;
-    w.bar = pl.bar;
-    w.tl = res_tile(pl, g.NY, g.NZ);
-    w.RJ = RJ;
-    w.HX = HX;
-    w.NX = g.NX;
-    w.NY = g.NY;
-    w.NZ = g.NZ;
-    w.HXY = (unsigned)HX * (unsigned)g.MY;
-    w.zs = (unsigned)(RJ * HX);
-    w.HS = g.HS;
-    T *tabs = w.xs1 + hsz;
-    dd *red = reinterpret_cast<dd *>(h3d_dyn_smem +
-                                     (((2 * (size_t)hsz + (size_t)pl.TK * ntab) * sizeof(T) + 15) & ~(size_t)15));
-    uint2 *rt = reinterpret_cast<uint2 *>(red + 2 * 68);
-    w.rt = rt;
-    w.nring = (unsigned)(2 * w.tl.tj + 2 * w.tl.tk) * (unsigned)HX;
+    const int HX = g.HX, ntab = 6 * g.K * g.K + g.K;
+    const ResView<T> w = res_view(g, pl, h3d_dyn_smem);
+    const unsigned hsz = (unsigned)(w.xs1 - w.xs0);  // one LDS half
+    T *tabs = const_cast<T *>(w.tabs);
+    dd *red = res_red(w);
+    uint2 *rt = const_cast<uint2 *>(w.rt);
     const size_t HS = g.HS;
     ResRegs<T> rg;
     res_setup<T, FLUX>(g, cf, w, ntab, tabs, rt, rg);
