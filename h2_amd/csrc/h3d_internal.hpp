@@ -212,7 +212,8 @@ struct ResCg {
 constexpr size_t kResLdsMax = 150 * 1024;
 constexpr int kResMaxWG = 1024;
 // synchronisation words: an error word, then a pass counter per workgroup
-// (64 bytes apart); the host zeroes them whenever a solve selects a plan
+// (64 bytes apart), then the grid barrier's; the host zeroes them whenever
+// a solve selects a plan
 constexpr int kResBarWords = 64 + 16 * kResMaxWG + 16 * 10;
 constexpr int kResBarErrWord = 0;
 
