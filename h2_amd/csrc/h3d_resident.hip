@@ -2,33 +2,40 @@
 // small grids as ONE launch: the five red-black sweeps (ten colour passes,
 // rbsor_core! at :848-915 with ω = 1) run inside a grid of co-resident
 // workgroups that keep the iterate in LDS from the first pass to the last.
+// The same machinery runs larger units as one launch: the preconditioner
+// with CalcAX! and its sums fused (k_rb_resident), and the whole iteration
+// loops of PBiCGSTAB! and CG! (k_rb_bicg_solve, k_rb_cg_solve).
 //
 // Why: on a grid whose vectors sit in the Infinity Cache (240x240x30 and
 // below) a colour pass launched on its own is latency bound.  Every launch
 // pays the dispatch of two rounds of workgroups, one dependent round trip
 // for x, b and the class bytes, the table staging, and the drain; ten of
-// them per preconditioner are most of a BiCGSTAB iteration.  Here the
-// right-hand side, the diagonal and the conductance table indices of every
-// cell are formed once per launch and held in registers, x lives in LDS,
-// and between passes only the rows a neighbouring tile needs cross HBM.
+// them per preconditioner are most of a BiCGSTAB iteration.  Here each
+// cell's diagonal and conductance table indices (formed once per solve by
+// k_rb_res_coef) and its right-hand side are held in registers, x lives
+// in LDS, and between passes only the rows a neighbouring tile needs cross
+// HBM.
 //
 // Decomposition: the interior rows (j, k) are cut into tiles of TJ x TK
 // rows (RbRes, planned on the host), one 1024-thread workgroup per tile,
 // all resident at once (at most one tile per CU).  LDS holds both colour
 // halves of the tile's rows plus a one-row ring in j and k (the rows the
-// stencil reads from the neighbouring tiles) and the staged conductance
-// tables of the tile's planes.  Thread t owns the slots t, t+1024, ... of
-// the tile's (row, m) index space in both halves.
+// stencil reads from the neighbouring tiles), the staged conductance
+// tables of the tile's planes and a table of the ring cells' addresses.
+// Thread t owns the slots t, t+1024, ... of the tile's (row, m) index
+// space in both halves.
 //
 // A pass updates the cells of one half from LDS into LDS, and stores the
-// rows on the tile's edge into x in HBM (every row in the last sweep); the
-// workgroup then publishes its pass counter.  A pass first updates the
-// rows inside the tile, whose neighbours are all in LDS, while the
-// neighbouring tiles finish the previous pass; then it waits for the
-// counters of the (at most four) tiles whose edge rows it reads, loads
-// those ring rows of the half the previous pass updated, and updates its
-// own edge rows.  No grid-wide barrier: each tile waits only for its
-// neighbours.  Cross-workgroup data moves with device-scope (sc1) loads
+// rows on the tile's edge into x in HBM (and, after the last sweep, the
+// other rows when a caller reads x back from HBM); the workgroup then
+// publishes its pass counter.  A pass first updates the rows inside the
+// tile, whose neighbours are all in LDS, while the neighbouring tiles
+// finish the previous pass; then it waits for the counters of the (at
+// most four) tiles whose edge rows it reads, loads those ring rows of the
+// half the previous pass updated, and updates its own edge rows.  The
+// passes need no grid-wide barrier: each tile waits only for its
+// neighbours (the sums of the fused launches and of the loops do, through
+// res_grid_sync).  Cross-workgroup data moves with device-scope (sc1) loads
 // and stores, which the L2s of the eight XCDs keep coherent without the
 // write-back and invalidate a device-scope release/acquire fence costs;
 // each wave waits for its stores before the counter is published.  Every
