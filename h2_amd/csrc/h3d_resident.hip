@@ -271,9 +271,10 @@ __device__ __forceinline__ void res_load_ring(T *xo, const T *xg, const uint2 *r
     __syncthreads();
 }
 
-// Row r of a tile's tj x tk rows in the order the passes visit them: the
-// rows on the tile's edge (the ones neighbouring tiles read) first, so that
-// their stores are in flight while the inner rows are computed.
+// Row r of a tile's tj x tk rows in slot order: the rows on the tile's edge
+// (the ones neighbouring tiles read) first, so that the cells of a pass's
+// edge phase, which is on the path from tile to tile, take the low slots of
+// every thread and spread over all of them.
 __device__ __forceinline__ void res_row(int r, int tj, int tk, int &jj, int &kk)
 {
     const int ij = max(tj - 2, 0), ik = max(tk - 2, 0);
@@ -556,7 +557,7 @@ __device__ __forceinline__ void res_gs5(const ResView<T> &w, T *x, const ResRegs
                     if (rg.valid[u][h]) {
                         xo[e] = xn;
                         // HBM receives the rows neighbours read, and in the
-                        // last sweep (final values) every row
+                        // last sweep (final values) every row if asked
                         if (phase == 1) sc_store(xg + base, xn);
                         else if (sweep == 4 && all_rows) xg[base] = xn;
                     }
@@ -572,9 +573,10 @@ __device__ __forceinline__ void res_gs5(const ResView<T> &w, T *x, const ResRegs
 }
 
 // CalcAX! (src/NonUniform.jl:454-511) of x, read from LDS (both halves and
-// their rings in place): ax = A x on the tile's cells (not stored when ax is
-// null), tav = the same values for the caller, v[0] += Σ ax·y with y = x (DYX), dy, or ax itself,
-// v[1] += Σ ax·(dz_bb ? bv : dz) when NS = 2.  Every thread must call it.
+// their rings in place): ax = A x on the tile's cells (not stored when ax
+// is null), tav = the same values for the caller, v[0] += Σ ax·y with y = x
+// (DYX), dy, or ax itself, v[1] += Σ ax·(dz_bb ? bv : dz) when NS = 2.
+// Every thread must call it.
 template <typename T, bool FLUX, int NS, bool DYX = false>
 __device__ __forceinline__ void res_calc_ax_core(const ResView<T> &w, const ResRegs<T> &rg,
                                                  const T (&bv)[kResSlots][2], T *ax, const T *dy, const T *dz,
