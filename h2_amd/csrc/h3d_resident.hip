@@ -918,8 +918,9 @@ __global__ void __launch_bounds__(kResThreads) k_rb_resident(int mode, T *__rest
 // rounded totals, so every workgroup holds the same solver state (in LDS);
 // workgroup 0 writes the history and, at the end, the state.  The tables,
 // slots and per-cell constants are set up once per solve instead of twice
-// per iteration.  s lives in registers (it is never stored), q is reused
-// from CalcAX!'s registers.  The three partial buffers of the in-launch
+// per iteration.  s and t live in registers (neither is stored), s_ goes
+// to HBM only on the tiles' edge rows (the update reads it from LDS), and
+// q is reused from CalcAX!'s registers.  The three partial buffers of the in-launch
 // sums (4 doubles per workgroup each) are each reused once per iteration:
 // every workgroup has read one before any can pass the next barrier.
 template <typename T, bool FLUX>
