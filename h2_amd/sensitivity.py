@@ -86,14 +86,47 @@ def halo_lambda(m, lam):
     return lam
 
 
-def conductance_change(m, lam_p, lam_0):
+def changed_box(lam_p, lam_0):
+    """Per axis the index range [lo, hi) (halo coordinates) of the cells
+    where the two λ fields differ, widened by one cell on each side: only
+    faces with a cell in it change their conductance.  None when equal."""
+    diff = np.asarray(lam_p) != np.asarray(lam_0)
+    if not diff.any():
+        return None
+    box = []
+    for ax in range(3):
+        idx = np.nonzero(diff.any(axis=tuple(a for a in range(3) if a != ax)))[0]
+        box.append((max(int(idx[0]) - 1, 0), min(int(idx[-1]) + 2, diff.shape[ax])))
+    return box
+
+
+def conductance_change(m, lam_p, lam_0, box=True):
     """Per face direction, G·(λf(λp) - λf(λ0)) over the interior cells:
     the change of the flux-form face conductances (G the face's area over
-    distance, λf the harmonic mean with factor 2 at the halo)."""
+    distance, λf the harmonic mean with factor 2 at the halo).  Each entry
+    is (nb, sel, dG): nb the neighbour's slice of the full array, sel the
+    interior indices (0-based within the interior) of the nonzero changes
+    and dG their values.  With box, the changes are formed only over the
+    cells of changed_box of the halo λ fields (the same values: every face
+    outside it is unchanged)."""
     NX, NY, NZ = m.NX, m.NY, m.NZ
+    n = (NX, NY, NZ)
     I = (slice(1, NX + 1), slice(1, NY + 1), slice(1, NZ + 1))
     lp, l0 = halo_lambda(m, lam_p), halo_lambda(m, lam_0)
-    dz = m.dZ[None, None, 1:NZ + 1]
+    if box:
+        bx = changed_box(lp, l0)
+        if bx is None:
+            return []
+        # the interior cells of the box, in halo coordinates
+        s = [max(1, bx[a][0]) for a in range(3)]
+        e = [min(n[a] + 1, bx[a][1]) for a in range(3)]
+        if any(e[a] <= s[a] for a in range(3)):
+            return []
+    else:
+        s, e = [1, 1, 1], [NX + 1, NY + 1, NZ + 1]
+    C = tuple(slice(s[a], e[a]) for a in range(3))
+    shp = tuple(e[a] - s[a] for a in range(3))
+    dz = m.dZ[None, None, s[2]:e[2]]
     zc = m.ZC
     out = []
     for ax in range(3):
@@ -101,18 +134,23 @@ def conductance_change(m, lam_p, lam_0):
             nb = list(I)
             nb[ax] = slice(I[ax].start + d, I[ax].stop + d)
             nb = tuple(nb)
-            n_ax = (NX, NY, NZ)[ax]
-            inner = np.ones((NX, NY, NZ), dtype=bool)
-            edge = [slice(None)] * 3
-            edge[ax] = 0 if d < 0 else n_ax - 1
-            inner[tuple(edge)] = False
+            cn = list(C)
+            cn[ax] = slice(C[ax].start + d, C[ax].stop + d)
+            cn = tuple(cn)
+            inner = np.ones(shp, dtype=bool)
+            # the face on the domain's edge (the halo neighbour), if in C
+            edge_at = 1 if d < 0 else n[ax]
+            if s[ax] <= edge_at < e[ax]:
+                edge = [slice(None)] * 3
+                edge[ax] = edge_at - s[ax]
+                inner[tuple(edge)] = False
             fac = np.where(inner, 1.0, 2.0)
             if ax == 0:
                 G = m.dy * dz / m.dx
             elif ax == 1:
                 G = m.dx * dz / m.dy
             else:
-                k = np.arange(1, NZ + 1)
+                k = np.arange(s[2], e[2])
                 G = (m.dx * m.dy / np.abs(zc[k + d] - zc[k]))[None, None, :]
 
             def lamf(a, b):
@@ -120,9 +158,11 @@ def conductance_change(m, lam_p, lam_0):
                     v = 2.0 * a * b / (a + b) * fac
                 return np.where(a + b > 0, v, 0.0)
 
-            dG = G * (lamf(lp[nb], lp[I]) - lamf(l0[nb], l0[I]))
+            dG = G * (lamf(lp[cn], lp[C]) - lamf(l0[cn], l0[C]))
             sel = np.nonzero(dG)
-            out.append((nb, sel, dG[sel]))
+            vals = dG[sel]
+            sel = tuple(sel[a] + (s[a] - 1) for a in range(3))
+            out.append((nb, sel, vals))
     return out
 
 

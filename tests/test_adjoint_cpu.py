@@ -90,3 +90,64 @@ def test_adjoint_matches_full_central_differences(problem):
             return ones, hs
         fd = richardson_fd(prob, beta, perturb, 1e-2) / prob.hsrc[ci]
         assert abs(adj["dJ_dhsrc"][ci] - fd) <= 2e-6 * abs(fd), (ci, adj["dJ_dhsrc"][ci], fd)
+
+
+def _theta_with_case_halo(c, rng):
+    """A random interior θ (250-350 K) with the halo the oracle case sets."""
+    th = c.arr["theta"].copy(order="F")
+    th[1:-1, 1:-1, 1:-1] = 250.0 + 100.0 * rng.random(th[1:-1, 1:-1, 1:-1].shape)
+    return th
+
+
+def test_operator_change_matches_oracle_calc_ax(problem):
+    """(A(λp) - A(λ0))·θ of the flux form (operator_change, the change the
+    transient central difference applies) against the difference of the
+    oracle's CalcAX! of the scaled and unscaled cases on the same θ, for
+    every group the case holds: the same within rounding (the convective
+    and ap0 terms cancel)."""
+    OA, prob, m = problem
+    for k in [k for k in range(S.NGROUPS) if np.any(prob.groups == k)]:
+        _check_operator_change(prob, m, k, 1.5 if k % 2 else 0.6)
+
+
+def _check_operator_change(prob, m, k, s):
+    ones = np.ones(S.NGROUPS)
+    sc = ones.copy()
+    sc[k] = s
+    c0, cp = prob.case(ones), prob.case(sc)
+    th = _theta_with_case_halo(c0, np.random.default_rng(k))
+    assert np.array_equal(c0.arr["theta"][0], cp.arr["theta"][0])
+    a0 = np.zeros(th.shape, order="F")
+    ap = np.zeros(th.shape, order="F")
+    c0.calc_ax(a0, th)
+    cp.calc_ax(ap, th)
+    groups = S.layer_groups(m)
+    lam0 = np.array(m.lam, order="F")
+    assert np.array_equal(lam0[1:-1, 1:-1, 1:-1], prob.lam0[1:-1, 1:-1, 1:-1])
+    d = S.operator_change(m, S.scaled_lambda(m, groups, sc), lam0, th)
+    want = (ap - a0)[1:-1, 1:-1, 1:-1]
+    got = d[1:-1, 1:-1, 1:-1]
+    assert np.max(np.abs(want)) > 0, k
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(ap)), k
+
+
+@pytest.mark.parametrize("k", [0, 5, 8, 11])
+def test_conductance_change_box_equals_full(problem, k):
+    """The changes formed over the changed box only are bitwise those over
+    the whole interior (entries and their application to θ)."""
+    OA, prob, m = problem
+    sc = np.ones(S.NGROUPS)
+    sc[k] = 1.0 + 1e-6
+    groups = S.layer_groups(m)
+    lam0 = np.array(m.lam, order="F")
+    lam = S.scaled_lambda(m, groups, sc)
+    full = S.conductance_change(m, lam, lam0, box=False)
+    boxed = S.conductance_change(m, lam, lam0)
+    th = _theta_with_case_halo(prob.case(np.ones(S.NGROUPS)), np.random.default_rng(7))
+    assert np.array_equal(S.apply_conductance_change(m, full, th), S.apply_conductance_change(m, boxed, th))
+    assert len(full) == len(boxed) == 6
+    for (nb_f, sel_f, v_f), (nb_b, sel_b, v_b) in zip(full, boxed):
+        assert nb_f == nb_b
+        assert all(np.array_equal(a, b) for a, b in zip(sel_f, sel_b))
+        assert np.array_equal(v_f, v_b)
+    assert S.conductance_change(m, lam0, lam0) == []

@@ -25,6 +25,7 @@ from h2_amd.transient import TransientRun
 FD_GROUP = 8
 
 
+@pytest.mark.timeout(480)  # about 95 s on one MI355X
 def test_c5_two_steps_adjoint_and_central_difference():
     print("case 960x960x120 power map, 2 steps of 1 ms, FP32-refined CG+GS(5), checkpoints every 2", flush=True)
     t0 = time.perf_counter()
