@@ -200,6 +200,173 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
         if (valid[a]) stb(xo, rr[a] * (unsigned)sizeof(T), xv[a]);
 }
 
+// The per-colour sweep with two neighbouring cells of a row per thread
+// (m = mb + 2l and mb + 2l + 1 of a 128-wide tile, one wave per row): each
+// load of x, b and the class bytes covers both cells in one instruction
+// (the cells are contiguous in every array), and the x neighbour that lies
+// in the next thread's pair comes from that thread by a DPP lane shift, or
+// for the wave's end thread from one more load.  Every cell's update is
+// k_rb_sweep's, so the results are bitwise the same.
+constexpr int kWideM = 2 * BX;
+constexpr int kDppWaveShr1 = 0x138;  // lane l reads lane l - 1 (lane 0: old)
+constexpr int kDppWaveShl1 = 0x130;  // lane l reads lane l + 1 (lane 63: old)
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_shift(T v, T old)
+{
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+        const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, CTRL, 0xf, 0xf,
+                                                                  false);
+        const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32),
+                                                                  CTRL, 0xf, 0xf, false);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+    }
+}
+
+// elements e and e + 1 (alignment: one element)
+template <typename T>
+__device__ __forceinline__ void ld_pair(const T *__restrict__ p, unsigned e, T &a, T &b)
+{
+    typedef T v2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+    const v2 v = *reinterpret_cast<const v2 *>(p + e);
+    a = v.x;
+    b = v.y;
+}
+
+template <typename T>
+__device__ __forceinline__ void st_pair(T *__restrict__ p, unsigned e, T a, T b)
+{
+    typedef T v2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+    v2 v;
+    v.x = a;
+    v.y = b;
+    *reinterpret_cast<v2 *>(p + e) = v;
+}
+
+__device__ __forceinline__ void ld_cls_pair(const uint8_t *__restrict__ p, unsigned e, int &a, int &b)
+{
+    uint16_t v;
+    __builtin_memcpy(&v, p + e, 2);
+    a = v & 255;
+    b = v >> 8;
+}
+
+template <typename T>
+__device__ __forceinline__ T gs_cell(const RbGeo<T> &g, const T *stab, const Cls &q, const Nb<T> &n, T pp, T bc,
+                                     int i, int j, int k)
+{
+    const Cond<T> o = rb_cond(g, stab, q, i, j, k);
+    const T dd = diag(o);
+    if (g.flux) return pp + (rb_flux(o, n, pp) - bc) / dd;
+    const T dp = (((rb_offdiag(o, n) - bc) / dd - pp)) * T(1);
+    return pp + T(1) * dp;
+}
+
+template <typename T, bool ZERO>
+__global__ void __launch_bounds__(256) k_rb_sweep_w(T *__restrict__ xo, const T *__restrict__ xt,
+                                                    const T *__restrict__ bo, const uint8_t *__restrict__ clo,
+                                                    const uint8_t *__restrict__ clt, RbGeo<T> g, int h,
+                                                    const KState *st)
+{
+    if (halted(st)) return;
+    const unsigned gx = ((unsigned)g.HX + kWideM - 1) / kWideM, gy = ((unsigned)g.NY + BY - 1) / BY;
+    const unsigned gz = (unsigned)g.NZ, nreal = gx * gy * gz, nb = (nreal + 7u) & ~7u;
+    const unsigned p = blockIdx.x;
+    const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
+    const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
+    const int HXi = g.HX;
+    const int mb = (int)bx * kWideM, m0 = mb + 2 * (int)threadIdx.x;
+    const int jr = (int)by * BY + (int)threadIdx.y + 1;
+    const int k = (int)min(bz, gz - 1u) + 1;
+    const bool rowok = L < nreal && jr <= g.NY;
+    const int j = min(jr, g.NY);
+    // the pair (mc, mc + 1): mc + 1 may be the next row's first element,
+    // which is in the array for every row and plane the loads touch
+    const int mc = min(m0, HXi - 1);
+    const unsigned HX = (unsigned)HXi, HXY = HX * (unsigned)g.MY;
+    const unsigned e = (unsigned)mc + HX * (unsigned)j + HXY * (unsigned)k;
+    const int s = (h + j + k) & 1;
+    // the x neighbour outside the wave's pairs: m = mb - 1 for the first
+    // thread (s = 0), mb + 128 for the last (s = 1), clamped into the row
+    // (a clamped one belongs to a cell outside the interior)
+    const int me = s ? min(mb + kWideM, HXi - 1) : max(mb - 1, 0);
+    const unsigned ee = (unsigned)me + HX * (unsigned)j + HXY * (unsigned)k;
+    T p0, p1, ym0, ym1, yp0, yp1, zm0, zm1, zp0, zp1, pp0 = T(0), pp1 = T(0), b0, b1;
+    ld_pair(xt, e, p0, p1);
+    ld_pair(xt, e - HX, ym0, ym1);
+    ld_pair(xt, e + HX, yp0, yp1);
+    ld_pair(xt, e - HXY, zm0, zm1);
+    ld_pair(xt, e + HXY, zp0, zp1);
+    if (!ZERO) ld_pair(xo, e, pp0, pp1);
+    ld_pair(bo, e, b0, b1);
+    const T xe = xt[ee];
+    int c0, c1, n0, n1, cym0, cym1, cyp0, cyp1, czm0, czm1, czp0, czp1;
+    ld_cls_pair(clo, e, c0, c1);
+    ld_cls_pair(clt, e, n0, n1);
+    ld_cls_pair(clt, e - HX, cym0, cym1);
+    ld_cls_pair(clt, e + HX, cyp0, cyp1);
+    ld_cls_pair(clt, e - HXY, czm0, czm1);
+    ld_cls_pair(clt, e + HXY, czp0, czp1);
+    const int ce = clt[ee];
+    const T *stab = rb_stage(g, k);
+    // x neighbours: cell A = m0 has other[m0 - 1 + s], other[m0 + s], cell
+    // B = m0 + 1 has other[m0 + s], other[m0 + 1 + s]
+    const T sr = dpp_shift<kDppWaveShr1>(p1, xe);  // other[m0 - 1]
+    const T sl = dpp_shift<kDppWaveShl1>(p0, xe);  // other[m0 + 2]
+    const int csr = dpp_shift<kDppWaveShr1>(n1, ce), csl = dpp_shift<kDppWaveShl1>(n0, ce);
+    Nb<T> na, nbb;
+    Cls qa, qb;
+    na.xm = s ? p0 : sr;
+    na.xp = s ? p1 : p0;
+    nbb.xm = s ? p1 : p0;
+    nbb.xp = s ? sl : p1;
+    na.ym = ym0;
+    nbb.ym = ym1;
+    na.yp = yp0;
+    nbb.yp = yp1;
+    na.zm = zm0;
+    nbb.zm = zm1;
+    na.zp = zp0;
+    nbb.zp = zp1;
+    qa.c = c0;
+    qb.c = c1;
+    qa.xm = s ? n0 : csr;
+    qa.xp = s ? n1 : n0;
+    qb.xm = s ? n1 : n0;
+    qb.xp = s ? csl : n1;
+    qa.ym = cym0;
+    qb.ym = cym1;
+    qa.yp = cyp0;
+    qb.yp = cyp1;
+    qa.zm = czm0;
+    qb.zm = czm1;
+    qa.zp = czp0;
+    qb.zp = czp1;
+    const int ia = 2 * m0 + s, ib = ia + 2;
+    const bool va = rowok && m0 < HXi && ia >= 1 && ia <= g.NX;
+    const bool vb = rowok && m0 + 1 < HXi && ib >= 1 && ib <= g.NX;
+    const T xa = gs_cell(g, stab, qa, na, pp0, b0, ia, j, k);
+    const T xb = gs_cell(g, stab, qb, nbb, pp1, b1, ib, j, k);
+    if (va && vb) {
+        st_pair(xo, e, xa, xb);
+    } else {
+        if (va) xo[e] = xa;
+        if (vb) xo[e + 1] = xb;
+    }
+}
+
+template <typename T>
+static unsigned rb_wide_nb(const RbGeo<T> &g)
+{
+    const unsigned gx = ((unsigned)g.HX + kWideM - 1) / kWideM, gy = ((unsigned)g.NY + BY - 1) / BY;
+    return (gx * gy * (unsigned)g.NZ + 7u) & ~7u;
+}
+
 // Rows per thread of the sweep, by precision and grid size (measured, HIP
 // events, one full sweep): FP64 one row on small grids (17.7 against
 // 18.4 us at 240x240x30), two on large ones (91.0 against 95.8 us at
@@ -208,7 +375,9 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
 // (12.2 against 13.4 us with four), three on large ones (51.9 against
 // 58.4 us with four at 480x480x60, 548 against 615 us at 960x960x120:
 // 73-77 VGPRs keep six waves per SIMD, four rows' 97 only four).  Large:
-// at least 2^21 cells per half.
+// at least 2^21 cells per half.  On large grids the FP64 sweep itself runs
+// in the two-cell form (k_rb_sweep_w); the rows form serves its pass with
+// the sum Σ b·x.
 template <typename T>
 constexpr int rb_rows_large()
 {
@@ -238,8 +407,23 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
 {
     constexpr int R = sizeof(T) == 4 ? 2 : 1;
     const int h = 1 - color;
-    if (g.HS >= ((size_t)1 << 21)) rb_sweep_rows<T, rb_rows_large<T>()>(x, b, g, h, from_zero, st, s);
-    else rb_sweep_rows<T, R>(x, b, g, h, from_zero, st, s);
+    // large grids: FP64 in the two-cell form (measured, HIP events, both
+    // colours: 75.9 against 84.0 us at 480x480x60, 880 against 1019 us at
+    // 960x960x120 for two rows per thread), FP32 in rows (58.1 against
+    // 51.5 us and 555 against 536 us in the two-cell form)
+    if (sizeof(T) == 8 && g.HS >= ((size_t)1 << 21)) {
+        const size_t own = h ? g.HS : 0, oth = h ? 0 : g.HS;
+        if (from_zero)
+            hipLaunchKernelGGL((k_rb_sweep_w<T, true>), dim3(rb_wide_nb(g)), dim3(BX, BY), rb_lds(g), s, x + own,
+                               x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+        else
+            hipLaunchKernelGGL((k_rb_sweep_w<T, false>), dim3(rb_wide_nb(g)), dim3(BX, BY), rb_lds(g), s, x + own,
+                               x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+    } else if (g.HS >= ((size_t)1 << 21)) {
+        rb_sweep_rows<T, rb_rows_large<T>()>(x, b, g, h, from_zero, st, s);
+    } else {
+        rb_sweep_rows<T, R>(x, b, g, h, from_zero, st, s);
+    }
     return hipGetLastError();
 }
 
@@ -531,16 +715,17 @@ bool rb_sweep2_fits(const RbGeo<T> &g)
 
 // Measured on one MI355X (HIP events, one full sweep, 64 x 8 footprint):
 // at 960x960x120 (transient) the two-colour launches with 60-plane chunks
-// take 933 us in FP64 and 520 us in FP32, against 1079 and 559 us for the
-// per-colour launches (40-plane chunks: 951 and 522 us; 120: 929 and
-// 527 us); at 480x480x60 and 240x240x30 every chunk length was slower
-// (best 132 against 96 us and 69 against 58 us at 480x480x60).
+// take 933 us in FP64 and 520 us in FP32, against 880 us (two-cell form)
+// and 536-559 us for the per-colour launches (40-plane chunks: 951 and
+// 522 us; 120: 929 and 527 us); at 480x480x60 and 240x240x30 every chunk
+// length was slower (best 132 against 96 us and 69 against 58 us at
+// 480x480x60).  So FP32 operators of 120 planes and more take 60-plane
+// chunks, the rest one launch per colour.
 int sweep_chunk_auto(int HX, int NY, int NZ, int elem_bytes)
 {
     (void)HX;
     (void)NY;
-    (void)elem_bytes;
-    return NZ >= 120 ? 60 : 0;
+    return NZ >= 120 && elem_bytes == 4 ? 60 : 0;
 }
 
 // The Gauss-Seidel value of a cell whose old value and neighbours are all
@@ -737,6 +922,130 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T 
             if (valid[r][a]) stb(ax, co[r][a] * sz, av[r][a]);
 }
 
+// CalcAX! in the two-cell form of k_rb_sweep_w: a thread takes the four
+// consecutive cells i = 2m0 .. 2m0 + 3 of a row (m0 = mb + 2l of a
+// 128-wide tile, one wave per row), two in each half, so that each load of
+// x, dy, dz and the class bytes covers two cells and the x neighbours at
+// the pairs' ends come from the next threads by DPP lane shifts (or, at
+// the wave's ends, from one more load each).  Every cell's value is
+// k_rb_calc_ax's and the sums are exactly rounded, so the results are
+// bitwise the same.
+template <typename T, int NS>
+__global__ void __launch_bounds__(256) k_rb_calc_ax_w(T *__restrict__ ax, const T *__restrict__ x,
+                                                      const T *__restrict__ dy, const T *__restrict__ dz,
+                                                      RbGeo<T> g, KState *st, RedBuf rb)
+{
+    if (halted(st)) return;
+    const unsigned gx = ((unsigned)g.HX + kWideM - 1) / kWideM, gy = ((unsigned)g.NY + BY - 1) / BY;
+    const unsigned gz = (unsigned)g.NZ, nreal = gx * gy * gz, nb = (nreal + 7u) & ~7u;
+    const unsigned p = blockIdx.x;
+    const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
+    const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
+    const int HXi = g.HX;
+    const int mb = (int)bx * kWideM, m0 = mb + 2 * (int)threadIdx.x;
+    const int jr = (int)by * BY + (int)threadIdx.y + 1;
+    const int k = (int)min(bz, gz - 1u) + 1;
+    const bool rowok = L < nreal && jr <= g.NY;
+    const int j = min(jr, g.NY);
+    const int mc = min(m0, HXi - 1);  // as in k_rb_sweep_w
+    const unsigned HX = (unsigned)HXi, HXY = HX * (unsigned)g.MY;
+    const unsigned e = (unsigned)mc + HX * (unsigned)j + HXY * (unsigned)k;
+    const int h0 = (j + k) & 1;
+    // half A holds the cells i = 2m0 and 2m0 + 2, half B i = 2m0 + 1, 2m0 + 3
+    const size_t oA = h0 ? g.HS : 0, oB = h0 ? 0 : g.HS;
+    const T *xA = x + oA, *xB = x + oB;
+    const uint8_t *cA = g.cls + oA, *cB = g.cls + oB;
+    // the x neighbours outside the wave's pairs: B at mb - 1 (first thread),
+    // A at mb + 128 (last thread), clamped into the row (a clamped one
+    // belongs to a cell outside the interior)
+    const unsigned eB = (unsigned)max(mb - 1, 0) + HX * (unsigned)j + HXY * (unsigned)k;
+    const unsigned eA = (unsigned)min(mb + kWideM, HXi - 1) + HX * (unsigned)j + HXY * (unsigned)k;
+    T A0, A1, B0, B1;
+    Nb<T> n[4];
+    ld_pair(xA, e, A0, A1);
+    ld_pair(xB, e, B0, B1);
+    // y and z neighbours: those of A's cells are in B, those of B's in A
+    ld_pair(xB, e - HX, n[0].ym, n[2].ym);
+    ld_pair(xB, e + HX, n[0].yp, n[2].yp);
+    ld_pair(xB, e - HXY, n[0].zm, n[2].zm);
+    ld_pair(xB, e + HXY, n[0].zp, n[2].zp);
+    ld_pair(xA, e - HX, n[1].ym, n[3].ym);
+    ld_pair(xA, e + HX, n[1].yp, n[3].yp);
+    ld_pair(xA, e - HXY, n[1].zm, n[3].zm);
+    ld_pair(xA, e + HXY, n[1].zp, n[3].zp);
+    const T xeB = xB[eB], xeA = xA[eA];
+    T dyv[4] = {T(0), T(0), T(0), T(0)}, dzv[4] = {T(0), T(0), T(0), T(0)};
+    if (NS >= 1 && dy) {
+        ld_pair(dy + oA, e, dyv[0], dyv[2]);
+        ld_pair(dy + oB, e, dyv[1], dyv[3]);
+    }
+    if (NS >= 2) {
+        ld_pair(dz + oA, e, dzv[0], dzv[2]);
+        ld_pair(dz + oB, e, dzv[1], dzv[3]);
+    }
+    Cls q[4];
+    ld_cls_pair(cA, e, q[0].c, q[2].c);
+    ld_cls_pair(cB, e, q[1].c, q[3].c);
+    ld_cls_pair(cB, e - HX, q[0].ym, q[2].ym);
+    ld_cls_pair(cB, e + HX, q[0].yp, q[2].yp);
+    ld_cls_pair(cB, e - HXY, q[0].zm, q[2].zm);
+    ld_cls_pair(cB, e + HXY, q[0].zp, q[2].zp);
+    ld_cls_pair(cA, e - HX, q[1].ym, q[3].ym);
+    ld_cls_pair(cA, e + HX, q[1].yp, q[3].yp);
+    ld_cls_pair(cA, e - HXY, q[1].zm, q[3].zm);
+    ld_cls_pair(cA, e + HXY, q[1].zp, q[3].zp);
+    const int ceB = cB[eB], ceA = cA[eA];
+    const T *stab = rb_stage(g, k);
+    // x neighbours (cell c = i - 2m0): 0: B[m0 - 1], B0; 1: A0, A1;
+    // 2: B0, B1; 3: A1, A[m0 + 2]
+    n[0].xm = dpp_shift<kDppWaveShr1>(B1, xeB);
+    n[0].xp = B0;
+    n[1].xm = A0;
+    n[1].xp = A1;
+    n[2].xm = B0;
+    n[2].xp = B1;
+    n[3].xm = A1;
+    n[3].xp = dpp_shift<kDppWaveShl1>(A0, xeA);
+    q[0].xm = dpp_shift<kDppWaveShr1>(q[3].c, ceB);
+    q[0].xp = q[1].c;
+    q[1].xm = q[0].c;
+    q[1].xp = q[2].c;
+    q[2].xm = q[1].c;
+    q[2].xp = q[3].c;
+    q[3].xm = q[2].c;
+    q[3].xp = dpp_shift<kDppWaveShl1>(q[0].c, ceA);
+    const T own[4] = {A0, B0, A1, B1};
+    T av[4];
+    bool valid[4];
+    dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int i = 2 * m0 + c;
+        valid[c] = rowok && m0 + (c >> 1) < HXi && i >= 1 && i <= g.NX;
+        const Cond<T> cd = rb_cond(g, stab, q[c], i, j, k);
+        av[c] = rb_apply(g, cd, n[c], own[c]);
+        const T yv = (NS >= 1 && dy) ? dyv[c] : av[c];
+        if (valid[c]) {
+            if (NS >= 1) v[0] = dd_add(v[0], (double)av[c] * (double)yv);
+            if (NS >= 2) v[1] = dd_add(v[1], (double)av[c] * (double)dzv[c]);
+        }
+    }
+    if (NS > 0) block_partials<NS>(v, rb);
+    T *aA = ax + oA, *aB = ax + oB;
+    if (valid[0] && valid[2]) {
+        st_pair(aA, e, av[0], av[2]);
+    } else {
+        if (valid[0]) aA[e] = av[0];
+        if (valid[2]) aA[e + 1] = av[2];
+    }
+    if (valid[1] && valid[3]) {
+        st_pair(aB, e, av[1], av[3]);
+    } else {
+        if (valid[1]) aB[e] = av[1];
+        if (valid[3]) aB[e + 1] = av[3];
+    }
+}
+
 // One row per thread: two rows measured slower at 960x960x120 (888 against
 // 838 µs for CalcAX!, 1013 against 975 µs with its dot) and equal at
 // 240x240x30.
@@ -748,9 +1057,17 @@ hipError_t launch_rb_calc_ax(T *ax, const T *x, const RbGeo<T> &g, const T *dot_
 {
     constexpr int R = kAxRows;
     const int ns = fin < 0 ? 0 : (dot_z ? 2 : 1);
-    const unsigned nb = rb_rows_nb<R>(g);
+    // large grids in FP64: the two-cell form
+    const bool wide = sizeof(T) == 8 && g.HS >= ((size_t)1 << 21);
+    const unsigned nb = wide ? rb_wide_nb(g) : rb_rows_nb<R>(g);
     const dim3 grid(nb), block(BX, BY);
-    if (ns == 0)
+    if (wide && ns == 0)
+        hipLaunchKernelGGL((k_rb_calc_ax_w<T, 0>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+    else if (wide && ns == 1)
+        hipLaunchKernelGGL((k_rb_calc_ax_w<T, 1>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+    else if (wide)
+        hipLaunchKernelGGL((k_rb_calc_ax_w<T, 2>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+    else if (ns == 0)
         hipLaunchKernelGGL((k_rb_calc_ax<T, 0, R>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     else if (ns == 1)
         hipLaunchKernelGGL((k_rb_calc_ax<T, 1, R>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);

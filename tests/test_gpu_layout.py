@@ -71,10 +71,25 @@ def run_limited(shape, layout, solver, steady, precision, max_itr, chunk=None):
                                                      (L.SOLVER_CG, False, L.FP32_REFINED)])
 def test_large_grid_layouts_bitwise(solver, steady, precision):
     """260x260x60 has 2.13 M cells per colour half, where the colour-split
-    sweep takes more rows per thread; 30 iterations of each path."""
+    sweep takes its large-grid forms (FP64 two cells per thread, 131
+    half-row indices, so a row ends inside a pair; FP32 three rows per
+    thread); 30 iterations of each path."""
     print("case 260x260x60 solver", solver, "steady", steady, "precision", precision, flush=True)
     a = run_limited((260, 260, 60), L.LAYOUT_AUTO, solver, steady, precision, 30)
     b = run_limited((260, 260, 60), L.LAYOUT_STANDARD, solver, steady, precision, 30)
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("solver,steady", [(L.SOLVER_PBICGSTAB, True), (L.SOLVER_CG, False)])
+def test_large_grid_odd_extent_bitwise(solver, steady):
+    """257x270x60 (2.19 M cells per half, NX odd, an even number of
+    half-row indices): the FP64 sweep's two-cell form with a pair half
+    outside the interior at the row's end; 30 iterations."""
+    print("case 257x270x60 solver", solver, "steady", steady, flush=True)
+    a = run_limited((257, 270, 60), L.LAYOUT_AUTO, solver, steady, L.FP64, 30)
+    b = run_limited((257, 270, 60), L.LAYOUT_STANDARD, solver, steady, L.FP64, 30)
     assert a[0] == b[0]
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])
