@@ -207,7 +207,6 @@ __global__ void __launch_bounds__(256) k_rb_sweep(T *__restrict__ xo, const T *_
 // in the next thread's pair comes from that thread by a DPP lane shift, or
 // for the wave's end thread from one more load.  Every cell's update is
 // k_rb_sweep's, so the results are bitwise the same.
-constexpr int kWideM = 2 * BX;
 constexpr int kDppWaveShr1 = 0x138;  // lane l reads lane l - 1 (lane 0: old)
 constexpr int kDppWaveShl1 = 0x130;  // lane l reads lane l + 1 (lane 63: old)
 
@@ -267,20 +266,186 @@ __device__ __forceinline__ T gs_cell(const RbGeo<T> &g, const T *stab, const Cls
     return pp + T(1) * dp;
 }
 
-template <typename T, bool ZERO>
+// NC elements from e on (alignment: one element)
+template <int NC, typename T>
+__device__ __forceinline__ void ld_vec(const T *__restrict__ p, unsigned e, T (&o)[NC])
+{
+    typedef T vn __attribute__((ext_vector_type(NC), aligned(sizeof(T))));
+    const vn v = *reinterpret_cast<const vn *>(p + e);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o[c] = v[c];
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void st_vec(T *__restrict__ p, unsigned e, const T (&a)[NC])
+{
+    typedef T vn __attribute__((ext_vector_type(NC), aligned(sizeof(T))));
+    vn v;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = a[c];
+    *reinterpret_cast<vn *>(p + e) = v;
+}
+
+template <int NC>
+__device__ __forceinline__ void ld_cls_vec(const uint8_t *__restrict__ p, unsigned e, int (&o)[NC])
+{
+    if constexpr (NC == 4) {
+        uint32_t v;
+        __builtin_memcpy(&v, p + e, 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c] = (int)((v >> (8 * c)) & 255u);
+    } else {
+        uint16_t v;
+        __builtin_memcpy(&v, p + e, 2);
+        o[0] = v & 255;
+        o[1] = v >> 8;
+    }
+}
+
+// The per-colour sweep with NC neighbouring cells of a row per thread
+// (m = mb + NC·l ... of an NC·64-wide tile, one wave per row): each load of
+// x, b and the class bytes covers the NC cells in one instruction (they are
+// contiguous in every array), and the x neighbour that lies in the next
+// thread's cells comes from that thread by a DPP lane shift, or for the
+// wave's end thread from one more load.  Every cell's update is
+// k_rb_sweep's, so the results are bitwise the same.  DOT: also Σ b·x over
+// both halves (the pass of launch_rb_sweep_dot): the thread's new cells
+// and the other half's cells at the same offsets (its x-neighbour values),
+// with bt = b's other half.
+template <typename T, int NC, bool ZERO, bool DOT = false>
 __global__ void __launch_bounds__(256) k_rb_sweep_w(T *__restrict__ xo, const T *__restrict__ xt,
                                                     const T *__restrict__ bo, const uint8_t *__restrict__ clo,
                                                     const uint8_t *__restrict__ clt, RbGeo<T> g, int h,
-                                                    const KState *st)
+                                                    const KState *st, const T *__restrict__ bt = nullptr,
+                                                    RedBuf rbuf = RedBuf{})
 {
     if (halted(st)) return;
-    const unsigned gx = ((unsigned)g.HX + kWideM - 1) / kWideM, gy = ((unsigned)g.NY + BY - 1) / BY;
+    constexpr int TW = NC * BX;  // the tile's half-row indices
+    const unsigned gx = ((unsigned)g.HX + TW - 1) / TW, gy = ((unsigned)g.NY + BY - 1) / BY;
     const unsigned gz = (unsigned)g.NZ, nreal = gx * gy * gz, nb = (nreal + 7u) & ~7u;
     const unsigned p = blockIdx.x;
     const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
     const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
     const int HXi = g.HX;
-    const int mb = (int)bx * kWideM, m0 = mb + 2 * (int)threadIdx.x;
+    const int mb = (int)bx * TW, m0 = mb + NC * (int)threadIdx.x;
+    const int jr = (int)by * BY + (int)threadIdx.y + 1;
+    const int k = (int)min(bz, gz - 1u) + 1;
+    const bool rowok = L < nreal && jr <= g.NY;
+    const int j = min(jr, g.NY);
+    // the cells mc .. mc + NC - 1: past the row's end they run into the next
+    // row, which is in the array for every row and plane the loads touch
+    const int mc = min(m0, HXi - 1);
+    const unsigned HX = (unsigned)HXi, HXY = HX * (unsigned)g.MY;
+    const unsigned e = (unsigned)mc + HX * (unsigned)j + HXY * (unsigned)k;
+    const int s = (h + j + k) & 1;
+    // the x neighbour outside the wave's cells: m = mb - 1 for the first
+    // thread (s = 0), mb + TW for the last (s = 1), clamped into the row
+    // (a clamped one belongs to a cell outside the interior)
+    const int me = s ? min(mb + TW, HXi - 1) : max(mb - 1, 0);
+    const unsigned ee = (unsigned)me + HX * (unsigned)j + HXY * (unsigned)k;
+    T P[NC], ym[NC], yp[NC], zm[NC], zp[NC], pp[NC], bv[NC], bp[NC];
+    ld_vec<NC>(xt, e, P);
+    ld_vec<NC>(xt, e - HX, ym);
+    ld_vec<NC>(xt, e + HX, yp);
+    ld_vec<NC>(xt, e - HXY, zm);
+    ld_vec<NC>(xt, e + HXY, zp);
+    if (ZERO) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) pp[c] = T(0);
+    } else {
+        ld_vec<NC>(xo, e, pp);
+    }
+    ld_vec<NC>(bo, e, bv);
+    if (DOT) ld_vec<NC>(bt, e, bp);
+    const T xe = xt[ee];
+    int co[NC], cn[NC], cym[NC], cyp[NC], czm[NC], czp[NC];
+    ld_cls_vec<NC>(clo, e, co);
+    ld_cls_vec<NC>(clt, e, cn);
+    ld_cls_vec<NC>(clt, e - HX, cym);
+    ld_cls_vec<NC>(clt, e + HX, cyp);
+    ld_cls_vec<NC>(clt, e - HXY, czm);
+    ld_cls_vec<NC>(clt, e + HXY, czp);
+    const int ce = clt[ee];
+    const T *stab = rb_stage(g, k);
+    // x neighbours of cell c: other[m0 + c - 1 + s] and other[m0 + c + s];
+    // the ones past the thread's cells: other[m0 - 1] from the thread
+    // before, other[m0 + NC] from the thread after
+    const T sr = dpp_shift<kDppWaveShr1>(P[NC - 1], xe);
+    const T sl = dpp_shift<kDppWaveShl1>(P[0], xe);
+    const int csr = dpp_shift<kDppWaveShr1>(cn[NC - 1], ce), csl = dpp_shift<kDppWaveShl1>(cn[0], ce);
+    Nb<T> n[NC];
+    Cls q[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const T lo = c > 0 ? P[c - 1] : sr, hi = c < NC - 1 ? P[c + 1] : sl;
+        const int clo_ = c > 0 ? cn[c - 1] : csr, chi = c < NC - 1 ? cn[c + 1] : csl;
+        n[c].xm = s ? P[c] : lo;
+        n[c].xp = s ? hi : P[c];
+        n[c].ym = ym[c];
+        n[c].yp = yp[c];
+        n[c].zm = zm[c];
+        n[c].zp = zp[c];
+        q[c].c = co[c];
+        q[c].xm = s ? cn[c] : clo_;
+        q[c].xp = s ? chi : cn[c];
+        q[c].ym = cym[c];
+        q[c].yp = cyp[c];
+        q[c].zm = czm[c];
+        q[c].zp = czp[c];
+    }
+    T xn[NC];
+    bool valid[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int i = 2 * (m0 + c) + s;
+        valid[c] = rowok && m0 + c < HXi && i >= 1 && i <= g.NX;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) xn[c] = gs_cell(g, stab, q[c], n[c], pp[c], bv[c], 2 * (m0 + c) + s, j, k);
+    if (DOT) {
+        // the other half's cell at offset m0 + c is i = 2(m0 + c) + 1 - s
+        dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int io = 2 * (m0 + c) + 1 - s;
+            if (valid[c]) v[0] = dd_add(v[0], (double)bv[c] * (double)xn[c]);
+            if (rowok && m0 + c < HXi && io >= 1 && io <= g.NX) v[0] = dd_add(v[0], (double)bp[c] * (double)P[c]);
+        }
+        // the workgroup sum before the stores, as in k_rb_sweep
+        block_partials<1>(v, rbuf);
+    }
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) all = all && valid[c];
+    if (all) {
+        st_vec<NC>(xo, e, xn);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (valid[c]) xo[e + (unsigned)c] = xn[c];
+    }
+}
+
+// The FP64 form of k_rb_sweep_w (NC = 2) written out for the two cells: the
+// same loads, lane shifts and updates, but the compiler allocates 80 VGPRs
+// for it against 88 for the generic form with NC = 2 (six waves per SIMD
+// against five), which measured 2-3 % slower (77.8 against 75.4 us per
+// full sweep at 480x480x60, 890 against 881 us at 960x960x120).
+template <typename T, bool ZERO, bool DOT = false>
+__global__ void __launch_bounds__(256) k_rb_sweep_w2(T *__restrict__ xo, const T *__restrict__ xt,
+                                                     const T *__restrict__ bo, const uint8_t *__restrict__ clo,
+                                                     const uint8_t *__restrict__ clt, RbGeo<T> g, int h,
+                                                     const KState *st, const T *__restrict__ bt = nullptr,
+                                                     RedBuf rbuf = RedBuf{})
+{
+    if (halted(st)) return;
+    const unsigned gx = ((unsigned)g.HX + (2 * BX) - 1) / (2 * BX), gy = ((unsigned)g.NY + BY - 1) / BY;
+    const unsigned gz = (unsigned)g.NZ, nreal = gx * gy * gz, nb = (nreal + 7u) & ~7u;
+    const unsigned p = blockIdx.x;
+    const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
+    const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
+    const int HXi = g.HX;
+    const int mb = (int)bx * (2 * BX), m0 = mb + 2 * (int)threadIdx.x;
     const int jr = (int)by * BY + (int)threadIdx.y + 1;
     const int k = (int)min(bz, gz - 1u) + 1;
     const bool rowok = L < nreal && jr <= g.NY;
@@ -294,7 +459,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w(T *__restrict__ xo, const T 
     // the x neighbour outside the wave's pairs: m = mb - 1 for the first
     // thread (s = 0), mb + 128 for the last (s = 1), clamped into the row
     // (a clamped one belongs to a cell outside the interior)
-    const int me = s ? min(mb + kWideM, HXi - 1) : max(mb - 1, 0);
+    const int me = s ? min(mb + (2 * BX), HXi - 1) : max(mb - 1, 0);
     const unsigned ee = (unsigned)me + HX * (unsigned)j + HXY * (unsigned)k;
     T p0, p1, ym0, ym1, yp0, yp1, zm0, zm1, zp0, zp1, pp0 = T(0), pp1 = T(0), b0, b1;
     ld_pair(xt, e, p0, p1);
@@ -304,6 +469,8 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w(T *__restrict__ xo, const T 
     ld_pair(xt, e + HXY, zp0, zp1);
     if (!ZERO) ld_pair(xo, e, pp0, pp1);
     ld_pair(bo, e, b0, b1);
+    T bp0 = T(0), bp1 = T(0);
+    if (DOT) ld_pair(bt, e, bp0, bp1);
     const T xe = xt[ee];
     int c0, c1, n0, n1, cym0, cym1, cyp0, cyp1, czm0, czm1, czp0, czp1;
     ld_cls_pair(clo, e, c0, c1);
@@ -352,6 +519,17 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w(T *__restrict__ xo, const T 
     const bool vb = rowok && m0 + 1 < HXi && ib >= 1 && ib <= g.NX;
     const T xa = gs_cell(g, stab, qa, na, pp0, b0, ia, j, k);
     const T xb = gs_cell(g, stab, qb, nbb, pp1, b1, ib, j, k);
+    if (DOT) {
+        // the other half's cells at offsets m0, m0 + 1: i = 2m0 + 1 - s, + 2
+        const int ja = 2 * m0 + 1 - s, jb2 = ja + 2;
+        dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+        if (va) v[0] = dd_add(v[0], (double)b0 * (double)xa);
+        if (rowok && m0 < HXi && ja >= 1 && ja <= g.NX) v[0] = dd_add(v[0], (double)bp0 * (double)p0);
+        if (vb) v[0] = dd_add(v[0], (double)b1 * (double)xb);
+        if (rowok && m0 + 1 < HXi && jb2 >= 1 && jb2 <= g.NX) v[0] = dd_add(v[0], (double)bp1 * (double)p1);
+        // the workgroup sum before the stores, as in k_rb_sweep
+        block_partials<1>(v, rbuf);
+    }
     if (va && vb) {
         st_pair(xo, e, xa, xb);
     } else {
@@ -360,10 +538,19 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w(T *__restrict__ xo, const T 
     }
 }
 
+// The large grids' sweep and CalcAX!: the NC-cell form (kSweepWide), NC
+// cells of a half per thread (one 16-byte load: two FP64 or four FP32
+// values; FP32 with two cells measured slower than three rows per thread,
+// 58.1 against 51.5 us at 480x480x60), or the rows form.
 template <typename T>
+constexpr bool kSweepWide = true;
+template <typename T>
+constexpr int kSweepCells = sizeof(T) == 8 ? 2 : 4;
+
+template <typename T, int NC = 2>
 static unsigned rb_wide_nb(const RbGeo<T> &g)
 {
-    const unsigned gx = ((unsigned)g.HX + kWideM - 1) / kWideM, gy = ((unsigned)g.NY + BY - 1) / BY;
+    const unsigned gx = ((unsigned)g.HX + NC * BX - 1) / (NC * BX), gy = ((unsigned)g.NY + BY - 1) / BY;
     return (gx * gy * (unsigned)g.NZ + 7u) & ~7u;
 }
 
@@ -375,9 +562,9 @@ static unsigned rb_wide_nb(const RbGeo<T> &g)
 // (12.2 against 13.4 us with four), three on large ones (51.9 against
 // 58.4 us with four at 480x480x60, 548 against 615 us at 960x960x120:
 // 73-77 VGPRs keep six waves per SIMD, four rows' 97 only four).  Large:
-// at least 2^21 cells per half.  On large grids the FP64 sweep itself runs
-// in the two-cell form (k_rb_sweep_w); the rows form serves its pass with
-// the sum Σ b·x.
+// at least 2^21 cells per half.  On large grids the sweep runs in the
+// NC-cell form (k_rb_sweep_w) while kSweepWide holds; the rows form stays
+// for the small grids and as the large grids' alternative.
 template <typename T>
 constexpr int rb_rows_large()
 {
@@ -407,18 +594,29 @@ hipError_t launch_rb_sweep(T *x, const T *b, const RbGeo<T> &g, int color, const
 {
     constexpr int R = sizeof(T) == 4 ? 2 : 1;
     const int h = 1 - color;
-    // large grids: FP64 in the two-cell form (measured, HIP events, both
-    // colours: 75.9 against 84.0 us at 480x480x60, 880 against 1019 us at
-    // 960x960x120 for two rows per thread), FP32 in rows (58.1 against
-    // 51.5 us and 555 against 536 us in the two-cell form)
-    if (sizeof(T) == 8 && g.HS >= ((size_t)1 << 21)) {
+    // large grids: the NC-cell form (measured, HIP events, both colours:
+    // FP64 two cells 75-78 against 84.0 us at 480x480x60 and 881 against
+    // 1019 us at 960x960x120 for two rows per thread; FP32 four cells 47.0
+    // against 51.5 us and 467 against 536 us for three rows)
+    if (kSweepWide<T> && g.HS >= ((size_t)1 << 21)) {
+        constexpr int NC = kSweepCells<T>;
         const size_t own = h ? g.HS : 0, oth = h ? 0 : g.HS;
-        if (from_zero)
-            hipLaunchKernelGGL((k_rb_sweep_w<T, true>), dim3(rb_wide_nb(g)), dim3(BX, BY), rb_lds(g), s, x + own,
-                               x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
-        else
-            hipLaunchKernelGGL((k_rb_sweep_w<T, false>), dim3(rb_wide_nb(g)), dim3(BX, BY), rb_lds(g), s, x + own,
-                               x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+        const unsigned nb = rb_wide_nb<T, NC>(g);
+        if constexpr (NC == 2) {
+            if (from_zero)
+                hipLaunchKernelGGL((k_rb_sweep_w2<T, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own, x + oth,
+                                   b + own, g.cls + own, g.cls + oth, g, h, st);
+            else
+                hipLaunchKernelGGL((k_rb_sweep_w2<T, false>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+                                   x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+        } else {
+            if (from_zero)
+                hipLaunchKernelGGL((k_rb_sweep_w<T, NC, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+                                   x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+            else
+                hipLaunchKernelGGL((k_rb_sweep_w<T, NC, false>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+                                   x + oth, b + own, g.cls + own, g.cls + oth, g, h, st);
+        }
     } else if (g.HS >= ((size_t)1 << 21)) {
         rb_sweep_rows<T, rb_rows_large<T>()>(x, b, g, h, from_zero, st, s);
     } else {
@@ -442,7 +640,16 @@ hipError_t launch_rb_sweep_dot(T *x, const T *b, const RbGeo<T> &g, int fin, KSt
     const int h = 0;  // colour 1
     const size_t own = 0, oth = g.HS;
     unsigned nb;
-    if (g.HS >= ((size_t)1 << 21)) {
+    if (kSweepWide<T> && g.HS >= ((size_t)1 << 21)) {
+        constexpr int NC = kSweepCells<T>;
+        nb = rb_wide_nb<T, NC>(g);
+        if constexpr (NC == 2)
+            hipLaunchKernelGGL((k_rb_sweep_w2<T, false, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+                               x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, b + oth, rb);
+        else
+            hipLaunchKernelGGL((k_rb_sweep_w<T, NC, false, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
+                               x + oth, b + own, g.cls + own, g.cls + oth, g, h, st, b + oth, rb);
+    } else if (g.HS >= ((size_t)1 << 21)) {
         constexpr int RL = rb_rows_large<T>();
         nb = rb_rows_nb<RL>(g);
         hipLaunchKernelGGL((k_rb_sweep<T, RL, false, true>), dim3(nb), dim3(BX, BY), rb_lds(g), s, x + own,
@@ -715,17 +922,19 @@ bool rb_sweep2_fits(const RbGeo<T> &g)
 
 // Measured on one MI355X (HIP events, one full sweep, 64 x 8 footprint):
 // at 960x960x120 (transient) the two-colour launches with 60-plane chunks
-// take 933 us in FP64 and 520 us in FP32, against 880 us (two-cell form)
-// and 536-559 us for the per-colour launches (40-plane chunks: 951 and
-// 522 us; 120: 929 and 527 us); at 480x480x60 and 240x240x30 every chunk
-// length was slower (best 132 against 96 us and 69 against 58 us at
-// 480x480x60).  So FP32 operators of 120 planes and more take 60-plane
-// chunks, the rest one launch per colour.
+// take 933 us in FP64 and 520 us in FP32 (40-plane chunks: 951 and 522 us;
+// 120: 929 and 527 us), against 881-892 us and 467 us for the per-colour
+// launches of the NC-cell form; at 480x480x60 and 240x240x30 every chunk
+// length was slower than the per-colour launches of any form (best 132
+// against 96 us and 69 against 58 us at 480x480x60).  So every operator
+// takes one launch per colour unless the caller sets a chunk.
 int sweep_chunk_auto(int HX, int NY, int NZ, int elem_bytes)
 {
     (void)HX;
     (void)NY;
-    return NZ >= 120 && elem_bytes == 4 ? 60 : 0;
+    (void)NZ;
+    (void)elem_bytes;
+    return 0;
 }
 
 // The Gauss-Seidel value of a cell whose old value and neighbours are all
@@ -922,27 +1131,27 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax(T *__restrict__ ax, const T 
             if (valid[r][a]) stb(ax, co[r][a] * sz, av[r][a]);
 }
 
-// CalcAX! in the two-cell form of k_rb_sweep_w: a thread takes the four
-// consecutive cells i = 2m0 .. 2m0 + 3 of a row (m0 = mb + 2l of a
-// 128-wide tile, one wave per row), two in each half, so that each load of
-// x, dy, dz and the class bytes covers two cells and the x neighbours at
-// the pairs' ends come from the next threads by DPP lane shifts (or, at
-// the wave's ends, from one more load each).  Every cell's value is
-// k_rb_calc_ax's and the sums are exactly rounded, so the results are
-// bitwise the same.
-template <typename T, int NS>
+// CalcAX! in the form of k_rb_sweep_w: a thread takes the 2·NC consecutive
+// cells i = 2m0 .. 2m0 + 2NC - 1 of a row (m0 = mb + NC·l of an NC·64-wide
+// tile, one wave per row), NC in each half, so that each load of x, dy,
+// dz and the class bytes covers NC cells and the x neighbours at the ends
+// come from the next threads by DPP lane shifts (or, at the wave's ends,
+// from one more load each).  Every cell's value is k_rb_calc_ax's and the
+// sums are exactly rounded, so the results are bitwise the same.
+template <typename T, int NC, int NS>
 __global__ void __launch_bounds__(256) k_rb_calc_ax_w(T *__restrict__ ax, const T *__restrict__ x,
                                                       const T *__restrict__ dy, const T *__restrict__ dz,
                                                       RbGeo<T> g, KState *st, RedBuf rb)
 {
     if (halted(st)) return;
-    const unsigned gx = ((unsigned)g.HX + kWideM - 1) / kWideM, gy = ((unsigned)g.NY + BY - 1) / BY;
+    constexpr int TW = NC * BX;
+    const unsigned gx = ((unsigned)g.HX + TW - 1) / TW, gy = ((unsigned)g.NY + BY - 1) / BY;
     const unsigned gz = (unsigned)g.NZ, nreal = gx * gy * gz, nb = (nreal + 7u) & ~7u;
     const unsigned p = blockIdx.x;
     const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
     const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
     const int HXi = g.HX;
-    const int mb = (int)bx * kWideM, m0 = mb + 2 * (int)threadIdx.x;
+    const int mb = (int)bx * TW, m0 = mb + NC * (int)threadIdx.x;
     const int jr = (int)by * BY + (int)threadIdx.y + 1;
     const int k = (int)min(bz, gz - 1u) + 1;
     const bool rowok = L < nreal && jr <= g.NY;
@@ -951,98 +1160,109 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax_w(T *__restrict__ ax, const 
     const unsigned HX = (unsigned)HXi, HXY = HX * (unsigned)g.MY;
     const unsigned e = (unsigned)mc + HX * (unsigned)j + HXY * (unsigned)k;
     const int h0 = (j + k) & 1;
-    // half A holds the cells i = 2m0 and 2m0 + 2, half B i = 2m0 + 1, 2m0 + 3
+    // half A holds the cells i = 2(m0 + c), half B i = 2(m0 + c) + 1
     const size_t oA = h0 ? g.HS : 0, oB = h0 ? 0 : g.HS;
     const T *xA = x + oA, *xB = x + oB;
     const uint8_t *cA = g.cls + oA, *cB = g.cls + oB;
-    // the x neighbours outside the wave's pairs: B at mb - 1 (first thread),
-    // A at mb + 128 (last thread), clamped into the row (a clamped one
+    // the x neighbours outside the wave's cells: B at mb - 1 (first thread),
+    // A at mb + TW (last thread), clamped into the row (a clamped one
     // belongs to a cell outside the interior)
     const unsigned eB = (unsigned)max(mb - 1, 0) + HX * (unsigned)j + HXY * (unsigned)k;
-    const unsigned eA = (unsigned)min(mb + kWideM, HXi - 1) + HX * (unsigned)j + HXY * (unsigned)k;
-    T A0, A1, B0, B1;
-    Nb<T> n[4];
-    ld_pair(xA, e, A0, A1);
-    ld_pair(xB, e, B0, B1);
-    // y and z neighbours: those of A's cells are in B, those of B's in A
-    ld_pair(xB, e - HX, n[0].ym, n[2].ym);
-    ld_pair(xB, e + HX, n[0].yp, n[2].yp);
-    ld_pair(xB, e - HXY, n[0].zm, n[2].zm);
-    ld_pair(xB, e + HXY, n[0].zp, n[2].zp);
-    ld_pair(xA, e - HX, n[1].ym, n[3].ym);
-    ld_pair(xA, e + HX, n[1].yp, n[3].yp);
-    ld_pair(xA, e - HXY, n[1].zm, n[3].zm);
-    ld_pair(xA, e + HXY, n[1].zp, n[3].zp);
+    const unsigned eA = (unsigned)min(mb + TW, HXi - 1) + HX * (unsigned)j + HXY * (unsigned)k;
+    // [0]: half A's cells (neighbours in B), [1]: half B's (neighbours in A)
+    T own[2][NC], ym[2][NC], yp[2][NC], zm[2][NC], zp[2][NC];
+    ld_vec<NC>(xA, e, own[0]);
+    ld_vec<NC>(xB, e, own[1]);
+    ld_vec<NC>(xB, e - HX, ym[0]);
+    ld_vec<NC>(xB, e + HX, yp[0]);
+    ld_vec<NC>(xB, e - HXY, zm[0]);
+    ld_vec<NC>(xB, e + HXY, zp[0]);
+    ld_vec<NC>(xA, e - HX, ym[1]);
+    ld_vec<NC>(xA, e + HX, yp[1]);
+    ld_vec<NC>(xA, e - HXY, zm[1]);
+    ld_vec<NC>(xA, e + HXY, zp[1]);
     const T xeB = xB[eB], xeA = xA[eA];
-    T dyv[4] = {T(0), T(0), T(0), T(0)}, dzv[4] = {T(0), T(0), T(0), T(0)};
+    T dyv[2][NC] = {}, dzv[2][NC] = {};
     if (NS >= 1 && dy) {
-        ld_pair(dy + oA, e, dyv[0], dyv[2]);
-        ld_pair(dy + oB, e, dyv[1], dyv[3]);
+        ld_vec<NC>(dy + oA, e, dyv[0]);
+        ld_vec<NC>(dy + oB, e, dyv[1]);
     }
     if (NS >= 2) {
-        ld_pair(dz + oA, e, dzv[0], dzv[2]);
-        ld_pair(dz + oB, e, dzv[1], dzv[3]);
+        ld_vec<NC>(dz + oA, e, dzv[0]);
+        ld_vec<NC>(dz + oB, e, dzv[1]);
     }
-    Cls q[4];
-    ld_cls_pair(cA, e, q[0].c, q[2].c);
-    ld_cls_pair(cB, e, q[1].c, q[3].c);
-    ld_cls_pair(cB, e - HX, q[0].ym, q[2].ym);
-    ld_cls_pair(cB, e + HX, q[0].yp, q[2].yp);
-    ld_cls_pair(cB, e - HXY, q[0].zm, q[2].zm);
-    ld_cls_pair(cB, e + HXY, q[0].zp, q[2].zp);
-    ld_cls_pair(cA, e - HX, q[1].ym, q[3].ym);
-    ld_cls_pair(cA, e + HX, q[1].yp, q[3].yp);
-    ld_cls_pair(cA, e - HXY, q[1].zm, q[3].zm);
-    ld_cls_pair(cA, e + HXY, q[1].zp, q[3].zp);
+    int cc[2][NC], cym[2][NC], cyp[2][NC], czm[2][NC], czp[2][NC];
+    ld_cls_vec<NC>(cA, e, cc[0]);
+    ld_cls_vec<NC>(cB, e, cc[1]);
+    ld_cls_vec<NC>(cB, e - HX, cym[0]);
+    ld_cls_vec<NC>(cB, e + HX, cyp[0]);
+    ld_cls_vec<NC>(cB, e - HXY, czm[0]);
+    ld_cls_vec<NC>(cB, e + HXY, czp[0]);
+    ld_cls_vec<NC>(cA, e - HX, cym[1]);
+    ld_cls_vec<NC>(cA, e + HX, cyp[1]);
+    ld_cls_vec<NC>(cA, e - HXY, czm[1]);
+    ld_cls_vec<NC>(cA, e + HXY, czp[1]);
     const int ceB = cB[eB], ceA = cA[eA];
     const T *stab = rb_stage(g, k);
-    // x neighbours (cell c = i - 2m0): 0: B[m0 - 1], B0; 1: A0, A1;
-    // 2: B0, B1; 3: A1, A[m0 + 2]
-    n[0].xm = dpp_shift<kDppWaveShr1>(B1, xeB);
-    n[0].xp = B0;
-    n[1].xm = A0;
-    n[1].xp = A1;
-    n[2].xm = B0;
-    n[2].xp = B1;
-    n[3].xm = A1;
-    n[3].xp = dpp_shift<kDppWaveShl1>(A0, xeA);
-    q[0].xm = dpp_shift<kDppWaveShr1>(q[3].c, ceB);
-    q[0].xp = q[1].c;
-    q[1].xm = q[0].c;
-    q[1].xp = q[2].c;
-    q[2].xm = q[1].c;
-    q[2].xp = q[3].c;
-    q[3].xm = q[2].c;
-    q[3].xp = dpp_shift<kDppWaveShl1>(q[0].c, ceA);
-    const T own[4] = {A0, B0, A1, B1};
-    T av[4];
-    bool valid[4];
+    // x neighbours: A's cell c has B[m0 + c - 1] and B[m0 + c], B's cell c
+    // has A[m0 + c] and A[m0 + c + 1]
+    const T bsr = dpp_shift<kDppWaveShr1>(own[1][NC - 1], xeB);   // B[m0 - 1]
+    const T asl = dpp_shift<kDppWaveShl1>(own[0][0], xeA);        // A[m0 + NC]
+    const int cbsr = dpp_shift<kDppWaveShr1>(cc[1][NC - 1], ceB);
+    const int casl = dpp_shift<kDppWaveShl1>(cc[0][0], ceA);
+    T av[2][NC];
+    bool valid[2][NC];
     dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int i = 2 * m0 + c;
-        valid[c] = rowok && m0 + (c >> 1) < HXi && i >= 1 && i <= g.NX;
-        const Cond<T> cd = rb_cond(g, stab, q[c], i, j, k);
-        av[c] = rb_apply(g, cd, n[c], own[c]);
-        const T yv = (NS >= 1 && dy) ? dyv[c] : av[c];
-        if (valid[c]) {
-            if (NS >= 1) v[0] = dd_add(v[0], (double)av[c] * (double)yv);
-            if (NS >= 2) v[1] = dd_add(v[1], (double)av[c] * (double)dzv[c]);
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            Nb<T> n;
+            Cls q;
+            if (a == 0) {
+                n.xm = c > 0 ? own[1][c - 1] : bsr;
+                n.xp = own[1][c];
+                q.xm = c > 0 ? cc[1][c - 1] : cbsr;
+                q.xp = cc[1][c];
+            } else {
+                n.xm = own[0][c];
+                n.xp = c < NC - 1 ? own[0][c + 1] : asl;
+                q.xm = cc[0][c];
+                q.xp = c < NC - 1 ? cc[0][c + 1] : casl;
+            }
+            n.ym = ym[a][c];
+            n.yp = yp[a][c];
+            n.zm = zm[a][c];
+            n.zp = zp[a][c];
+            q.c = cc[a][c];
+            q.ym = cym[a][c];
+            q.yp = cyp[a][c];
+            q.zm = czm[a][c];
+            q.zp = czp[a][c];
+            const int i = 2 * (m0 + c) + a;
+            valid[a][c] = rowok && m0 + c < HXi && i >= 1 && i <= g.NX;
+            const Cond<T> cd = rb_cond(g, stab, q, i, j, k);
+            av[a][c] = rb_apply(g, cd, n, own[a][c]);
+            const T yv = (NS >= 1 && dy) ? dyv[a][c] : av[a][c];
+            if (valid[a][c]) {
+                if (NS >= 1) v[0] = dd_add(v[0], (double)av[a][c] * (double)yv);
+                if (NS >= 2) v[1] = dd_add(v[1], (double)av[a][c] * (double)dzv[a][c]);
+            }
         }
-    }
     if (NS > 0) block_partials<NS>(v, rb);
-    T *aA = ax + oA, *aB = ax + oB;
-    if (valid[0] && valid[2]) {
-        st_pair(aA, e, av[0], av[2]);
-    } else {
-        if (valid[0]) aA[e] = av[0];
-        if (valid[2]) aA[e + 1] = av[2];
-    }
-    if (valid[1] && valid[3]) {
-        st_pair(aB, e, av[1], av[3]);
-    } else {
-        if (valid[1]) aB[e] = av[1];
-        if (valid[3]) aB[e + 1] = av[3];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        T *ah = ax + (a ? oB : oA);
+        bool all = true;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) all = all && valid[a][c];
+        if (all) {
+            st_vec<NC>(ah, e, av[a]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (valid[a][c]) ah[e + (unsigned)c] = av[a][c];
+        }
     }
 }
 
@@ -1057,16 +1277,19 @@ hipError_t launch_rb_calc_ax(T *ax, const T *x, const RbGeo<T> &g, const T *dot_
 {
     constexpr int R = kAxRows;
     const int ns = fin < 0 ? 0 : (dot_z ? 2 : 1);
-    // large grids in FP64: the two-cell form
-    const bool wide = sizeof(T) == 8 && g.HS >= ((size_t)1 << 21);
-    const unsigned nb = wide ? rb_wide_nb(g) : rb_rows_nb<R>(g);
+    // large grids: the NC-cell form (measured, HIP events: FP64 53.5
+    // against 66.7 us at 480x480x60 and 677 against 822 us at 960x960x120,
+    // FP32 32.1 against 40.5 us and 334 against 432 us)
+    const bool wide = g.HS >= ((size_t)1 << 21);
+    constexpr int NC = kSweepCells<T>;
+    const unsigned nb = wide ? rb_wide_nb<T, NC>(g) : rb_rows_nb<R>(g);
     const dim3 grid(nb), block(BX, BY);
     if (wide && ns == 0)
-        hipLaunchKernelGGL((k_rb_calc_ax_w<T, 0>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+        hipLaunchKernelGGL((k_rb_calc_ax_w<T, NC, 0>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     else if (wide && ns == 1)
-        hipLaunchKernelGGL((k_rb_calc_ax_w<T, 1>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+        hipLaunchKernelGGL((k_rb_calc_ax_w<T, NC, 1>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     else if (wide)
-        hipLaunchKernelGGL((k_rb_calc_ax_w<T, 2>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
+        hipLaunchKernelGGL((k_rb_calc_ax_w<T, NC, 2>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     else if (ns == 0)
         hipLaunchKernelGGL((k_rb_calc_ax<T, 0, R>), grid, block, rb_lds(g), s, ax, x, dot_y, dot_z, g, st, rb);
     else if (ns == 1)

@@ -93,9 +93,8 @@ extern "C" {
 #define H3D_OPT_SWEEP_CHUNK 4  /* Gauss-Seidel sweeps of the colour-split layout: 0 = one launch per colour,
                                   C >= 1 = the two colours of a sweep in one launch marching chunks of C
                                   z-planes (C capped at NZ and 125; operators with more than 16 classes
-                                  keep one launch per colour), -1 = chosen from the grid (default: chunks
-                                  of 60 planes for the FP32 operator when NZ >= 120, else one launch per
-                                  colour); h3d_cg
+                                  keep one launch per colour), -1 = chosen from the grid (default; one
+                                  launch per colour on every grid measured); h3d_cg
                                   runs its last sweep per colour either way (that pass also forms
                                   rho = (r, z)); bitwise identical */
 #define H3D_OPT_RESIDENT 6     /* Gauss-Seidel preconditioner of the colour-split layout on grids whose

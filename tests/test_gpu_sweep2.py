@@ -82,10 +82,8 @@ def test_timing_entry_points():
 
 
 def test_auto_chunk_on_a_deep_grid():
-    """The default (-1) picks the two-colour launches for the FP32
-    operator with at least 120 planes (the FP32-refined solve's inner
-    sweeps) and the per-colour launches for FP64: bitwise the per-colour
-    solve either way."""
+    """The default (-1) on an operator of 120 planes: bitwise the
+    per-colour solve, FP64 and FP32-refined."""
     print("case 24x20x120 pbicgstab transient auto chunk", flush=True)
     ref = solve(24, 20, 120, 0, L.SOLVER_PBICGSTAB, False, tol=1e-8, power="map")
     assert ref[0][0]
