@@ -3,7 +3,7 @@ cells) with the seeded power map scaled by 1 + 0.5·sin(2πt/20 ms),
 implicit-Euler steps of Δt = 1 ms from 300 K (the reference's step is
 calRHS! then the Krylov solve with is_steady = false, src/heat3ds.jl:122-151).
 Two steps, each an FP32-refined CG+GS(5) solve to a true relative residual of
-1e-8 (two-colour sweep launches, the default at this depth), a checkpoint
+1e-8 (per-colour sweep launches, four FP32 cells per thread), a checkpoint
 every 2 steps (so the adjoint recomputes step 1 from θ⁰), the adjoint of
 the time-integrated smooth maximum, and one central difference (relative
 step 1e-6) on a layer conductivity.  Checks: every step's true FP64
