@@ -82,14 +82,19 @@ def test_large_grid_layouts_bitwise(solver, steady, precision):
     assert np.array_equal(a[2], b[2])
 
 
-@pytest.mark.parametrize("solver,steady", [(L.SOLVER_PBICGSTAB, True), (L.SOLVER_CG, False)])
-def test_large_grid_odd_extent_bitwise(solver, steady):
-    """257x270x60 (2.19 M cells per half, NX odd, an even number of
-    half-row indices): the FP64 sweep's two-cell form with a pair half
-    outside the interior at the row's end; 30 iterations."""
-    print("case 257x270x60 solver", solver, "steady", steady, flush=True)
-    a = run_limited((257, 270, 60), L.LAYOUT_AUTO, solver, steady, L.FP64, 30)
-    b = run_limited((257, 270, 60), L.LAYOUT_STANDARD, solver, steady, L.FP64, 30)
+@pytest.mark.parametrize("nx", [257, 256])
+@pytest.mark.parametrize("solver,steady,precision", [(L.SOLVER_PBICGSTAB, True, L.FP64),
+                                                     (L.SOLVER_CG, False, L.FP64),
+                                                     (L.SOLVER_CG, False, L.FP32_REFINED)])
+def test_large_grid_odd_extent_bitwise(nx, solver, steady, precision):
+    """nx x 270 x 60 (over 2.1 M cells per half): 257 (NX odd, 130
+    half-row indices) and 256 (129 half-row indices), so that the last
+    thread of a row holds one, two or three cells of the large-grid kernels
+    (two FP64 or four FP32 cells per thread) inside the row; 30 iterations
+    against the reference layout."""
+    print("case %dx270x60 solver" % nx, solver, "steady", steady, "precision", precision, flush=True)
+    a = run_limited((nx, 270, 60), L.LAYOUT_AUTO, solver, steady, precision, 30)
+    b = run_limited((nx, 270, 60), L.LAYOUT_STANDARD, solver, steady, precision, 30)
     assert a[0] == b[0]
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])
