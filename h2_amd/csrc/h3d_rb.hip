@@ -328,7 +328,8 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w(T *__restrict__ xo, const T 
     const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
     const int HXi = g.HX;
     const int mb = (int)bx * TW, m0 = mb + NC * (int)threadIdx.x;
-    const int jr = (int)by * BY + (int)threadIdx.y + 1;
+    // one row per wave: its row index is wave-uniform (scalar registers)
+    const int jr = (int)by * BY + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;
     const int k = (int)min(bz, gz - 1u) + 1;
     const bool rowok = L < nreal && jr <= g.NY;
     const int j = min(jr, g.NY);
@@ -446,7 +447,8 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w2(T *__restrict__ xo, const T
     const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
     const int HXi = g.HX;
     const int mb = (int)bx * (2 * BX), m0 = mb + 2 * (int)threadIdx.x;
-    const int jr = (int)by * BY + (int)threadIdx.y + 1;
+    // one row per wave: its row index is wave-uniform (scalar registers)
+    const int jr = (int)by * BY + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;
     const int k = (int)min(bz, gz - 1u) + 1;
     const bool rowok = L < nreal && jr <= g.NY;
     const int j = min(jr, g.NY);
@@ -1152,7 +1154,8 @@ __global__ void __launch_bounds__(256) k_rb_calc_ax_w(T *__restrict__ ax, const 
     const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
     const int HXi = g.HX;
     const int mb = (int)bx * TW, m0 = mb + NC * (int)threadIdx.x;
-    const int jr = (int)by * BY + (int)threadIdx.y + 1;
+    // one row per wave: its row index is wave-uniform (scalar registers)
+    const int jr = (int)by * BY + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;
     const int k = (int)min(bz, gz - 1u) + 1;
     const bool rowok = L < nreal && jr <= g.NY;
     const int j = min(jr, g.NY);
