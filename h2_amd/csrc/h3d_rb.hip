@@ -17,9 +17,12 @@
 //
 // Work decomposition: a workgroup is 64 (half-row index m) by 4 (rows j)
 // threads of one plane (the two-colour sweep, k_rb_sweep2, marches a 64 x 8
-// footprint through a chunk of planes instead).  "Pair" kernels handle the two cells i = 2m and
-// 2m+1 of a thread (one in each half); the sweep handles the one cell of
-// the colour being updated.  Every thread computes from clamped, always
+// footprint through a chunk of planes instead).  "Pair" kernels handle the
+// two cells i = 2m and 2m+1 of a thread (one in each half); the sweep
+// handles the one cell of the colour being updated.  On large grids the
+// sweep, CalcAX! and their sums take NC neighbouring cells of a half per
+// thread instead, one row per wave (k_rb_sweep_w, k_rb_sweep_w2,
+// k_rb_calc_ax_w).  Every thread computes from clamped, always
 // in-bounds indices and only its stores and sums are predicated on the
 // cell being an interior cell, so no branch separates a thread's loads and
 // they are all in flight together.  Workgroups are renumbered so that each
@@ -440,13 +443,14 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w2(T *__restrict__ xo, const T
                                                      RedBuf rbuf = RedBuf{})
 {
     if (halted(st)) return;
-    const unsigned gx = ((unsigned)g.HX + (2 * BX) - 1) / (2 * BX), gy = ((unsigned)g.NY + BY - 1) / BY;
+    constexpr int TW = 2 * BX;  // the tile's half-row indices
+    const unsigned gx = ((unsigned)g.HX + TW - 1) / TW, gy = ((unsigned)g.NY + BY - 1) / BY;
     const unsigned gz = (unsigned)g.NZ, nreal = gx * gy * gz, nb = (nreal + 7u) & ~7u;
     const unsigned p = blockIdx.x;
     const unsigned L = (nb >> 3) * (p & 7u) + (p >> 3);
     const unsigned bx = L % gx, t1 = L / gx, bz = t1 % gz, by = t1 / gz;
     const int HXi = g.HX;
-    const int mb = (int)bx * (2 * BX), m0 = mb + 2 * (int)threadIdx.x;
+    const int mb = (int)bx * TW, m0 = mb + 2 * (int)threadIdx.x;
     // one row per wave: its row index is wave-uniform (scalar registers)
     const int jr = (int)by * BY + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;
     const int k = (int)min(bz, gz - 1u) + 1;
@@ -461,7 +465,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep_w2(T *__restrict__ xo, const T
     // the x neighbour outside the wave's pairs: m = mb - 1 for the first
     // thread (s = 0), mb + 128 for the last (s = 1), clamped into the row
     // (a clamped one belongs to a cell outside the interior)
-    const int me = s ? min(mb + (2 * BX), HXi - 1) : max(mb - 1, 0);
+    const int me = s ? min(mb + TW, HXi - 1) : max(mb - 1, 0);
     const unsigned ee = (unsigned)me + HX * (unsigned)j + HXY * (unsigned)k;
     T p0, p1, ym0, ym1, yp0, yp1, zm0, zm1, zp0, zp1, pp0 = T(0), pp1 = T(0), b0, b1;
     ld_pair(xt, e, p0, p1);
