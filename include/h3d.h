@@ -301,8 +301,9 @@ int h3d_sor(h3d_ctx *ctx, int x, int b, double omega, double dt, int is_steady, 
  * launch).  which: 0 = one full red-black GS sweep (both colours),
  * 1 = CalcAX!, 2 = CalcAX! with its fused dot; 3 and 4 = the FP32 operator's
  * sweep and CalcAX! (the kernels of the H3D_FP32_REFINED inner solve);
- * 5 and 6 = one full sweep as one two-colour launch, FP64 and FP32
- * (H3D_OPT_SWEEP_CHUNK; H3D_ERR_ARG when the operator has none); 7 and 8 =
+ * 5 and 6 = one full sweep as one two-colour launch, FP64 and FP32, with
+ * the chunk H3D_OPT_SWEEP_CHUNK sets (H3D_ERR_ARG when it sets none, as the
+ * default does, or the operator has none); 7 and 8 =
  * the whole preconditioner (five sweeps from zero) as one resident launch,
  * FP64 and FP32 (H3D_OPT_RESIDENT; H3D_ERR_ARG when the grid does not fit);
  * 9 and 10 = PBiCGSTAB!'s first resident launch of an iteration (BiCG1!,
