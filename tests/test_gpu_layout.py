@@ -100,6 +100,22 @@ def test_large_grid_odd_extent_bitwise(nx, solver, steady, precision):
     assert np.array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize("shape,solver,steady,precision", [
+    ((480, 480, 60), L.SOLVER_PBICGSTAB, True, L.FP64),
+    ((480, 480, 60), L.SOLVER_CG, False, L.FP32_REFINED),
+    ((960, 960, 120), L.SOLVER_PBICGSTAB, False, L.FP64)])
+def test_full_size_grids_bitwise(shape, solver, steady, precision):
+    """The large configurations' own grids (480x480x60 steady, 960x960x120
+    transient): 10 iterations of the colour-split kernels (NC cells per
+    thread) bitwise the reference layout's."""
+    print("case %dx%dx%d solver" % shape, solver, "steady", steady, "precision", precision, flush=True)
+    a = run_limited(shape, L.LAYOUT_AUTO, solver, steady, precision, 10)
+    b = run_limited(shape, L.LAYOUT_STANDARD, solver, steady, precision, 10)
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
+
+
 @pytest.mark.parametrize("precision", [L.FP64, L.FP32_REFINED])
 def test_large_grid_two_colour_sweeps_bitwise(precision):
     """The two-colour sweep launches (chunks of 30 planes) with CG!, whose
