@@ -119,7 +119,7 @@ __device__ __forceinline__ RbRows rb_rows(const RbGeo<T> &g)
     RbRows r;
     r.live = L < nreal && m < g.HX;
     r.m = min(m, g.HX - 1);
-    r.j0 = (int)by * BY * R + threadIdx.y + 1;
+    r.j0 = (int)by * BY * R + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;  // wave-uniform
     r.k = (int)bz + 1;
     return r;
 }

@@ -29,7 +29,9 @@ __device__ __forceinline__ RbThread rb_thread(const RbGeo<T> &g)
     const unsigned bx = L % (unsigned)g.gx, t = L / (unsigned)g.gx;
     const unsigned bz = t % gz, by = t / gz;
     const int m = (int)bx * BX + threadIdx.x;
-    const int j = (int)by * BY + threadIdx.y + 1;
+    // a 64 x 4 workgroup is four waves of one row each: the row index is
+    // wave-uniform (scalar registers)
+    const int j = (int)by * BY + __builtin_amdgcn_readfirstlane((int)threadIdx.y) + 1;
     RbThread r;
     r.live = L < nreal && j <= g.NY && m < g.HX;
     r.m = min(m, g.HX - 1);
